@@ -1,0 +1,137 @@
+/*
+ * zhenmode.h -- C ABI of the MI355X mode-split step (libzhenmode_hip.so).
+ *
+ * The library replaces the compiled entry points that the reference builds in
+ * make_solver_global / _compile_solver
+ * (src/zhenmode/model/solver/factory.py:30-68, 721-799).  A handle holds one
+ * frozen parameter set (the reference's FDPhysParams, solver/state.py:46-187)
+ * on the GPU, exactly as the reference's jitted closures hold theirs.
+ *
+ * Layout: 3-D fields are (nz, ny, nx) with i (longitude) fastest, 2-D fields
+ * (ny, nx).  The reference stores (nx, ny, nz); the Python wrapper presents the
+ * same buffers to users as (nx, ny, nz) views, so no copy happens per step.
+ * All device pointers are in the compute dtype chosen at zm_create (float32 or
+ * float64).  Plain pointers and sizes only; no torch types cross the ABI.
+ *
+ * Errors: every call returns ZM_OK (0) or a nonzero code; zm_last_error gives
+ * the message (per handle, or per thread for a failed zm_create when the
+ * handle is NULL).  As in the reference, a step never reports a numerical
+ * failure: callers classify the state (runtime/monitor.py:20-30).
+ * Threading: one handle per stream; calls on a handle must be serialised.
+ */
+#ifndef ZHENMODE_H
+#define ZHENMODE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZM_OK 0
+#define ZM_EINVAL 1   /* invalid configuration or argument (ValueError) */
+#define ZM_EHIP 2     /* HIP runtime failure */
+#define ZM_ENOMEM 3   /* device allocation failure */
+
+typedef struct zm_handle zm_handle;
+
+/* Frozen parameters of the legacy mode-split step, derived on the host in
+ * float64 (fd_metrics.py:9-148, factory.py:397-695).  Array pointers are host
+ * pointers and are copied at zm_create. */
+typedef struct zm_config {
+  int32_t nx, ny, nz;
+  int32_t dtype_bytes;          /* 4 = float32, 8 = float64 */
+  double dt;                    /* baroclinic step [s] */
+  double dt_bt;                 /* barotropic substep dt / n_subcyc */
+  int32_t n_subcyc;             /* barotropic substeps (factory.py:540-542) */
+  int32_t n_nu;                 /* nu_h substeps per half-step (step.py:99) */
+  int32_t conv_nsub;            /* convective substeps (factory.py:565-566) */
+  int32_t adv_nsub;             /* must be 1 (factory.py:567) */
+  int32_t polar_cap_rows, polar_cap_taper;
+  double nu_h, nu_v, kappa_h, kappa_v, kappa_conv, nu_bi, kappa_bi;
+  double T_ref, S_ref, r_bot;
+  double lambda_bulk, restore_coef_S, ice_freeze_temp_c, ice_salt_flux;
+  double dy, H_sw, dz_surface, dz_bnd_top, dz_bnd_bot, d2z_h0_top, d2z_h0_bot;
+  const double* cos_lat;        /* [ny] */
+  const double* dx_row;         /* [ny] dx of each latitude row */
+  const double* f_row;          /* [ny] Coriolis parameter of each row */
+  const double* dz_node;        /* [nz] */
+  const double* dz_iface;       /* [nz-1] */
+  const double* dz;             /* [nz-1] */
+  const double* dz_norm;        /* [nz-1] */
+  const double* d2z_hm;         /* [nz-2] */
+  const double* d2z_hp;         /* [nz-2] */
+  const double* d2z_denom;      /* [nz-2] */
+  const double* dz_denom_interior; /* [nz-2] */
+  const double* cap_weights;    /* [polar_cap_rows + polar_cap_taper] */
+  const int32_t* nwet;          /* [ny*nx] wet levels per column */
+  const double* wet_mask;       /* [ny*nx] */
+  const double* tau_x;          /* [ny*nx] default forcing */
+  const double* tau_y;
+  const double* q_heat;
+  const double* t_atm;
+  const double* s_ref_surf;     /* SSS restoring target */
+  const double* coastal_restore_coef;
+  const double* coastal_restore_T;
+  const double* coastal_bulk_lambda;
+  const double* kappa_h_2d;     /* kappa_h + coastal kappa_h */
+  const double* kappa_v_2d;     /* kappa_v + coastal kappa_v */
+} zm_config;
+
+/* Prognostic state (reference JaxStateG(u, v, T, S, eta, ice),
+ * solver/state.py:10): device pointers. */
+typedef struct zm_state {
+  void* u;
+  void* v;
+  void* T;
+  void* S;
+  void* eta;
+  void* ice;
+} zm_state;
+
+/* Stage selectors for zm_stage. */
+#define ZM_STAGE_LINEAR 0      /* _linear_half_step, step.py:76-181 */
+#define ZM_STAGE_NONLINEAR 1   /* _explicit_full_step, step.py:249-320 */
+#define ZM_STAGE_BAROTROPIC 2  /* mode-split block, step.py:393-428 */
+#define ZM_STAGE_TRACER_RES 3  /* _compute_tracer_residual -> out.T, out.S */
+#define ZM_STAGE_MOMENTUM_RES 4 /* _compute_momentum_residual -> out.u, out.v */
+
+/* Replaces make_solver_global's parameter freezing and jit compilation
+ * (factory.py:30-718).  Returns ZM_EINVAL with a message for an invalid or
+ * unsupported configuration. */
+int zm_create(const zm_config* cfg, zm_handle** out);
+int zm_destroy(zm_handle* h);
+const char* zm_last_error(const zm_handle* h);
+int32_t zm_version(void);
+
+/* step(state) -> state (factory.py:724-726).  Reads *in, writes *out (which may
+ * alias *in); nsteps > 1 continues in place on *out. */
+int zm_step(zm_handle* h, const zm_state* in, const zm_state* out, int32_t nsteps, void* stream);
+
+/* step_dyn(state, tau_x, tau_y, q_heat, T_atm_3d) -> state (factory.py:735-750):
+ * one step with runtime 2-D forcing (device pointers, (ny, nx); t_atm may be
+ * NULL to keep the frozen field). */
+int zm_step_dyn(zm_handle* h, const zm_state* in, const zm_state* out, const void* tau_x,
+                const void* tau_y, const void* q_heat, const void* t_atm, void* stream);
+
+/* diagnostics(state) -> (rho, p, w) (factory.py:752-760); 3-D outputs. */
+int zm_diagnostics(zm_handle* h, const zm_state* in, void* rho, void* pressure, void* w, void* stream);
+
+/* One stage of the step (ZM_STAGE_*), for stage-level parity checks. */
+int zm_stage(zm_handle* h, int32_t stage, const zm_state* in, const zm_state* out, void* stream);
+
+/* Use hipGraph capture for zm_step/zm_step_dyn (default 1). */
+int zm_set_graph(zm_handle* h, int32_t enabled);
+
+/* Average duration [ms] of one launch of a step kernel, timed with HIP events
+ * around `reps` back-to-back launches on the handle's own stream.
+ * which: 0 = nu_h Laplacian substep, 1 = tracer residual, 2 = momentum
+ * residual, 3 = x de-alias, 4 = one barotropic substep (2 launches),
+ * 5 = whole step. */
+int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZHENMODE_H */

@@ -1,0 +1,100 @@
+"""ctypes binding of ``libzhenmode_hip.so`` (declarations in include/zhenmode.h).
+
+There is no CPU fallback: a missing or unloadable library raises immediately.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libzhenmode_hip.so"
+
+ZM_OK, ZM_EINVAL, ZM_EHIP, ZM_ENOMEM = 0, 1, 2, 3
+STAGE_LINEAR, STAGE_NONLINEAR, STAGE_BAROTROPIC, STAGE_TRACER_RES, STAGE_MOMENTUM_RES = range(5)
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+class ZmConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("dtype_bytes", ctypes.c_int32),
+        ("dt", ctypes.c_double), ("dt_bt", ctypes.c_double),
+        ("n_subcyc", ctypes.c_int32), ("n_nu", ctypes.c_int32),
+        ("conv_nsub", ctypes.c_int32), ("adv_nsub", ctypes.c_int32),
+        ("polar_cap_rows", ctypes.c_int32), ("polar_cap_taper", ctypes.c_int32),
+        ("nu_h", ctypes.c_double), ("nu_v", ctypes.c_double), ("kappa_h", ctypes.c_double),
+        ("kappa_v", ctypes.c_double), ("kappa_conv", ctypes.c_double), ("nu_bi", ctypes.c_double),
+        ("kappa_bi", ctypes.c_double), ("T_ref", ctypes.c_double), ("S_ref", ctypes.c_double),
+        ("r_bot", ctypes.c_double), ("lambda_bulk", ctypes.c_double),
+        ("restore_coef_S", ctypes.c_double), ("ice_freeze_temp_c", ctypes.c_double),
+        ("ice_salt_flux", ctypes.c_double),
+        ("dy", ctypes.c_double), ("H_sw", ctypes.c_double), ("dz_surface", ctypes.c_double),
+        ("dz_bnd_top", ctypes.c_double), ("dz_bnd_bot", ctypes.c_double),
+        ("d2z_h0_top", ctypes.c_double), ("d2z_h0_bot", ctypes.c_double),
+        ("cos_lat", _D), ("dx_row", _D), ("f_row", _D),
+        ("dz_node", _D), ("dz_iface", _D), ("dz", _D), ("dz_norm", _D),
+        ("d2z_hm", _D), ("d2z_hp", _D), ("d2z_denom", _D), ("dz_denom_interior", _D),
+        ("cap_weights", _D),
+        ("nwet", _I), ("wet_mask", _D),
+        ("tau_x", _D), ("tau_y", _D), ("q_heat", _D), ("t_atm", _D),
+        ("s_ref_surf", _D), ("coastal_restore_coef", _D), ("coastal_restore_T", _D),
+        ("coastal_bulk_lambda", _D), ("kappa_h_2d", _D), ("kappa_v_2d", _D),
+    ]
+
+
+class ZmState(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in ("u", "v", "T", "S", "eta", "ice")]
+
+
+EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", "zm_step_dyn",
+           "zm_diagnostics", "zm_stage", "zm_set_graph", "zm_time_kernel")
+
+_lib = None
+
+
+def load():
+    """Load the library once; raise if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m zhenmode_amd.build`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp = ctypes.c_void_p
+    lib.zm_create.argtypes = [ctypes.POINTER(ZmConfig), ctypes.POINTER(vp)]
+    lib.zm_create.restype = ctypes.c_int
+    lib.zm_destroy.argtypes = [vp]
+    lib.zm_destroy.restype = ctypes.c_int
+    lib.zm_last_error.argtypes = [vp]
+    lib.zm_last_error.restype = ctypes.c_char_p
+    lib.zm_version.argtypes = []
+    lib.zm_version.restype = ctypes.c_int32
+    lib.zm_step.argtypes = [vp, ctypes.POINTER(ZmState), ctypes.POINTER(ZmState), ctypes.c_int32, vp]
+    lib.zm_step.restype = ctypes.c_int
+    lib.zm_step_dyn.argtypes = [vp, ctypes.POINTER(ZmState), ctypes.POINTER(ZmState), vp, vp, vp, vp, vp]
+    lib.zm_step_dyn.restype = ctypes.c_int
+    lib.zm_diagnostics.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp, vp, vp]
+    lib.zm_diagnostics.restype = ctypes.c_int
+    lib.zm_stage.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ZmState), ctypes.POINTER(ZmState), vp]
+    lib.zm_stage.restype = ctypes.c_int
+    lib.zm_set_graph.argtypes = [vp, ctypes.c_int32]
+    lib.zm_set_graph.restype = ctypes.c_int
+    lib.zm_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.zm_time_kernel.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc, handle=None):
+    """Raise the reference's exception types for a nonzero status."""
+    if rc == ZM_OK:
+        return
+    msg = load().zm_last_error(handle).decode(errors="replace")
+    if rc == ZM_EINVAL:
+        raise ValueError(msg)
+    if rc == ZM_ENOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)

@@ -1,0 +1,96 @@
+// Shared device-side definitions for the mode-split step kernels.
+//
+// Layout: every 3-D field is stored i-innermost, index ((k * ny) + j) * nx + i,
+// with i the periodic longitude, j the latitude (0 = south) and k the level
+// (0 = surface).  2-D fields are (j * nx + i).  The wet mask is contiguous from
+// the surface in every column, so it is carried as nwet[j*nx+i], the number of
+// wet levels: level k is wet iff k < nwet.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zm {
+
+constexpr double kG = 9.81;
+constexpr double kRho0 = 1025.0;
+constexpr double kAlpha = 2.0e-4;
+constexpr double kBeta = 7.6e-4;
+constexpr double kCp = 3992.0;
+
+// Geometry: metric rows, vertical coefficients and the wet-level count.
+template <typename real>
+struct Geo {
+  int nx, ny, nz;
+  int n2;       // nx * ny
+  long long n3; // nx * ny * nz
+  // per latitude row [ny]
+  const real* cosl;   // cos(lat)
+  const real* idx;    // 1/dx
+  const real* idx2;   // 1/dx^2
+  const real* tanR;   // tan(lat)/R (from f of row 0)
+  const real* frow;   // f = 2 Omega sin(lat)
+  const real* cfn;    // 0.5 (cos_j + cos_{j+1 mod ny})
+  const real* cfs;    // 0.5 (cos_j + cos_{j-1 mod ny})
+  const real* idyc;   // (1/dy) / cos_j
+  real idy, idy2, hidy;   // 1/dy, 1/dy^2, 0.5/dy
+  // vertical
+  const real* dzn;    // node thickness [nz]
+  const real* idzn;   // 1/dz_node [nz]
+  const real* dzi;    // interface distance [nz-1]
+  const real* idzi;   // 1/dz_iface [nz-1]
+  const real* dz3;    // layer thickness dz [nz-1]
+  const real* dznorm; // dz/H_sw [nz-1]
+  const real* hm;     // |z_{k-1}-z_k| for interior k (index k-1) [nz-2]
+  const real* hp;     // |z_{k+1}-z_k| [nz-2]
+  const real* hmp;    // hm + hp [nz-2]
+  const real* iden;   // 1 / (hm hp (hm+hp)/2) [nz-2]
+  const real* iddz;   // 1 / ((z_{k+1}-z_k) + (z_k-z_{k-1})) [nz-2]
+  real idz_top, idz_bot;  // 1/(z1-z0), 1/(z_{nz-1}-z_{nz-2}) (negative)
+  real ih0top2, ih0bot2;  // 1/h0^2 at top and bottom
+  const int* nwet;        // wet levels per column [n2]
+};
+
+// Scalars and 2-D fields of the physics.
+template <typename real>
+struct Phys {
+  real dt, dth;                // baroclinic dt and dt/2
+  real nu_h, nu_v, kappa_h, nu_bi, kappa_bi, kappa_conv;
+  real T_ref, S_ref, r_bot;
+  real H_sw, dz_surface;
+  real wind_factor;            // 1 / (rho0 dz_surface)
+  real inv_rho0H;              // 1 / (rho0 H_sw)
+  real lambda_bulk, restore_coef_S, ice_freeze, ice_salt_flux;
+  real heat_factor;            // 1 / (dz_surface rho0 cp)
+  int conv_nsub;
+  int ncap, nb;                // polar cap rows and band width (ncap + taper)
+  const real* capw;            // pole-inward blend weights [nb]
+  const real* wet2;            // wet_mask [n2]
+  const real* kh2;             // kappa_h + coastal kappa_h [n2]
+  const real* kv2;             // kappa_v + coastal kappa_v [n2]
+  const real* tau_x;           // [n2]
+  const real* tau_y;
+  const real* q_heat;
+  const real* t_atm;
+  const real* s_ref2;          // SSS target [n2]
+  const real* crc;             // coastal restore coefficient [n2]
+  const real* crT;             // coastal restore target [n2]
+  const real* cbl;             // coastal bulk lambda [n2]
+  const real* rot_c;           // cos(f dt/2) per row
+  const real* rot_s;           // sin(f dt/2) per row
+};
+
+__device__ __forceinline__ int wrap_e(int i, int nx) { return i + 1 == nx ? 0 : i + 1; }
+__device__ __forceinline__ int wrap_w(int i, int nx) { return i == 0 ? nx - 1 : i - 1; }
+
+// Map a launch-order block id onto a tile id so that each of the 8 XCDs gets
+// one contiguous range of tiles (neighbouring tiles share halos through the
+// same L2).  Bijective for any block count.
+__device__ __forceinline__ long long xcd_tile(long long L, long long B) {
+  const long long nxcd = 8;
+  long long x = L % nxcd, q = L / nxcd;
+  long long per = B / nxcd, rem = B % nxcd;
+  return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
+}
+
+}  // namespace zm

@@ -1,0 +1,609 @@
+// Host orchestration of the mode-split step and the C ABI (include/zhenmode.h).
+//
+// One step (reference _step_impl, timestepping/step.py:341-458):
+//   L(dt/2) -> N(dt) -> L(dt/2) -> barotropic sub-cycle -> final masks,
+// issued as a fixed kernel sequence on one stream and, by default, replayed
+// from a hipGraph keyed by the state/forcing pointers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/zhenmode.h"
+#include "zm_common.h"
+#include "zm_kernels.h"
+
+namespace {
+
+thread_local std::string g_thread_error;
+
+struct HipError {
+  hipError_t code;
+  std::string where;
+};
+
+#define ZM_HIP(call)                                                  \
+  do {                                                                \
+    hipError_t e_ = (call);                                           \
+    if (e_ != hipSuccess) throw HipError{e_, std::string(#call)};     \
+  } while (0)
+
+struct ConfigError {
+  std::string msg;
+};
+
+using GraphKey = std::tuple<std::vector<const void*>, int>;
+
+struct SolverBase {
+  virtual ~SolverBase() = default;
+  virtual void step(const zm_state& in, const zm_state& out, int nsteps, const void* const* forcing,
+                    hipStream_t s) = 0;
+  virtual void diagnostics(const zm_state& in, void* rho, void* p, void* w, hipStream_t s) = 0;
+  virtual void stage(int which, const zm_state& in, const zm_state& out, hipStream_t s) = 0;
+  virtual double time_kernel(int which, int reps) = 0;
+  bool use_graph = true;
+};
+
+template <typename real>
+struct Solver : SolverBase {
+  zm::Geo<real> g{};
+  zm::Phys<real> P{};
+  int nx = 0, ny = 0, nz = 0;
+  long long n2 = 0, n3 = 0;
+  int n_sub = 0, n_nu = 0, N1 = 1, N2 = 1, keep = 0;
+  real dt_bt = 0, dtH = 0, drag = 0, decay_bot = 0, dts = 0;
+  std::vector<void*> allocs;
+  real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fden = nullptr;
+  // workspace
+  real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
+  real *Fz1, *Fz2, *cT, *cS;
+  int* gate;
+  real *dT1, *dS1, *Tp, *Sp, *du1, *dv1, *up, *vp, *T2, *S2, *u2, *v2, *pbc, *au, *av, *axu, *axv;
+  real *u3, *v3, *T3, *S3, *tmpA, *tmpB;
+  real *Frx, *Fry, *ubt0, *vbt0, *ubt, *vbt, *etaw, *means;
+  hipStream_t own = nullptr;
+  std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+
+  template <typename T>
+  T* dev_alloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) throw HipError{e, "hipMalloc"};
+    allocs.push_back(p);
+    ZM_HIP(hipMemset(p, 0, count * sizeof(T)));
+    return static_cast<T*>(p);
+  }
+
+  real* upload(const std::vector<double>& h) {
+    real* d = dev_alloc<real>(h.size());
+    std::vector<real> t(h.begin(), h.end());
+    if (!t.empty()) ZM_HIP(hipMemcpy(d, t.data(), t.size() * sizeof(real), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  real* upload2d(const double* h, double fill) {
+    std::vector<double> v(n2, fill);
+    if (h) std::copy(h, h + n2, v.begin());
+    return upload(v);
+  }
+
+  ~Solver() override {
+    for (auto& kv : graphs) {
+      hipGraphExecDestroy(kv.second.second);
+      hipGraphDestroy(kv.second.first);
+    }
+    if (own) hipStreamDestroy(own);
+    for (void* p : allocs) hipFree(p);
+  }
+
+  explicit Solver(const zm_config& c) {
+    nx = c.nx;
+    ny = c.ny;
+    nz = c.nz;
+    n2 = (long long)nx * ny;
+    n3 = n2 * nz;
+    n_sub = c.n_subcyc;
+    n_nu = c.n_nu;
+    const int ncap = c.polar_cap_rows > 0 ? c.polar_cap_rows : 0;
+    const int nb = ncap > 0 ? ncap + c.polar_cap_taper : 0;
+    // geometry rows
+    std::vector<double> cosl(ny), idx(ny), idx2(ny), tanR(ny), frow(ny), cfn(ny), cfs(ny), idyc(ny);
+    std::vector<double> rc(ny), rs(ny), fd(ny), fden_h(ny);
+    const double idy = 1.0 / c.dy;
+    for (int j = 0; j < ny; ++j) {
+      cosl[j] = c.cos_lat[j];
+      idx[j] = 1.0 / c.dx_row[j];
+      idx2[j] = 1.0 / (c.dx_row[j] * c.dx_row[j]);
+      const double sinl = c.f_row[j] / (2.0 * 7.2921e-5);
+      tanR[j] = (sinl / c.cos_lat[j]) / 6371.0e3;
+      frow[j] = c.f_row[j];
+      cfn[j] = 0.5 * (c.cos_lat[j] + c.cos_lat[(j + 1) % ny]);
+      cfs[j] = 0.5 * (c.cos_lat[j] + c.cos_lat[(j + ny - 1) % ny]);
+      idyc[j] = idy / c.cos_lat[j];
+      rc[j] = std::cos(c.f_row[j] * (c.dt / 2.0));
+      rs[j] = std::sin(c.f_row[j] * (c.dt / 2.0));
+      fd[j] = c.f_row[j] * c.dt_bt;
+      fden_h[j] = 1.0 + fd[j] * fd[j];
+    }
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.n2 = (int)n2;
+    g.n3 = n3;
+    g.cosl = upload(cosl);
+    g.idx = upload(idx);
+    g.idx2 = upload(idx2);
+    g.tanR = upload(tanR);
+    g.frow = upload(frow);
+    g.cfn = upload(cfn);
+    g.cfs = upload(cfs);
+    g.idyc = upload(idyc);
+    g.idy = (real)idy;
+    g.idy2 = (real)(idy * idy);
+    g.hidy = (real)(0.5 * idy);
+    std::vector<double> dzn(c.dz_node, c.dz_node + nz), idzn(nz);
+    for (int k = 0; k < nz; ++k) idzn[k] = 1.0 / dzn[k];
+    std::vector<double> dzi(c.dz_iface, c.dz_iface + nz - 1), idzi(nz - 1);
+    for (int k = 0; k + 1 < nz; ++k) idzi[k] = 1.0 / dzi[k];
+    std::vector<double> dz3(c.dz, c.dz + nz - 1), dznorm(c.dz_norm, c.dz_norm + nz - 1);
+    const int ni = nz > 2 ? nz - 2 : 0;
+    std::vector<double> hm(ni), hp(ni), hmp(ni), iden(ni), iddz(ni);
+    for (int k = 0; k < ni; ++k) {
+      hm[k] = c.d2z_hm[k];
+      hp[k] = c.d2z_hp[k];
+      hmp[k] = c.d2z_hm[k] + c.d2z_hp[k];
+      iden[k] = 1.0 / c.d2z_denom[k];
+      iddz[k] = 1.0 / c.dz_denom_interior[k];
+    }
+    g.dzn = upload(dzn);
+    g.idzn = upload(idzn);
+    g.dzi = upload(dzi);
+    g.idzi = upload(idzi);
+    g.dz3 = upload(dz3);
+    g.dznorm = upload(dznorm);
+    g.hm = upload(hm);
+    g.hp = upload(hp);
+    g.hmp = upload(hmp);
+    g.iden = upload(iden);
+    g.iddz = upload(iddz);
+    g.idz_top = (real)(1.0 / c.dz_bnd_top);
+    g.idz_bot = (real)(1.0 / c.dz_bnd_bot);
+    g.ih0top2 = (real)(1.0 / (c.d2z_h0_top * c.d2z_h0_top));
+    g.ih0bot2 = (real)(1.0 / (c.d2z_h0_bot * c.d2z_h0_bot));
+    int* nw = dev_alloc<int>(n2);
+    ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
+    g.nwet = nw;
+
+    P.dt = (real)c.dt;
+    P.dth = (real)(c.dt / 2.0);
+    P.nu_h = (real)c.nu_h;
+    P.nu_v = (real)c.nu_v;
+    P.kappa_h = (real)c.kappa_h;
+    P.nu_bi = (real)c.nu_bi;
+    P.kappa_bi = (real)c.kappa_bi;
+    P.kappa_conv = (real)c.kappa_conv;
+    P.T_ref = (real)c.T_ref;
+    P.S_ref = (real)c.S_ref;
+    P.r_bot = (real)c.r_bot;
+    P.H_sw = (real)c.H_sw;
+    P.dz_surface = (real)c.dz_surface;
+    P.wind_factor = (real)(1.0 / (1025.0 * c.dz_surface));
+    P.inv_rho0H = (real)(1.0 / (1025.0 * c.H_sw));
+    P.lambda_bulk = (real)c.lambda_bulk;
+    P.restore_coef_S = (real)c.restore_coef_S;
+    P.ice_freeze = (real)c.ice_freeze_temp_c;
+    P.ice_salt_flux = (real)c.ice_salt_flux;
+    P.heat_factor = (real)((1.0 / c.dz_surface) / (1025.0 * 3992.0));
+    P.conv_nsub = c.conv_nsub;
+    P.ncap = ncap;
+    P.nb = nb;
+    std::vector<double> capw(nb > 0 ? nb : 1, 0.0);
+    for (int q = 0; q < nb; ++q) capw[q] = c.cap_weights[q];
+    P.capw = upload(capw);
+    P.wet2 = upload2d(c.wet_mask, 0.0);
+    P.kh2 = upload2d(c.kappa_h_2d, c.kappa_h);
+    P.kv2 = upload2d(c.kappa_v_2d, c.kappa_v);
+    P.tau_x = upload2d(c.tau_x, 0.0);
+    P.tau_y = upload2d(c.tau_y, 0.0);
+    P.q_heat = upload2d(c.q_heat, 0.0);
+    P.t_atm = upload2d(c.t_atm, 0.0);
+    P.s_ref2 = upload2d(c.s_ref_surf, 0.0);
+    P.crc = upload2d(c.coastal_restore_coef, 0.0);
+    P.crT = upload2d(c.coastal_restore_T, 0.0);
+    P.cbl = upload2d(c.coastal_bulk_lambda, 0.0);
+    P.rot_c = upload(rc);
+    P.rot_s = upload(rs);
+    fdt = upload(fd);
+    fden = upload(fden_h);
+
+    dt_bt = (real)c.dt_bt;
+    dtH = (real)(c.dt_bt * c.H_sw);
+    drag = (real)(1.0 / (1.0 + c.r_bot * c.dt_bt));
+    decay_bot = (real)std::exp(-c.r_bot * c.dt);
+    dts = (real)((c.dt / 2.0) / n_nu);
+
+    // DFT factorisation nx = N1 * N2 with N1 the largest divisor <= sqrt(nx)
+    N1 = 1;
+    for (int d = 1; d * d <= nx; ++d)
+      if (nx % d == 0) N1 = d;
+    N2 = nx / N1;
+    keep = std::max(1, (int)((nx / 2) * 2.0 / 3.0));
+    std::vector<double> wr(nx), wi(nx);
+    for (int e = 0; e < nx; ++e) {
+      const double a = -2.0 * M_PI * (double)e / (double)nx;
+      wr[e] = std::cos(a);
+      wi[e] = std::sin(a);
+    }
+    twr = upload(wr);
+    twi = upload(wi);
+
+    real** f3[] = {&u1, &v1, &T1, &S1, &nAu, &nAv, &nBu, &nBv, &Lu, &Lv, &DT, &DS, &cT, &cS,
+                   &dT1, &dS1, &Tp, &Sp, &du1, &dv1, &up, &vp, &T2, &S2, &u2, &v2, &pbc,
+                   &au, &av, &axu, &axv, &u3, &v3, &T3, &S3, &tmpA, &tmpB};
+    for (real** p : f3) *p = dev_alloc<real>(n3);
+    Fz1 = dev_alloc<real>(n3 + n2);
+    Fz2 = dev_alloc<real>(n3 + n2);
+    gate = dev_alloc<int>(n2);
+    real** f2[] = {&Frx, &Fry, &ubt0, &vbt0, &ubt, &vbt, &etaw};
+    for (real** p : f2) *p = dev_alloc<real>(n2);
+    means = dev_alloc<real>((size_t)4 * nz * (2 * (nb > 0 ? nb : 1)));
+    ZM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    ZM_HIP(hipDeviceSynchronize());
+  }
+
+  // ---- stages -----------------------------------------------------------
+  void linear(const real* u0, const real* v0, const real* T0, const real* S0, real* uo, real* vo, real* To,
+              real* So, hipStream_t s) {
+    const bool bih_u = P.nu_bi > real(0), bih_t = P.kappa_bi > real(0);
+    if (bih_t) zm::launch_d1_unit(g, T0, S0, DT, DS, s);
+    const real* su = u0;
+    const real* sv = v0;
+    for (int n = 0; n < n_nu; ++n) {
+      real* du = (n % 2 == 0) ? nAu : nBu;
+      real* dv = (n % 2 == 0) ? nAv : nBv;
+      zm::launch_nu_substep(g, su, sv, du, dv, (n == 0 && bih_u) ? Lu : (real*)nullptr,
+                            (n == 0 && bih_u) ? Lv : (real*)nullptr, P.nu_h, dts, s);
+      su = du;
+      sv = dv;
+    }
+    zm::launch_linear_finish(g, P, u0, v0, T0, S0, su, sv, Lu, Lv, DT, DS, uo, vo, To, So, s);
+  }
+
+  void nonlinear(const zm::Phys<real>& Pf, const real* eta, real* uo, real* vo, real* To, real* So,
+                 hipStream_t s) {
+    // tracer residual #1 at the N-step start (u1, v1, T1, S1)
+    zm::launch_fz(g, u1, v1, Fz1, s);
+    zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, s);
+    zm::launch_tracer_residual(g, Pf, 0, u1, v1, T1, S1, Fz1, cT, cS, gate, dT1, dS1, T1, S1, Tp, Sp, s);
+    // momentum residual #1 at (u1, v1, T_pred, S_pred)
+    zm::launch_pbc(g, Pf, Tp, Sp, pbc, s);
+    zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
+                      (real*)nullptr, s);
+    zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
+    zm::launch_dealias_x(nx, N1, N2, keep, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+    zm::launch_mom_residual(g, Pf, 0, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, up, vp, decay_bot, s);
+    // tracer residual #2 at (u_pred, v_pred, T_pred, S_pred) -> T_new
+    zm::launch_fz(g, up, vp, Fz2, s);
+    zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, s);
+    zm::launch_tracer_residual(g, Pf, 1, up, vp, Tp, Sp, Fz2, cT, cS, gate, dT1, dS1, T1, S1, To, So, s);
+    // momentum residual #2 at (u1, v1, T_new, S_new); advection of (u1, v1) is reused
+    zm::launch_pbc(g, Pf, To, So, pbc, s);
+    zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
+                      (real*)nullptr, s);
+    zm::launch_mom_residual(g, Pf, 1, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, uo, vo, decay_bot, s);
+  }
+
+  void barotropic(const zm::Phys<real>& Pf, const real* eta_in, hipStream_t s) {
+    zm::launch_pbc(g, Pf, T3, S3, pbc, s);
+    zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, u3, v3, ubt0, vbt0, s);
+    zm::launch_copy(eta_in, etaw, n2, s);
+    zm::launch_copy((const real*)ubt0, ubt, n2, s);
+    zm::launch_copy((const real*)vbt0, vbt, n2, s);
+    for (int q = 0; q < n_sub; ++q) {
+      zm::launch_bt_eta(g, Pf, etaw, ubt, vbt, dtH, s);
+      zm::launch_bt_vel(g, Pf, etaw, ubt, vbt, Frx, Fry, fdt, fden, dt_bt, drag, s);
+    }
+  }
+
+  void one_step(const zm_state& in, const zm_state& out, const zm::Phys<real>& Pf, hipStream_t s) {
+    const real* ui = (const real*)in.u;
+    const real* vi = (const real*)in.v;
+    const real* Ti = (const real*)in.T;
+    const real* Si = (const real*)in.S;
+    const real* ei = (const real*)in.eta;
+    linear(ui, vi, Ti, Si, u1, v1, T1, S1, s);
+    nonlinear(Pf, ei, u2, v2, T2, S2, s);
+    linear(u2, v2, T2, S2, u3, v3, T3, S3, s);
+    barotropic(Pf, ei, s);
+    zm::launch_cap_means(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, s);
+    zm::launch_final(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, ui, vi, Ti, Si, (real*)out.u,
+                     (real*)out.v, (real*)out.T, (real*)out.S, s);
+    zm::launch_copy((const real*)etaw, (real*)out.eta, n2, s);
+    if (out.ice != in.ice) zm::launch_copy((const real*)in.ice, (real*)out.ice, n2, s);
+  }
+
+  zm::Phys<real> forced(const void* const* forcing) const {
+    zm::Phys<real> Pf = P;
+    if (forcing) {
+      if (forcing[0]) Pf.tau_x = (const real*)forcing[0];
+      if (forcing[1]) Pf.tau_y = (const real*)forcing[1];
+      if (forcing[2]) Pf.q_heat = (const real*)forcing[2];
+      if (forcing[3]) Pf.t_atm = (const real*)forcing[3];
+    }
+    return Pf;
+  }
+
+  void issue(const zm_state& in, const zm_state& out, int nsteps, const zm::Phys<real>& Pf, hipStream_t s) {
+    one_step(in, out, Pf, s);
+    for (int q = 1; q < nsteps; ++q) one_step(out, out, Pf, s);
+  }
+
+  void step(const zm_state& in, const zm_state& out, int nsteps, const void* const* forcing,
+            hipStream_t s) override {
+    const zm::Phys<real> Pf = forced(forcing);
+    if (!use_graph) {
+      issue(in, out, nsteps, Pf, s);
+      ZM_HIP(hipGetLastError());
+      return;
+    }
+    std::vector<const void*> key = {in.u, in.v, in.T, in.S, in.eta, in.ice,
+                                    out.u, out.v, out.T, out.S, out.eta, out.ice,
+                                    Pf.tau_x, Pf.tau_y, Pf.q_heat, Pf.t_atm};
+    GraphKey gk{key, nsteps};
+    auto it = graphs.find(gk);
+    if (it == graphs.end()) {
+      if (graphs.size() >= 8) {
+        auto first = graphs.begin();
+        ZM_HIP(hipStreamSynchronize(s));
+        hipGraphExecDestroy(first->second.second);
+        hipGraphDestroy(first->second.first);
+        graphs.erase(first);
+      }
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      ZM_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
+      issue(in, out, nsteps, Pf, own);
+      hipError_t e = hipStreamEndCapture(own, &graph);
+      if (e != hipSuccess) throw HipError{e, "hipStreamEndCapture"};
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        throw HipError{e, "hipGraphInstantiate"};
+      }
+      it = graphs.emplace(gk, std::make_pair(graph, exec)).first;
+    }
+    ZM_HIP(hipGraphLaunch(it->second.second, s));
+  }
+
+  void diagnostics(const zm_state& in, void* rho, void* pres, void* w, hipStream_t s) override {
+    zm::launch_pbc(g, P, (const real*)in.T, (const real*)in.S, pbc, s);
+    zm::launch_fz(g, (const real*)in.u, (const real*)in.v, Fz1, s);
+    zm::launch_diag(g, P, (const real*)in.T, (const real*)in.S, (const real*)in.eta, pbc, Fz1, (real*)rho,
+                    (real*)pres, (real*)w, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void stage(int which, const zm_state& in, const zm_state& out, hipStream_t s) override {
+    const real* ui = (const real*)in.u;
+    const real* vi = (const real*)in.v;
+    const real* Ti = (const real*)in.T;
+    const real* Si = (const real*)in.S;
+    const real* ei = (const real*)in.eta;
+    switch (which) {
+      case ZM_STAGE_LINEAR:
+        linear(ui, vi, Ti, Si, (real*)out.u, (real*)out.v, (real*)out.T, (real*)out.S, s);
+        break;
+      case ZM_STAGE_NONLINEAR:
+        zm::launch_copy(ui, u1, n3, s);
+        zm::launch_copy(vi, v1, n3, s);
+        zm::launch_copy(Ti, T1, n3, s);
+        zm::launch_copy(Si, S1, n3, s);
+        nonlinear(P, ei, (real*)out.u, (real*)out.v, (real*)out.T, (real*)out.S, s);
+        break;
+      case ZM_STAGE_BAROTROPIC:
+        zm::launch_copy(ui, u3, n3, s);
+        zm::launch_copy(vi, v3, n3, s);
+        zm::launch_copy(Ti, T3, n3, s);
+        zm::launch_copy(Si, S3, n3, s);
+        barotropic(P, ei, s);
+        zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, ui, vi, Ti, Si, (real*)out.u,
+                         (real*)out.v, (real*)out.T, (real*)out.S, s);
+        zm::launch_copy((const real*)etaw, (real*)out.eta, n2, s);
+        break;
+      case ZM_STAGE_TRACER_RES:
+        zm::launch_fz(g, ui, vi, Fz1, s);
+        zm::launch_conv(g, P, Ti, Si, cT, cS, gate, s);
+        zm::launch_tracer_residual(g, P, 0, ui, vi, Ti, Si, Fz1, cT, cS, gate, (real*)out.T, (real*)out.S, Ti,
+                                   Si, tmpA, tmpB, s);
+        break;
+      case ZM_STAGE_MOMENTUM_RES:
+        zm::launch_fz(g, ui, vi, Fz1, s);
+        zm::launch_pbc(g, P, Ti, Si, pbc, s);
+        zm::launch_bt_rho(g, P, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
+                          (real*)nullptr, s);
+        zm::launch_mom_adv(g, ui, vi, Fz1, au, av, s);
+        zm::launch_dealias_x(nx, N1, N2, keep, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+        zm::launch_mom_residual(g, P, 0, axu, axv, ui, vi, pbc, ei, Frx, Fry, (real*)out.u, (real*)out.v, tmpA,
+                                tmpB, decay_bot, s);
+        break;
+      default:
+        throw ConfigError{"unknown stage"};
+    }
+    ZM_HIP(hipGetLastError());
+  }
+
+  double time_kernel(int which, int reps) override {
+    hipEvent_t a, b;
+    ZM_HIP(hipEventCreate(&a));
+    ZM_HIP(hipEventCreate(&b));
+    hipStream_t s = own;
+    auto body = [&](int r) {
+      switch (which) {
+        case 0:
+          zm::launch_nu_substep(g, (r % 2) ? nBu : nAu, (r % 2) ? nBv : nAv, (r % 2) ? nAu : nBu,
+                                (r % 2) ? nAv : nBv, (real*)nullptr, (real*)nullptr, P.nu_h, dts, s);
+          break;
+        case 1:
+          zm::launch_tracer_residual(g, P, 0, u1, v1, T1, S1, Fz1, cT, cS, gate, dT1, dS1, T1, S1, Tp, Sp, s);
+          break;
+        case 2:
+          zm::launch_mom_residual(g, P, 0, axu, axv, u1, v1, pbc, etaw, Frx, Fry, du1, dv1, up, vp, decay_bot,
+                                  s);
+          break;
+        case 3:
+          zm::launch_dealias_x(nx, N1, N2, keep, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+          break;
+        case 4:
+          zm::launch_bt_eta(g, P, etaw, ubt, vbt, dtH, s);
+          zm::launch_bt_vel(g, P, etaw, ubt, vbt, Frx, Fry, fdt, fden, dt_bt, drag, s);
+          break;
+        default:
+          throw ConfigError{"unknown kernel selector"};
+      }
+    };
+    body(0);
+    ZM_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < reps; ++r) body(r);
+    ZM_HIP(hipEventRecord(b, s));
+    ZM_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    ZM_HIP(hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    ZM_HIP(hipGetLastError());
+    return (double)ms / reps;
+  }
+};
+
+void validate(const zm_config& c) {
+  auto bad = [](const std::string& m) { throw ConfigError{m}; };
+  if (c.nx < 2 || c.ny < 2 || c.nz < 2) bad("nx, ny, nz must be >= 2");
+  if (c.nx > 1024) bad("nx > 1024 is not supported by the row kernels");
+  if (c.dtype_bytes != 4 && c.dtype_bytes != 8) bad("dtype must be float32 or float64");
+  if (!(c.dt > 0) || !(c.dt_bt > 0)) bad("dt and dt_bt must be positive");
+  if (c.n_subcyc < 1 || c.n_nu < 1 || c.conv_nsub < 1) bad("substep counts must be >= 1");
+  if (c.adv_nsub != 1) bad("adv_nsub > 1 (vertical advection sub-cycling) is not supported");
+  if (c.polar_cap_rows < 0 || c.polar_cap_taper < 0) bad("polar cap rows must be >= 0");
+  if (c.polar_cap_rows > 0 && 2 * (c.polar_cap_rows + c.polar_cap_taper) > c.ny)
+    bad("polar cap bands must not overlap");
+  const void* req[] = {c.cos_lat, c.dx_row, c.f_row, c.dz_node, c.dz_iface, c.dz, c.dz_norm, c.nwet, c.wet_mask};
+  for (const void* p : req)
+    if (!p) bad("missing required array");
+  if (c.nz > 2 && (!c.d2z_hm || !c.d2z_hp || !c.d2z_denom || !c.dz_denom_interior)) bad("missing vertical arrays");
+  if (c.polar_cap_rows > 0 && !c.cap_weights) bad("missing polar cap weights");
+  const long long n2 = (long long)c.nx * c.ny;
+  for (long long q = 0; q < n2; ++q)
+    if (c.nwet[q] < 0 || c.nwet[q] > c.nz) bad("nwet out of range");
+  for (int j = 0; j < c.ny; ++j)
+    if (!(c.dx_row[j] > 0) || !(c.cos_lat[j] > 0)) bad("dx and cos(lat) must be positive");
+}
+
+int fail(zm_handle* h, const std::string& m, int code);
+
+}  // namespace
+
+struct zm_handle {
+  SolverBase* solver = nullptr;
+  int dtype_bytes = 0;
+  std::string err;
+};
+
+namespace {
+int fail(zm_handle* h, const std::string& m, int code) {
+  if (h)
+    h->err = m;
+  else
+    g_thread_error = m;
+  return code;
+}
+
+template <typename F>
+int guarded(zm_handle* h, F&& f) {
+  try {
+    f();
+    return ZM_OK;
+  } catch (const ConfigError& e) {
+    return fail(h, e.msg, ZM_EINVAL);
+  } catch (const HipError& e) {
+    return fail(h, e.where + ": " + hipGetErrorString(e.code),
+                e.code == hipErrorOutOfMemory ? ZM_ENOMEM : ZM_EHIP);
+  } catch (const std::exception& e) {
+    return fail(h, e.what(), ZM_EHIP);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int32_t zm_version(void) { return 1; }
+
+int zm_create(const zm_config* cfg, zm_handle** out) {
+  if (!cfg || !out) return fail(nullptr, "null argument", ZM_EINVAL);
+  *out = nullptr;
+  zm_handle* h = new zm_handle();
+  int rc = guarded(nullptr, [&] {
+    validate(*cfg);
+    if (cfg->dtype_bytes == 4)
+      h->solver = new Solver<float>(*cfg);
+    else
+      h->solver = new Solver<double>(*cfg);
+    h->dtype_bytes = cfg->dtype_bytes;
+  });
+  if (rc != ZM_OK) {
+    delete h->solver;
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return ZM_OK;
+}
+
+int zm_destroy(zm_handle* h) {
+  if (!h) return ZM_OK;
+  delete h->solver;
+  delete h;
+  return ZM_OK;
+}
+
+const char* zm_last_error(const zm_handle* h) { return h ? h->err.c_str() : g_thread_error.c_str(); }
+
+int zm_step(zm_handle* h, const zm_state* in, const zm_state* out, int32_t nsteps, void* stream) {
+  if (!h || !in || !out || nsteps < 1) return fail(h, "invalid zm_step arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->step(*in, *out, nsteps, nullptr, (hipStream_t)stream); });
+}
+
+int zm_step_dyn(zm_handle* h, const zm_state* in, const zm_state* out, const void* tau_x, const void* tau_y,
+                const void* q_heat, const void* t_atm, void* stream) {
+  if (!h || !in || !out || !tau_x || !tau_y || !q_heat) return fail(h, "invalid zm_step_dyn arguments", ZM_EINVAL);
+  const void* forcing[4] = {tau_x, tau_y, q_heat, t_atm};
+  return guarded(h, [&] { h->solver->step(*in, *out, 1, forcing, (hipStream_t)stream); });
+}
+
+int zm_diagnostics(zm_handle* h, const zm_state* in, void* rho, void* pressure, void* w, void* stream) {
+  if (!h || !in || !rho || !pressure || !w) return fail(h, "invalid zm_diagnostics arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->diagnostics(*in, rho, pressure, w, (hipStream_t)stream); });
+}
+
+int zm_stage(zm_handle* h, int32_t which, const zm_state* in, const zm_state* out, void* stream) {
+  if (!h || !in || !out) return fail(h, "invalid zm_stage arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->stage(which, *in, *out, (hipStream_t)stream); });
+}
+
+int zm_set_graph(zm_handle* h, int32_t enabled) {
+  if (!h) return fail(h, "null handle", ZM_EINVAL);
+  h->solver->use_graph = enabled != 0;
+  return ZM_OK;
+}
+
+int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch) {
+  if (!h || reps < 1 || !ms_per_launch) return fail(h, "invalid zm_time_kernel arguments", ZM_EINVAL);
+  return guarded(h, [&] { *ms_per_launch = h->solver->time_kernel(which, reps); });
+}
+
+}  // extern "C"

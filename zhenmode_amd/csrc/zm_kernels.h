@@ -1,0 +1,68 @@
+// Host-side launchers for the mode-split kernels (explicitly instantiated for
+// float and double in zm_kernels.hip).
+#pragma once
+
+#include "zm_common.h"
+
+namespace zm {
+
+template <typename real>
+void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,
+                       real* lv, real nu_h, real dts, hipStream_t s);
+template <typename real>
+void launch_d1_unit(const Geo<real>& g, const real* T, const real* S, real* dT, real* dS, hipStream_t s);
+template <typename real>
+void launch_linear_finish(const Geo<real>& g, const Phys<real>& P, const real* u0, const real* v0,
+                          const real* T0, const real* S0, const real* unu, const real* vnu, const real* Lu0,
+                          const real* Lv0, const real* DT0, const real* DS0, real* uo, real* vo, real* To,
+                          real* So, hipStream_t s);
+template <typename real>
+void launch_fz(const Geo<real>& g, const real* u, const real* v, real* Fz, hipStream_t s);
+template <typename real>
+void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
+                 int* gate, hipStream_t s);
+template <typename real>
+void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* u, const real* v,
+                            const real* T, const real* S, const real* Fz, const real* cT, const real* cS,
+                            const int* gate, real* dT1, real* dS1, const real* Tb, const real* Sb, real* To,
+                            real* So, hipStream_t s);
+template <typename real>
+void launch_pbc(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* pbc,
+                hipStream_t s);
+template <typename real>
+void launch_bt_rho(const Geo<real>& g, const Phys<real>& P, const real* pbc, real* Fx, real* Fy, const real* u,
+                   const real* v, real* ubt0, real* vbt0, hipStream_t s);
+template <typename real>
+void launch_mom_adv(const Geo<real>& g, const real* u, const real* v, const real* Fz, real* au, real* av,
+                    hipStream_t s);
+template <typename real>
+void launch_dealias_x(int N, int N1, int N2, int keep, int nrows, const real* twr, const real* twi,
+                      const real* ina, const real* inb, real* outa, real* outb, hipStream_t s);
+template <typename real>
+void launch_mom_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* ax, const real* ay,
+                         const real* u, const real* v, const real* pbc, const real* eta, const real* Frx,
+                         const real* Fry, real* du1, real* dv1, real* uo, real* vo, real decay_bot,
+                         hipStream_t s);
+template <typename real>
+void launch_bt_eta(const Geo<real>& g, const Phys<real>& P, real* eta, const real* ubt, const real* vbt,
+                   real dtH, hipStream_t s);
+template <typename real>
+void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, real* ubt, real* vbt,
+                   const real* Frx, const real* Fry, const real* fdt, const real* fden, real dt_bt, real drag,
+                   hipStream_t s);
+template <typename real>
+void launch_cap_means(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
+                      const real* S, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
+                      real* means, hipStream_t s);
+template <typename real>
+void launch_final(const Geo<real>& g, const Phys<real>& P, const real* u3, const real* v3, const real* T3,
+                  const real* S3, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
+                  const real* means, const real* uin, const real* vin, const real* Tin, const real* Sin,
+                  real* uo, real* vo, real* To, real* So, hipStream_t s);
+template <typename real>
+void launch_diag(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, const real* eta,
+                 const real* pbc, const real* Fz, real* rho, real* pres, real* w, hipStream_t s);
+template <typename real>
+void launch_copy(const real* a, real* b, long long n, hipStream_t s);
+
+}  // namespace zm

@@ -1,0 +1,1151 @@
+// Mode-split step kernels for gfx950 (MI355X), templated on the compute type.
+//
+// Each kernel restates one stage of the reference's legacy mode-split step
+// (src/zhenmode/model/solver/timestepping/step.py:341-458); the reference
+// function it follows is named above each kernel.  All face gates of the
+// reference (wet * neighbour-wet products) are evaluated as selects on the
+// wet-level count, which equals the reference's 0/1 products for finite data.
+#include "zm_common.h"
+#include "zm_kernels.h"
+
+namespace zm {
+
+// ---------------------------------------------------------------------------
+// neighbourhood of a column (i, j)
+// ---------------------------------------------------------------------------
+struct Nb {
+  int i, j, c2;
+  int e2, w2;       // east / west (periodic)
+  int n2e, s2e;     // north / south, edge-clamped (mirror ghost)
+  bool hasN, hasS;  // j < ny-1, j > 0
+  int nwC, nwE, nwW, nwN, nwS;  // wet-level counts (N/S edge-clamped)
+};
+
+template <typename real>
+__device__ __forceinline__ Nb make_nb(const Geo<real>& g, int i, int j) {
+  Nb n;
+  n.i = i;
+  n.j = j;
+  n.c2 = j * g.nx + i;
+  n.e2 = j * g.nx + wrap_e(i, g.nx);
+  n.w2 = j * g.nx + wrap_w(i, g.nx);
+  n.hasN = j + 1 < g.ny;
+  n.hasS = j > 0;
+  n.n2e = n.hasN ? n.c2 + g.nx : n.c2;
+  n.s2e = n.hasS ? n.c2 - g.nx : n.c2;
+  n.nwC = g.nwet[n.c2];
+  n.nwE = g.nwet[n.e2];
+  n.nwW = g.nwet[n.w2];
+  n.nwN = g.nwet[n.n2e];
+  n.nwS = g.nwet[n.s2e];
+  return n;
+}
+
+// _laplacian_h legacy branch (numerics/horizontal.py:181-215).
+template <typename real>
+__device__ __forceinline__ real lap_h(const real* __restrict__ f, long long plane, const Nb& n,
+                                      int k, const Geo<real>& g) {
+  const real u = f[plane + n.c2];
+  const real ue = f[plane + n.e2], uw = f[plane + n.w2];
+  const real un = f[plane + n.n2e], us = f[plane + n.s2e];
+  const bool wc = k < n.nwC;
+  const real fe = (wc && k < n.nwE) ? (ue - u) : real(0);
+  const real fw = (wc && k < n.nwW) ? (u - uw) : real(0);
+  const real fn = (wc && k < n.nwN) ? (un - u) : real(0);
+  const real fs = (wc && k < n.nwS) ? (u - us) : real(0);
+  const real d2x = (fe - fw) * g.idx2[n.j];
+  const real d2y = (fn - fs) * g.idy2;
+  const real dudy = (un - us) * g.hidy;
+  return d2x + d2y + (-dudy * g.tanR[n.j]);
+}
+
+// _horizontal_diffusion_flux (numerics/horizontal.py:217-238); kap == nullptr
+// means the unit coefficient of _horizontal_biharmonic_tracer.
+template <typename real>
+__device__ __forceinline__ real d1_flux(const real* __restrict__ t, long long plane, const Nb& n,
+                                        int k, const Geo<real>& g, const real* __restrict__ kap) {
+  const real c = t[plane + n.c2];
+  const bool wc = k < n.nwC;
+  const real kc = kap ? kap[n.c2] : real(1);
+  real zE = 0, zW = 0, mN = 0, mS = 0;
+  if (wc && k < n.nwE) {
+    const real ke = kap ? kap[n.e2] : real(1);
+    zE = (real(0.5) * (kc + ke)) * (t[plane + n.e2] - c) * g.idx[n.j];
+  }
+  if (wc && k < n.nwW) {
+    const real kw = kap ? kap[n.w2] : real(1);
+    zW = (real(0.5) * (kw + kc)) * (c - t[plane + n.w2]) * g.idx[n.j];
+  }
+  if (n.hasN && wc && k < n.nwN) {
+    const real kn = kap ? kap[n.n2e] : real(1);
+    mN = (real(0.5) * (kc + kn)) * (t[plane + n.n2e] - c) * g.idy * g.cfn[n.j];
+  }
+  if (n.hasS && wc && k < n.nwS) {
+    const real ks = kap ? kap[n.s2e] : real(1);
+    mS = (real(0.5) * (ks + kc)) * (c - t[plane + n.s2e]) * g.idy * g.cfn[n.j - 1];
+  }
+  return (zE - zW) * g.idx[n.j] + (mN - mS) * g.idyc[n.j];
+}
+
+// _d2_dz2 with _fill_ghost_bottom (numerics/vertical.py:122-135, 275-293).
+template <typename real>
+__device__ __forceinline__ real d2dz2(const real* __restrict__ f, int c2, int n2, int k, int nw,
+                                      const Geo<real>& g) {
+  const int kb = nw > 0 ? nw - 1 : 0;
+  auto val = [&](int kk) { return f[(long long)(kk < kb ? kk : kb) * n2 + c2]; };
+  if (k == 0) return real(2) * (val(1) - val(0)) * g.ih0top2;
+  if (k == g.nz - 1) return real(2) * (val(g.nz - 2) - val(g.nz - 1)) * g.ih0bot2;
+  return (val(k + 1) * g.hm[k - 1] + val(k - 1) * g.hp[k - 1] - val(k) * g.hmp[k - 1]) * g.iden[k - 1];
+}
+
+// _gradient_conservative_3d (numerics/horizontal.py:126-151) of a field given
+// at the five points (per-level face gates, wrapped y neighbour closed at walls).
+template <typename real>
+__device__ __forceinline__ void grad_cons(real pc, real pe, real pw, real pn, real ps, const Nb& n,
+                                          int k, const Geo<real>& g, real& gx, real& gy) {
+  const bool wc = k < n.nwC;
+  const real dxp = (wc && k < n.nwE) ? (pe - pc) : real(0);
+  const real dxm = (wc && k < n.nwW) ? (pc - pw) : real(0);
+  gx = g.idx[n.j] * real(0.5) * (dxp + dxm);
+  const real dyp = (n.hasN && wc && k < n.nwN) ? (pn - pc) * g.cfn[n.j] : real(0);
+  const real dym = (n.hasS && wc && k < n.nwS) ? (pc - ps) * g.cfs[n.j] : real(0);
+  gy = g.idyc[n.j] * real(0.5) * (dyp + dym);
+}
+
+__device__ __forceinline__ void cell_of(long long c, int n2, int nx, int& i, int& j, int& k) {
+  k = (int)(c / n2);
+  const int r = (int)(c - (long long)k * n2);
+  j = r / nx;
+  i = r - j * nx;
+}
+
+#define ZM_CELL_PROLOGUE                                                     \
+  const long long c = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x; \
+  if (c >= g.n3) return;                                                     \
+  int i, j, k;                                                               \
+  cell_of(c, g.n2, g.nx, i, j, k);                                           \
+  const Nb n = make_nb(g, i, j);                                             \
+  const long long plane = (long long)k * g.n2;
+
+// ---------------------------------------------------------------------------
+// nu_h sub-cycle: one explicit Laplacian substep of u and v
+// (timestepping/step.py:94-107).  Optionally also stores lap(u), lap(v).
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(256) void k_nu_substep(Geo<real> g, const real* __restrict__ u,
+                                                    const real* __restrict__ v, real* __restrict__ uo,
+                                                    real* __restrict__ vo, real* __restrict__ lu,
+                                                    real* __restrict__ lv, real nu_h, real dts) {
+  ZM_CELL_PROLOGUE
+  const real Lu = lap_h(u, plane, n, k, g);
+  const real Lv = lap_h(v, plane, n, k, g);
+  uo[c] = u[c] + nu_h * Lu * dts;
+  vo[c] = v[c] + nu_h * Lv * dts;
+  if (lu) {
+    lu[c] = Lu;
+    lv[c] = Lv;
+  }
+}
+
+// Unit-coefficient D1 of T and S (first half of _horizontal_biharmonic_tracer).
+template <typename real>
+__global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __restrict__ T,
+                                                 const real* __restrict__ S, real* __restrict__ dT,
+                                                 real* __restrict__ dS) {
+  ZM_CELL_PROLOGUE
+  dT[c] = d1_flux(T, plane, n, k, g, (const real*)nullptr);
+  dS[c] = d1_flux(S, plane, n, k, g, (const real*)nullptr);
+}
+
+// Remainder of _linear_half_step after the nu_h sub-cycle (step.py:111-173):
+// tracer D1, biharmonics, vertical diffusion, land/ghost hold, Coriolis rotation.
+template <typename real>
+__global__ __launch_bounds__(256) void k_linear_finish(
+    Geo<real> g, Phys<real> P, const real* __restrict__ u0, const real* __restrict__ v0,
+    const real* __restrict__ T0, const real* __restrict__ S0, const real* __restrict__ unu,
+    const real* __restrict__ vnu, const real* __restrict__ Lu0, const real* __restrict__ Lv0,
+    const real* __restrict__ DT0, const real* __restrict__ DS0, real* __restrict__ uo,
+    real* __restrict__ vo, real* __restrict__ To, real* __restrict__ So) {
+  ZM_CELL_PROLOGUE
+  const real dth = P.dth;
+  real u = unu[c], v = vnu[c];
+  if (P.nu_bi > real(0)) {
+    u = u - P.nu_bi * lap_h(Lu0, plane, n, k, g) * dth;
+    v = v - P.nu_bi * lap_h(Lv0, plane, n, k, g) * dth;
+  }
+  u = u + P.nu_v * d2dz2(u0, n.c2, g.n2, k, n.nwC, g) * dth;
+  v = v + P.nu_v * d2dz2(v0, n.c2, g.n2, k, n.nwC, g) * dth;
+  const real t0 = T0[c], s0 = S0[c];
+  real T = t0 + d1_flux(T0, plane, n, k, g, P.kh2) * dth;
+  real S = s0 + d1_flux(S0, plane, n, k, g, P.kh2) * dth;
+  if (P.kappa_bi > real(0)) {
+    T = T - P.kappa_bi * d1_flux(DT0, plane, n, k, g, (const real*)nullptr) * dth;
+    S = S - P.kappa_bi * d1_flux(DS0, plane, n, k, g, (const real*)nullptr) * dth;
+  }
+  const real kv = P.kv2[n.c2];
+  T = T + kv * d2dz2(T0, n.c2, g.n2, k, n.nwC, g) * dth;
+  S = S + kv * d2dz2(S0, n.c2, g.n2, k, n.nwC, g) * dth;
+  const bool wc = k < n.nwC;
+  if (!wc) {
+    u = u0[c];
+    v = v0[c];
+    T = t0;
+    S = s0;
+  }
+  const real cr = P.rot_c[j], sr = P.rot_s[j];
+  uo[c] = cr * u + sr * v;
+  vo[c] = -sr * u + cr * v;
+  To[c] = T;
+  So[c] = S;
+}
+
+// ---------------------------------------------------------------------------
+// Fz: _divergence_h (horizontal.py:255-281) accumulated from the seafloor up
+// (_vertical_transport_iface, transport.py:110-133).  Fz has nz+1 levels.
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ real div_h(const real* __restrict__ u, const real* __restrict__ v,
+                                      long long plane, const Nb& n, int k, const Geo<real>& g) {
+  const real uc = u[plane + n.c2];
+  const bool wc = k < n.nwC;
+  const real fxe = (wc && k < n.nwE) ? real(0.5) * (uc + u[plane + n.e2]) : real(0);
+  const real fxw = (wc && k < n.nwW) ? real(0.5) * (u[plane + n.w2] + uc) : real(0);
+  const real vc = v[plane + n.c2];
+  const real fyn = (n.hasN && wc && k < n.nwN) ? real(0.5) * (vc + v[plane + n.n2e]) * g.cfn[n.j] : real(0);
+  const real fys = (n.hasS && wc && k < n.nwS) ? real(0.5) * (v[plane + n.s2e] + vc) * g.cfn[n.j - 1] : real(0);
+  return (fxe - fxw) * g.idx[n.j] + (fyn - fys) * g.idyc[n.j];
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_fz(Geo<real> g, const real* __restrict__ u,
+                                            const real* __restrict__ v, real* __restrict__ Fz) {
+  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  if (c2 >= g.n2) return;
+  const int j = c2 / g.nx, i = c2 - j * g.nx;
+  const Nb n = make_nb(g, i, j);
+  real acc = 0;
+  Fz[(long long)g.nz * g.n2 + c2] = real(0);
+  for (int k = g.nz - 1; k >= 0; --k) {
+    const long long plane = (long long)k * g.n2;
+    acc = acc + div_h(u, v, plane, n, k, g) * g.dzn[k];
+    Fz[plane + c2] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Convective mixing: column gate (_convective_mask, physics/vertical.py:63-82)
+// and the legacy sub-cycle of _conv_flux_tendency (tendencies.py:175-191,
+// physics/vertical.py:13-37).  One column per lane; the working column sits in
+// LDS as [k][lane].  conv buffers are written only where the gate is set.
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ real rho_anom(real T, real S, const Phys<real>& P) {
+  return real(kRho0) * (real(-kAlpha) * (T - P.T_ref) + real(kBeta) * (S - P.S_ref));
+}
+
+template <typename real>
+__global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const real* __restrict__ T,
+                                             const real* __restrict__ S, real* __restrict__ cT,
+                                             real* __restrict__ cS, int* __restrict__ gate) {
+  extern __shared__ unsigned char zm_lds[];
+  real* tT = reinterpret_cast<real*>(zm_lds);
+  real* tS = tT + (size_t)g.nz * 64;
+  const int lane = threadIdx.x;
+  const int c2 = blockIdx.x * 64 + lane;
+  if (c2 >= g.n2) return;
+  const int nw = g.nwet[c2];
+  const int n2 = g.n2;
+  bool unstable = false;
+  if (nw >= 2) {
+    real rprev = rho_anom(T[c2], S[c2], P);
+    for (int k = 1; k < nw; ++k) {
+      const real r = rho_anom(T[(long long)k * n2 + c2], S[(long long)k * n2 + c2], P);
+      unstable |= (rprev > r);
+      rprev = r;
+    }
+  }
+  gate[c2] = unstable ? 1 : 0;
+  if (!unstable || P.kappa_conv <= real(0)) {
+    gate[c2] = 0;
+    return;
+  }
+  for (int k = 0; k < g.nz; ++k) {
+    tT[k * 64 + lane] = T[(long long)k * n2 + c2];
+    tS[k * 64 + lane] = S[(long long)k * n2 + c2];
+  }
+  const int nc = P.conv_nsub;
+  const real kap = nc > 1 ? P.kappa_conv / real(nc) : P.kappa_conv;
+  const real h = P.dt / real(nc);
+  for (int s = 0; s < nc; ++s) {
+    real fTprev = 0, fSprev = 0;
+    for (int k = 0; k < g.nz; ++k) {
+      real fT = 0, fS = 0;
+      const real ot = tT[k * 64 + lane], os = tS[k * 64 + lane];
+      if (k + 1 < nw) {
+        fT = (-kap * (tT[(k + 1) * 64 + lane] - ot)) * g.idzi[k];
+        fS = (-kap * (tS[(k + 1) * 64 + lane] - os)) * g.idzi[k];
+      }
+      const real et = k < nw ? (fTprev - fT) * g.idzn[k] : real(0);
+      const real es = k < nw ? (fSprev - fS) * g.idzn[k] : real(0);
+      tT[k * 64 + lane] = ot + et * h;
+      tS[k * 64 + lane] = os + es * h;
+      const long long o = (long long)k * n2 + c2;
+      if (s == 0) {
+        cT[o] = et;
+        cS[o] = es;
+      } else {
+        cT[o] = cT[o] + et;
+        cS[o] = cS[o] + es;
+      }
+      fTprev = fT;
+      fSprev = fS;
+    }
+  }
+  if (nc > 1) {
+    for (int k = 0; k < g.nz; ++k) {
+      const long long o = (long long)k * n2 + c2;
+      cT[o] = cT[o] / real(nc);
+      cS[o] = cS[o] / real(nc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tracer residual (tendencies.py:120-271, legacy branches) at one node, fused
+// with the RK2 update of _explicit_full_step (step.py:258-307).
+//   mode 0: dT1 -> dT1buf, T_pred = T + dT1 dt
+//   mode 1: T_new = Tbase + 0.5 (dT1 + dT2) dt
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ real adv_scalar(const real* __restrict__ X, const real* __restrict__ u,
+                                           const real* __restrict__ v, const real* __restrict__ Fz,
+                                           long long plane, const Nb& n, int k, const Geo<real>& g) {
+  // _legacy_horizontal_tracer_fluxes (transport.py:224-290) + donor-cell
+  // vertical flux (transport.py:444-475).
+  const bool wc = k < n.nwC;
+  if (!wc) return real(0);
+  const real xc = X[plane + n.c2], uc = u[plane + n.c2], vc = v[plane + n.c2];
+  real fxe = 0, fxw = 0, fyn = 0, fys = 0;
+  if (k < n.nwE) fxe = (real(0.5) * (uc + u[plane + n.e2])) * (real(0.5) * (xc + X[plane + n.e2]));
+  if (k < n.nwW) fxw = (real(0.5) * (u[plane + n.w2] + uc)) * (real(0.5) * (X[plane + n.w2] + xc));
+  if (n.hasN && k < n.nwN)
+    fyn = (real(0.5) * (vc + v[plane + n.n2e])) * (real(0.5) * (xc + X[plane + n.n2e])) * g.cfn[n.j];
+  if (n.hasS && k < n.nwS)
+    fys = (real(0.5) * (v[plane + n.s2e] + vc)) * (real(0.5) * (X[plane + n.s2e] + xc)) * g.cfn[n.j - 1];
+  const real hdiv = (fxe - fxw) * g.idx[n.j] + (fyn - fys) * g.idyc[n.j];
+  real up, dn = 0;
+  const real fzk = Fz[plane + n.c2];
+  if (k == 0) {
+    up = fzk * xc;
+  } else {
+    // both k-1 and k are wet here (k < nwC)
+    up = fzk * (fzk > real(0) ? X[plane - g.n2 + n.c2] : xc);
+  }
+  if (k + 1 < n.nwC) {
+    const real fzn = Fz[plane + g.n2 + n.c2];
+    dn = fzn * (fzn > real(0) ? xc : X[plane + g.n2 + n.c2]);
+  }
+  const real divz = (dn - up) * g.idzn[k];
+  return -(hdiv + divz);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_tracer_residual(
+    Geo<real> g, Phys<real> P, int mode, const real* __restrict__ u, const real* __restrict__ v,
+    const real* __restrict__ T, const real* __restrict__ S, const real* __restrict__ Fz,
+    const real* __restrict__ cT, const real* __restrict__ cS, const int* __restrict__ gate,
+    real* __restrict__ dT1, real* __restrict__ dS1, const real* __restrict__ Tb,
+    const real* __restrict__ Sb, real* __restrict__ To, real* __restrict__ So) {
+  ZM_CELL_PROLOGUE
+  const bool wc = k < n.nwC;
+  const real advT = adv_scalar(T, u, v, Fz, plane, n, k, g);
+  const real advS = adv_scalar(S, u, v, Fz, plane, n, k, g);
+  const real dhT = d1_flux(T, plane, n, k, g, P.kh2);
+  const real dhS = d1_flux(S, plane, n, k, g, P.kh2);
+  const real kv = P.kv2[n.c2];
+  const real dvT = kv * d2dz2(T, n.c2, g.n2, k, n.nwC, g);
+  const real dvS = kv * d2dz2(S, n.c2, g.n2, k, n.nwC, g);
+  const bool gated = gate[n.c2] != 0;
+  const real convT = gated ? cT[c] : real(0);
+  const real convS = gated ? cS[c] : real(0);
+  real dTdt = 0, dSdt = 0;
+  if (wc) {
+    dTdt = advT + dhT + dvT;
+    dSdt = advS + dhS + dvS;
+    if (k == 0) {
+      const real sst = T[n.c2], sss = S[n.c2];
+      const real hf = P.heat_factor;
+      const real ta = P.t_atm[n.c2];
+      dTdt = dTdt + P.q_heat[n.c2] * hf;
+      dTdt = dTdt + P.lambda_bulk * (ta - sst) * hf;
+      dTdt = dTdt + P.cbl[n.c2] * (ta - sst) * hf;
+      dTdt = dTdt + convT;
+      dTdt = dTdt + P.crc[n.c2] * (P.crT[n.c2] - sst);
+      dSdt = dSdt + convS;
+      dSdt = dSdt + P.restore_coef_S * (P.s_ref2[n.c2] - sss);
+      dSdt = dSdt + P.ice_salt_flux * (sst <= P.ice_freeze ? real(1) : real(0));
+    } else {
+      dTdt = dTdt + convT;
+      dSdt = dSdt + convS;
+    }
+  }
+  dTdt = dTdt - dhT;
+  dSdt = dSdt - dhS;
+  dTdt = dTdt - dvT;
+  dSdt = dSdt - dvS;
+  if (mode == 0) {
+    dT1[c] = dTdt;
+    dS1[c] = dSdt;
+    To[c] = T[c] + dTdt * P.dt;
+    So[c] = S[c] + dSdt * P.dt;
+  } else {
+    To[c] = Tb[c] + real(0.5) * (dT1[c] + dTdt) * P.dt;
+    So[c] = Sb[c] + real(0.5) * (dS1[c] + dSdt) * P.dt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Baroclinic pressure: trapezoid prefix sum of the masked density anomaly
+// from the surface down (_baroclinic_pressure, dynamics/pressure.py:32-42).
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(256) void k_pbc(Geo<real> g, Phys<real> P, const real* __restrict__ T,
+                                             const real* __restrict__ S, real* __restrict__ pbc) {
+  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  if (c2 >= g.n2) return;
+  const int nw = g.nwet[c2];
+  real rprev = 0 < nw ? rho_anom(T[c2], S[c2], P) : real(0);
+  real acc = 0;
+  pbc[c2] = real(0);
+  for (int k = 1; k < g.nz; ++k) {
+    const long long o = (long long)k * g.n2 + c2;
+    const real r = k < nw ? rho_anom(T[o], S[o], P) : real(0);
+    acc = acc + real(kG) * (real(0.5) * (rprev + r)) * g.dz3[k - 1];
+    pbc[o] = acc;
+    rprev = r;
+  }
+}
+
+// Depth-weighted baroclinic pressure gradient F_rho (_compute_bt_rho_pgf legacy,
+// pressure.py:170-180); with ubt0 != nullptr also the legacy depth mean
+// velocity (_barotropic_velocity, transport.py:135-143).
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_rho(Geo<real> g, Phys<real> P, const real* __restrict__ pbc,
+                                                real* __restrict__ Fx, real* __restrict__ Fy,
+                                                const real* __restrict__ u, const real* __restrict__ v,
+                                                real* __restrict__ ubt0, real* __restrict__ vbt0) {
+  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  if (c2 >= g.n2) return;
+  const int j = c2 / g.nx, i = c2 - j * g.nx;
+  const Nb n = make_nb(g, i, j);
+  auto grad = [&](int k, real& gx, real& gy) {
+    const long long pl = (long long)k * g.n2;
+    const real pn = n.hasN ? pbc[pl + n.c2 + g.nx] : real(0);
+    const real ps = n.hasS ? pbc[pl + n.c2 - g.nx] : real(0);
+    grad_cons(pbc[pl + n.c2], pbc[pl + n.e2], pbc[pl + n.w2], pn, ps, n, k, g, gx, gy);
+  };
+  real gxp, gyp;
+  grad(0, gxp, gyp);
+  real sx = 0, sy = 0;
+  for (int k = 0; k + 1 < g.nz; ++k) {
+    real gxn, gyn;
+    grad(k + 1, gxn, gyn);
+    if (k + 1 < n.nwC) {
+      sx = sx + real(0.5) * (gxp + gxn) * g.dz3[k];
+      sy = sy + real(0.5) * (gyp + gyn) * g.dz3[k];
+    }
+    gxp = gxn;
+    gyp = gyn;
+  }
+  Fx[c2] = -sx * P.inv_rho0H;
+  Fy[c2] = -sy * P.inv_rho0H;
+  if (ubt0) {
+    real su = 0, sv = 0;
+    real up = u[c2], vp = v[c2];
+    for (int k = 0; k + 1 < g.nz; ++k) {
+      const long long o = (long long)(k + 1) * g.n2 + c2;
+      const real un = u[o], vn = v[o];
+      su = su + real(0.5) * (up + un) * g.dznorm[k];
+      sv = sv + real(0.5) * (vp + vn) * g.dznorm[k];
+      up = un;
+      vp = vn;
+    }
+    ubt0[c2] = su;
+    vbt0[c2] = sv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum advection before de-aliasing (_advection_flux_form,
+// transport.py:169-197): -(u d/dx + v d/dy + w d/dz) with the face-gated
+// gradient (horizontal.py:331-360) and w = -Fz * wet (transport.py:149-166).
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ void grad_gated(const real* __restrict__ f, long long plane, const Nb& n,
+                                           int k, const Geo<real>& g, real& gx, real& gy) {
+  const real fc = f[plane + n.c2];
+  const bool wc = k < n.nwC;
+  const real dxp = (wc && k < n.nwE) ? (f[plane + n.e2] - fc) : real(0);
+  const real dxm = (wc && k < n.nwW) ? (fc - f[plane + n.w2]) : real(0);
+  gx = g.idx[n.j] * real(0.5) * (dxp + dxm);
+  const real dyp = (wc && k < n.nwN) ? (f[plane + n.n2e] - fc) : real(0);
+  const real dym = (wc && k < n.nwS) ? (fc - f[plane + n.s2e]) : real(0);
+  gy = g.idy * real(0.5) * (dyp + dym);
+}
+
+template <typename real>
+__device__ __forceinline__ real d_dz_filled(const real* __restrict__ f, int c2, int n2, int k, int nw,
+                                            const Geo<real>& g) {
+  const int kb = nw > 0 ? nw - 1 : 0;
+  auto val = [&](int kk) { return f[(long long)(kk < kb ? kk : kb) * n2 + c2]; };
+  if (k == 0) return (val(1) - val(0)) * g.idz_top;
+  if (k == g.nz - 1) return (val(g.nz - 1) - val(g.nz - 2)) * g.idz_bot;
+  return (val(k + 1) - val(k - 1)) * g.iddz[k - 1];
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_mom_adv(Geo<real> g, const real* __restrict__ u,
+                                                 const real* __restrict__ v, const real* __restrict__ Fz,
+                                                 real* __restrict__ au, real* __restrict__ av) {
+  ZM_CELL_PROLOGUE
+  const bool wc = k < n.nwC;
+  const real w = wc ? -Fz[c] : real(0);
+  const real uc = u[c], vc = v[c];
+  real dudx, dudy, dvdx, dvdy;
+  grad_gated(u, plane, n, k, g, dudx, dudy);
+  grad_gated(v, plane, n, k, g, dvdx, dvdy);
+  const real dudz = d_dz_filled(u, n.c2, g.n2, k, n.nwC, g);
+  const real dvdz = d_dz_filled(v, n.c2, g.n2, k, n.nwC, g);
+  au[c] = -(uc * dudx + vc * dudy + w * dudz);
+  av[c] = -(uc * dvdx + vc * dvdy + w * dvdz);
+}
+
+// ---------------------------------------------------------------------------
+// x part of _dealias_h_fd (horizontal.py:297-300): the 2/3-rule projection
+// Re(IFFT(mask * FFT(row))).  The projection is real, so one complex row
+// z = a + i b carries the a row of u-advection and the b row of v-advection.
+// DFT of length N = N1 N2 by the four-step factorisation in LDS; the
+// twiddle table holds exp(-2 pi i e / N), e in [0, N).
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(256) void k_dealias_x(int N, int N1, int N2, int keep, int nrows,
+                                                   const real* __restrict__ twr,
+                                                   const real* __restrict__ twi,
+                                                   const real* __restrict__ ina,
+                                                   const real* __restrict__ inb,
+                                                   real* __restrict__ outa, real* __restrict__ outb) {
+  extern __shared__ unsigned char zm_lds[];
+  real* ar = reinterpret_cast<real*>(zm_lds);
+  real* ai = ar + N;
+  real* br = ai + N;
+  real* bi = br + N;
+  real* wr = bi + N;
+  real* wi = wr + N;
+  const int row = blockIdx.x;
+  if (row >= nrows) return;
+  const long long off = (long long)row * N;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    ar[t] = ina[off + t];
+    ai[t] = inb[off + t];
+    wr[t] = twr[t];
+    wi[t] = twi[t];
+  }
+  __syncthreads();
+  // forward, stage 1: Y[k1 N2 + n2] = W^{n2 k1} sum_n1 a[N2 n1 + n2] W^{N2 n1 k1}
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const int k1 = t / N2, n2 = t - k1 * N2;
+    const int step = N2 * k1;  // < N
+    int e = 0;
+    real sr = 0, si = 0;
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const real xr = ar[N2 * n1 + n2], xi = ai[N2 * n1 + n2];
+      sr += xr * wr[e] - xi * wi[e];
+      si += xr * wi[e] + xi * wr[e];
+      e += step;
+      if (e >= N) e -= N;
+    }
+    const int e2 = (n2 * k1) % N;
+    br[t] = sr * wr[e2] - si * wi[e2];
+    bi[t] = sr * wi[e2] + si * wr[e2];
+  }
+  __syncthreads();
+  // forward, stage 2 + mask: X[k1 + N1 k2] = sum_n2 Y[k1 N2 + n2] W^{N1 n2 k2}
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const int k2 = t / N1, k1 = t - k2 * N1;
+    real sr = 0, si = 0;
+    if (t <= keep || t >= N - keep) {
+      const int step = N1 * k2;
+      int e = 0;
+      for (int n2 = 0; n2 < N2; ++n2) {
+        const real yr = br[k1 * N2 + n2], yi = bi[k1 * N2 + n2];
+        sr += yr * wr[e] - yi * wi[e];
+        si += yr * wi[e] + yi * wr[e];
+        e += step;
+        if (e >= N) e -= N;
+      }
+    }
+    ar[t] = sr;
+    ai[t] = si;
+  }
+  __syncthreads();
+  // inverse, stage 1: Z[k1 N2 + n2] = conj(W)^{k1 n2} sum_k2 X[k1 + N1 k2] conj(W)^{N1 k2 n2}
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const int k1 = t / N2, n2 = t - k1 * N2;
+    const int step = N1 * n2 % N;
+    int e = 0;
+    real sr = 0, si = 0;
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const real xr = ar[k1 + N1 * k2], xi = ai[k1 + N1 * k2];
+      sr += xr * wr[e] + xi * wi[e];
+      si += xi * wr[e] - xr * wi[e];
+      e += step;
+      if (e >= N) e -= N;
+    }
+    const int e2 = (k1 * n2) % N;
+    br[t] = sr * wr[e2] + si * wi[e2];
+    bi[t] = si * wr[e2] - sr * wi[e2];
+  }
+  __syncthreads();
+  // inverse, stage 2: x[n2 + N2 n1] = (1/N) sum_k1 Z[k1 N2 + n2] conj(W)^{N2 k1 n1}
+  const real invN = real(1) / real(N);
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const int n1 = t / N2, n2 = t - n1 * N2;
+    const int step = N2 * n1;
+    int e = 0;
+    real sr = 0, si = 0;
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const real zr = br[k1 * N2 + n2], zi = bi[k1 * N2 + n2];
+      sr += zr * wr[e] + zi * wi[e];
+      si += zi * wr[e] - zr * wi[e];
+      e += step;
+      if (e >= N) e -= N;
+    }
+    outa[off + t] = sr * invN;
+    outb[off + t] = si * invN;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum residual (tendencies.py:65-98 and 273-345, legacy mode-split
+// branch) fused with the RK2 velocity update (step.py:263-319).
+//   mode 0: du1 -> buffers, u_pred = u + du1 dt
+//   mode 1: u_new = (u + 0.5 (du1 + du2) dt) * exp(-r_bot dt bm)
+// ax/av hold the x-de-aliased advection; the y binomial of _dealias_h_fd
+// (horizontal.py:301-305) is applied here.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(256) void k_mom_residual(
+    Geo<real> g, Phys<real> P, int mode, const real* __restrict__ ax, const real* __restrict__ ay,
+    const real* __restrict__ u, const real* __restrict__ v, const real* __restrict__ pbc,
+    const real* __restrict__ eta, const real* __restrict__ Frx, const real* __restrict__ Fry,
+    real* __restrict__ du1, real* __restrict__ dv1, real* __restrict__ uo, real* __restrict__ vo,
+    real decay_bot) {
+  ZM_CELL_PROLOGUE
+  const bool wc = k < n.nwC;
+  const real uc = u[c], vc = v[c];
+  const real f = g.frow[j];
+  // y binomial [1,4,6,4,1]/16 with 2-row edge padding
+  const int jm2 = j >= 2 ? j - 2 : 0, jm1 = j >= 1 ? j - 1 : 0;
+  const int jp1 = j + 1 < g.ny ? j + 1 : g.ny - 1, jp2 = j + 2 < g.ny ? j + 2 : g.ny - 1;
+  const long long rb = plane + i;
+  auto binom = [&](const real* a) {
+    return (a[rb + (long long)jm2 * g.nx] + real(4) * a[rb + (long long)jm1 * g.nx] +
+            real(6) * a[rb + (long long)j * g.nx] + real(4) * a[rb + (long long)jp1 * g.nx] +
+            a[rb + (long long)jp2 * g.nx]) / real(16);
+  };
+  const real advu = wc ? binom(ax) : real(0);
+  const real advv = wc ? binom(ay) : real(0);
+  // hydrostatic pressure p = rho0 g eta + p_bc at the five points
+  const real rg = real(kRho0 * kG);
+  const real pc = rg * eta[n.c2] + pbc[plane + n.c2];
+  const real pe = rg * eta[n.e2] + pbc[plane + n.e2];
+  const real pw = rg * eta[n.w2] + pbc[plane + n.w2];
+  const real pn = n.hasN ? rg * eta[n.c2 + g.nx] + pbc[plane + n.c2 + g.nx] : real(0);
+  const real ps = n.hasS ? rg * eta[n.c2 - g.nx] + pbc[plane + n.c2 - g.nx] : real(0);
+  real gpx, gpy;
+  grad_cons(pc, pe, pw, pn, ps, n, k, g, gpx, gpy);
+  const real pgfx = -gpx / real(kRho0), pgfy = -gpy / real(kRho0);
+  const real lapu = lap_h(u, plane, n, k, g), lapv = lap_h(v, plane, n, k, g);
+  const real dvu = P.nu_v * d2dz2(u, n.c2, g.n2, k, n.nwC, g);
+  const real dvv = P.nu_v * d2dz2(v, n.c2, g.n2, k, n.nwC, g);
+  real dudt = 0, dvdt = 0;
+  if (wc) {
+    dudt = advu + f * vc + pgfx + P.nu_h * lapu + dvu;
+    dvdt = advv + (-f) * uc + pgfy + P.nu_h * lapv + dvv;
+    if (k == 0) {
+      dudt = dudt + P.tau_x[n.c2] * P.wind_factor;
+      dvdt = dvdt + P.tau_y[n.c2] * P.wind_factor;
+    }
+    if (k == g.nz - 1) {
+      dudt = dudt + (-P.r_bot) * uc;
+      dvdt = dvdt + (-P.r_bot) * vc;
+    }
+  }
+  dudt = dudt - P.nu_h * lapu;
+  dvdt = dvdt - P.nu_h * lapv;
+  dudt = dudt - dvu;
+  dvdt = dvdt - dvv;
+  dudt = dudt - f * vc;
+  dvdt = dvdt + f * uc;
+  // cancel the eta part with the same layer-gated operator
+  const real en = n.hasN ? eta[n.c2 + g.nx] : real(0);
+  const real es = n.hasS ? eta[n.c2 - g.nx] : real(0);
+  real gex, gey;
+  grad_cons(eta[n.c2], eta[n.e2], eta[n.w2], en, es, n, k, g, gex, gey);
+  dudt = dudt + real(kG) * gex;
+  dvdt = dvdt + real(kG) * gey;
+  dudt = dudt - Frx[n.c2];
+  dvdt = dvdt - Fry[n.c2];
+  dudt = dudt - P.tau_x[n.c2] * P.inv_rho0H;
+  dvdt = dvdt - P.tau_y[n.c2] * P.inv_rho0H;
+  if (!wc) {
+    dudt = 0;
+    dvdt = 0;
+  }
+  const bool bm = wc && k == g.nz - 1;
+  if (bm) {
+    dudt = dudt + P.r_bot * uc;
+    dvdt = dvdt + P.r_bot * vc;
+  }
+  if (mode == 0) {
+    du1[c] = dudt;
+    dv1[c] = dvdt;
+    uo[c] = uc + dudt * P.dt;
+    vo[c] = vc + dvdt * P.dt;
+  } else {
+    real un = uc + real(0.5) * (du1[c] + dudt) * P.dt;
+    real vn = vc + real(0.5) * (dv1[c] + dvdt) * P.dt;
+    if (bm) {
+      un = un * decay_bot;
+      vn = vn * decay_bot;
+    }
+    uo[c] = un;
+    vo[c] = vn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Barotropic forward-backward substep, legacy mode-split branch of
+// _free_surface_step_fd (dynamics/barotropic.py:86-254).  One workgroup per
+// latitude row; rows of the polar-cap bands reduce their zonal means in LDS
+// (_apply_polar_cap legacy, horizontal.py:387-451).
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ real block_sum(real x, real* red) {
+  // 256-thread sum, result broadcast to all threads
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  real s = 0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int w = 0; w < nw; ++w) s += red[w];
+  return s;
+}
+
+template <typename real>
+__device__ __forceinline__ real cap_weight(const Phys<real>& P, int j, int ny) {
+  if (j < P.nb) return P.capw[j];
+  if (j >= ny - P.nb) return P.capw[ny - 1 - j];
+  return real(-1);
+}
+
+constexpr int kRowChunk = 4;  // cells per thread per row (nx <= 1024)
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_eta(Geo<real> g, Phys<real> P, real* __restrict__ eta,
+                                                const real* __restrict__ ubt, const real* __restrict__ vbt,
+                                                real dtH) {
+  __shared__ real red[8];
+  const int j = blockIdx.x;
+  const real* wet = P.wet2;
+  real en[kRowChunk];
+  real ssum = 0, wsum = 0;
+#pragma unroll
+  for (int q = 0; q < kRowChunk; ++q) {
+    const int i = threadIdx.x + q * 256;
+    en[q] = 0;
+    if (i >= g.nx) continue;
+    const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
+    const bool wc = wet[c2] != real(0);
+    const bool we = wet[e2] != real(0), ww = wet[w2] != real(0);
+    const real fe = (wc && we) ? real(0.5) * (ubt[c2] + ubt[e2]) : real(0);
+    const real fw = (ww && wc) ? real(0.5) * (ubt[w2] + ubt[c2]) : real(0);
+    const real divx = (fe - fw) * g.idx[j];
+    real fn = 0, fs = 0;
+    if (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) fn = real(0.5) * (vbt[c2] + vbt[c2 + g.nx]) * g.cfn[j];
+    if (j > 0 && wc && wet[c2 - g.nx] != real(0)) fs = real(0.5) * (vbt[c2 - g.nx] + vbt[c2]) * g.cfn[j - 1];
+    const real divy = (fn - fs) * g.idyc[j];
+    const real e = wc ? eta[c2] - dtH * (divx + divy) : real(0);
+    en[q] = e;
+    ssum += e;
+    wsum += wet[c2];
+  }
+  const real wb = cap_weight(P, j, g.ny);
+  real zm = 0;
+  if (wb >= real(0)) {
+    const real s = block_sum(ssum, red);
+    const real w = block_sum(wsum, red + 4);
+    zm = s / (w > real(1) ? w : real(1));
+  }
+#pragma unroll
+  for (int q = 0; q < kRowChunk; ++q) {
+    const int i = threadIdx.x + q * 256;
+    if (i >= g.nx) continue;
+    const int c2 = j * g.nx + i;
+    real e = en[q];
+    if (wb >= real(0)) e = wet[c2] != real(0) ? wb * zm + (real(1) - wb) * e : real(0);
+    eta[c2] = e;
+  }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const real* __restrict__ eta,
+                                                real* __restrict__ ubt, real* __restrict__ vbt,
+                                                const real* __restrict__ Frx, const real* __restrict__ Fry,
+                                                const real* __restrict__ fdt, const real* __restrict__ fden,
+                                                real dt_bt, real drag) {
+  __shared__ real red[16];
+  const int j = blockIdx.x;
+  const real* wet = P.wet2;
+  real un[kRowChunk], vn[kRowChunk];
+  real su = 0, sv = 0, sw = 0;
+  const real fd = fdt[j], den = fden[j];
+#pragma unroll
+  for (int q = 0; q < kRowChunk; ++q) {
+    const int i = threadIdx.x + q * 256;
+    un[q] = vn[q] = 0;
+    if (i >= g.nx) continue;
+    const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
+    const bool wc = wet[c2] != real(0);
+    const real ec = eta[c2];
+    const real dxp = (wc && wet[e2] != real(0)) ? eta[e2] - ec : real(0);
+    const real dxm = (wc && wet[w2] != real(0)) ? ec - eta[w2] : real(0);
+    const real gx = g.idx[j] * real(0.5) * (dxp + dxm);
+    const real dyp = (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) ? (eta[c2 + g.nx] - ec) * g.cfn[j] : real(0);
+    const real dym = (j > 0 && wc && wet[c2 - g.nx] != real(0)) ? (ec - eta[c2 - g.nx]) * g.cfs[j] : real(0);
+    const real gy = g.idyc[j] * real(0.5) * (dyp + dym);
+    const real Fx = Frx[c2] + P.tau_x[c2] * P.inv_rho0H;
+    const real Fy = Fry[c2] + P.tau_y[c2] * P.inv_rho0H;
+    const real us = ubt[c2] + dt_bt * (real(-kG) * gx + Fx);
+    const real vs = vbt[c2] + dt_bt * (real(-kG) * gy + Fy);
+    real a = (us + fd * vs) / den;
+    real b = (vs - fd * us) / den;
+    a = wc ? a * drag : real(0);
+    b = wc ? b * drag : real(0);
+    un[q] = a;
+    vn[q] = b;
+    su += a;
+    sv += b;
+    sw += wet[c2];
+  }
+  const real wb = cap_weight(P, j, g.ny);
+  real zu = 0, zv = 0;
+  if (wb >= real(0)) {
+    const real s1 = block_sum(su, red);
+    const real s2 = block_sum(sv, red + 4);
+    const real w = block_sum(sw, red + 8);
+    const real d = w > real(1) ? w : real(1);
+    zu = s1 / d;
+    zv = s2 / d;
+  }
+#pragma unroll
+  for (int q = 0; q < kRowChunk; ++q) {
+    const int i = threadIdx.x + q * 256;
+    if (i >= g.nx) continue;
+    const int c2 = j * g.nx + i;
+    real a = un[q], b = vn[q];
+    if (wb >= real(0)) {
+      const bool wc = wet[c2] != real(0);
+      a = wc ? wb * zu + (real(1) - wb) * a : real(0);
+      b = wc ? wb * zv + (real(1) - wb) * b : real(0);
+    }
+    ubt[c2] = a;
+    vbt[c2] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final masks (step.py:419-450): project the barotropic delta on every node,
+// polar-cap u, v, T, S with the 3-D wet mask, hold land/ghost at the pre-step
+// values and zero v on the wall rows.  k_cap_means reduces the band rows.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(256) void k_cap_means(Geo<real> g, Phys<real> P, const real* __restrict__ u,
+                                                   const real* __restrict__ v, const real* __restrict__ T,
+                                                   const real* __restrict__ S, const real* __restrict__ ubt,
+                                                   const real* __restrict__ ubt0, const real* __restrict__ vbt,
+                                                   const real* __restrict__ vbt0, real* __restrict__ means) {
+  __shared__ real red[20];
+  const int band_row = blockIdx.x % (2 * P.nb);
+  const int k = blockIdx.x / (2 * P.nb);
+  const int j = band_row < P.nb ? band_row : g.ny - 2 * P.nb + band_row;
+  const long long plane = (long long)k * g.n2;
+  real su = 0, sv = 0, sT = 0, sS = 0, sw = 0;
+  for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+    const int c2 = j * g.nx + i;
+    if (k < g.nwet[c2]) {
+      su += u[plane + c2] + (ubt[c2] - ubt0[c2]);
+      sv += v[plane + c2] + (vbt[c2] - vbt0[c2]);
+      sT += T[plane + c2];
+      sS += S[plane + c2];
+      sw += real(1);
+    }
+  }
+  su = block_sum(su, red);
+  sv = block_sum(sv, red + 4);
+  sT = block_sum(sT, red + 8);
+  sS = block_sum(sS, red + 12);
+  sw = block_sum(sw, red + 16);
+  if (threadIdx.x == 0) {
+    const real d = sw > real(1) ? sw : real(1);
+    const long long o = (long long)k * 2 * P.nb + band_row;
+    const long long stride = (long long)g.nz * 2 * P.nb;
+    means[o] = su / d;
+    means[o + stride] = sv / d;
+    means[o + 2 * stride] = sT / d;
+    means[o + 3 * stride] = sS / d;
+  }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_final(Geo<real> g, Phys<real> P, const real* __restrict__ u3,
+                                               const real* __restrict__ v3, const real* __restrict__ T3,
+                                               const real* __restrict__ S3, const real* __restrict__ ubt,
+                                               const real* __restrict__ ubt0, const real* __restrict__ vbt,
+                                               const real* __restrict__ vbt0, const real* __restrict__ means,
+                                               const real* __restrict__ uin, const real* __restrict__ vin,
+                                               const real* __restrict__ Tin, const real* __restrict__ Sin,
+                                               real* __restrict__ uo, real* __restrict__ vo,
+                                               real* __restrict__ To, real* __restrict__ So) {
+  const long long c = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (c >= g.n3) return;
+  int i, j, k;
+  cell_of(c, g.n2, g.nx, i, j, k);
+  const int c2 = j * g.nx + i;
+  const bool wc = k < g.nwet[c2];
+  real u = u3[c] + (ubt[c2] - ubt0[c2]);
+  real v = v3[c] + (vbt[c2] - vbt0[c2]);
+  real T = T3[c], S = S3[c];
+  const real wb = P.nb > 0 ? cap_weight(P, j, g.ny) : real(-1);
+  if (wb >= real(0)) {
+    const int band_row = j < P.nb ? j : j - (g.ny - 2 * P.nb);
+    const long long o = (long long)k * 2 * P.nb + band_row;
+    const long long stride = (long long)g.nz * 2 * P.nb;
+    const real om = real(1) - wb;
+    u = wb * means[o] + om * u;
+    v = wb * means[o + stride] + om * v;
+    T = wb * means[o + 2 * stride] + om * T;
+    S = wb * means[o + 3 * stride] + om * S;
+  }
+  if (!wc) {
+    u = uin[c];
+    v = vin[c];
+    T = Tin[c];
+    S = Sin[c];
+  }
+  if (j == 0 || j == g.ny - 1) v = real(0);
+  uo[c] = u;
+  vo[c] = v;
+  To[c] = T;
+  So[c] = S;
+}
+
+// Diagnostics (factory.py:752-760): rho = rho0 + rho', p = rho0 g eta + p_bc.
+template <typename real>
+__global__ __launch_bounds__(256) void k_diag(Geo<real> g, Phys<real> P, const real* __restrict__ T,
+                                              const real* __restrict__ S, const real* __restrict__ eta,
+                                              const real* __restrict__ pbc, const real* __restrict__ Fz,
+                                              real* __restrict__ rho, real* __restrict__ pres,
+                                              real* __restrict__ w) {
+  const long long c = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (c >= g.n3) return;
+  int i, j, k;
+  cell_of(c, g.n2, g.nx, i, j, k);
+  const int c2 = j * g.nx + i;
+  rho[c] = real(kRho0) + rho_anom(T[c], S[c], P);
+  pres[c] = real(kRho0 * kG) * eta[c2] + pbc[c];
+  w[c] = k < g.nwet[c2] ? -Fz[c] : real(0);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_copy(const real* __restrict__ a, real* __restrict__ b, long long n) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+template <typename real>
+void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,
+                       real* lv, real nu_h, real dts, hipStream_t s) {
+  hipLaunchKernelGGL(k_nu_substep<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, u, v, uo, vo, lu,
+                     lv, nu_h, dts);
+}
+
+template <typename real>
+void launch_d1_unit(const Geo<real>& g, const real* T, const real* S, real* dT, real* dS, hipStream_t s) {
+  hipLaunchKernelGGL(k_d1_unit<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, T, S, dT, dS);
+}
+
+template <typename real>
+void launch_linear_finish(const Geo<real>& g, const Phys<real>& P, const real* u0, const real* v0,
+                          const real* T0, const real* S0, const real* unu, const real* vnu, const real* Lu0,
+                          const real* Lv0, const real* DT0, const real* DS0, real* uo, real* vo, real* To,
+                          real* So, hipStream_t s) {
+  hipLaunchKernelGGL(k_linear_finish<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, u0, v0, T0,
+                     S0, unu, vnu, Lu0, Lv0, DT0, DS0, uo, vo, To, So);
+}
+
+template <typename real>
+void launch_fz(const Geo<real>& g, const real* u, const real* v, real* Fz, hipStream_t s) {
+  hipLaunchKernelGGL(k_fz<real>, dim3(blocks_for(g.n2, 256)), dim3(256), 0, s, g, u, v, Fz);
+}
+
+template <typename real>
+void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
+                 int* gate, hipStream_t s) {
+  const size_t lds = (size_t)2 * g.nz * 64 * sizeof(real);
+  hipLaunchKernelGGL(k_conv<real>, dim3(blocks_for(g.n2, 64)), dim3(64), lds, s, g, P, T, S, cT, cS, gate);
+}
+
+template <typename real>
+void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* u, const real* v,
+                            const real* T, const real* S, const real* Fz, const real* cT, const real* cS,
+                            const int* gate, real* dT1, real* dS1, const real* Tb, const real* Sb, real* To,
+                            real* So, hipStream_t s) {
+  hipLaunchKernelGGL(k_tracer_residual<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, mode, u, v,
+                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
+}
+
+template <typename real>
+void launch_pbc(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* pbc,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_pbc<real>, dim3(blocks_for(g.n2, 256)), dim3(256), 0, s, g, P, T, S, pbc);
+}
+
+template <typename real>
+void launch_bt_rho(const Geo<real>& g, const Phys<real>& P, const real* pbc, real* Fx, real* Fy, const real* u,
+                   const real* v, real* ubt0, real* vbt0, hipStream_t s) {
+  hipLaunchKernelGGL(k_bt_rho<real>, dim3(blocks_for(g.n2, 256)), dim3(256), 0, s, g, P, pbc, Fx, Fy, u, v,
+                     ubt0, vbt0);
+}
+
+template <typename real>
+void launch_mom_adv(const Geo<real>& g, const real* u, const real* v, const real* Fz, real* au, real* av,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_mom_adv<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, u, v, Fz, au, av);
+}
+
+template <typename real>
+void launch_dealias_x(int N, int N1, int N2, int keep, int nrows, const real* twr, const real* twi,
+                      const real* ina, const real* inb, real* outa, real* outb, hipStream_t s) {
+  const size_t lds = (size_t)6 * N * sizeof(real);
+  hipLaunchKernelGGL(k_dealias_x<real>, dim3(nrows), dim3(256), lds, s, N, N1, N2, keep, nrows, twr, twi, ina,
+                     inb, outa, outb);
+}
+
+template <typename real>
+void launch_mom_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* ax, const real* ay,
+                         const real* u, const real* v, const real* pbc, const real* eta, const real* Frx,
+                         const real* Fry, real* du1, real* dv1, real* uo, real* vo, real decay_bot,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_mom_residual<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, mode, ax, ay, u,
+                     v, pbc, eta, Frx, Fry, du1, dv1, uo, vo, decay_bot);
+}
+
+template <typename real>
+void launch_bt_eta(const Geo<real>& g, const Phys<real>& P, real* eta, const real* ubt, const real* vbt,
+                   real dtH, hipStream_t s) {
+  hipLaunchKernelGGL(k_bt_eta<real>, dim3(g.ny), dim3(256), 0, s, g, P, eta, ubt, vbt, dtH);
+}
+
+template <typename real>
+void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, real* ubt, real* vbt,
+                   const real* Frx, const real* Fry, const real* fdt, const real* fden, real dt_bt, real drag,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_bt_vel<real>, dim3(g.ny), dim3(256), 0, s, g, P, eta, ubt, vbt, Frx, Fry, fdt, fden,
+                     dt_bt, drag);
+}
+
+template <typename real>
+void launch_cap_means(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
+                      const real* S, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
+                      real* means, hipStream_t s) {
+  if (P.nb <= 0) return;
+  hipLaunchKernelGGL(k_cap_means<real>, dim3(2 * P.nb * g.nz), dim3(256), 0, s, g, P, u, v, T, S, ubt, ubt0,
+                     vbt, vbt0, means);
+}
+
+template <typename real>
+void launch_final(const Geo<real>& g, const Phys<real>& P, const real* u3, const real* v3, const real* T3,
+                  const real* S3, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
+                  const real* means, const real* uin, const real* vin, const real* Tin, const real* Sin,
+                  real* uo, real* vo, real* To, real* So, hipStream_t s) {
+  hipLaunchKernelGGL(k_final<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, u3, v3, T3, S3, ubt,
+                     ubt0, vbt, vbt0, means, uin, vin, Tin, Sin, uo, vo, To, So);
+}
+
+template <typename real>
+void launch_diag(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, const real* eta,
+                 const real* pbc, const real* Fz, real* rho, real* pres, real* w, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, T, S, eta, pbc, Fz, rho,
+                     pres, w);
+}
+
+template <typename real>
+void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy<real>, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, b, n);
+}
+
+#define ZM_INSTANTIATE(real)                                                                                 \
+  template void launch_nu_substep<real>(const Geo<real>&, const real*, const real*, real*, real*, real*,    \
+                                        real*, real, real, hipStream_t);                                      \
+  template void launch_d1_unit<real>(const Geo<real>&, const real*, const real*, real*, real*, hipStream_t); \
+  template void launch_linear_finish<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,   \
+                                           const real*, const real*, const real*, const real*, const real*, \
+                                           const real*, const real*, const real*, real*, real*, real*,      \
+                                           real*, hipStream_t);                                              \
+  template void launch_fz<real>(const Geo<real>&, const real*, const real*, real*, hipStream_t);            \
+  template void launch_conv<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, real*,     \
+                                  real*, int*, hipStream_t);                                                 \
+  template void launch_tracer_residual<real>(const Geo<real>&, const Phys<real>&, int, const real*,         \
+                                             const real*, const real*, const real*, const real*,            \
+                                             const real*, const real*, const int*, real*, real*,            \
+                                             const real*, const real*, real*, real*, hipStream_t);           \
+  template void launch_pbc<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, real*,      \
+                                 hipStream_t);                                                               \
+  template void launch_bt_rho<real>(const Geo<real>&, const Phys<real>&, const real*, real*, real*,         \
+                                    const real*, const real*, real*, real*, hipStream_t);                    \
+  template void launch_mom_adv<real>(const Geo<real>&, const real*, const real*, const real*, real*, real*,  \
+                                     hipStream_t);                                                           \
+  template void launch_dealias_x<real>(int, int, int, int, int, const real*, const real*, const real*,      \
+                                       const real*, real*, real*, hipStream_t);                              \
+  template void launch_mom_residual<real>(const Geo<real>&, const Phys<real>&, int, const real*,            \
+                                          const real*, const real*, const real*, const real*, const real*,  \
+                                          const real*, const real*, real*, real*, real*, real*, real,       \
+                                          hipStream_t);                                                      \
+  template void launch_bt_eta<real>(const Geo<real>&, const Phys<real>&, real*, const real*, const real*,   \
+                                    real, hipStream_t);                                                      \
+  template void launch_bt_vel<real>(const Geo<real>&, const Phys<real>&, const real*, real*, real*,         \
+                                    const real*, const real*, const real*, const real*, real, real,         \
+                                    hipStream_t);                                                            \
+  template void launch_cap_means<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,       \
+                                       const real*, const real*, const real*, const real*, const real*,     \
+                                       const real*, real*, hipStream_t);                                     \
+  template void launch_final<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,           \
+                                   const real*, const real*, const real*, const real*, const real*,         \
+                                   const real*, const real*, const real*, const real*, const real*,         \
+                                   const real*, real*, real*, real*, real*, hipStream_t);                    \
+  template void launch_diag<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,            \
+                                  const real*, const real*, const real*, real*, real*, real*, hipStream_t); \
+  template void launch_copy<real>(const real*, real*, long long, hipStream_t);
+
+ZM_INSTANTIATE(float)
+ZM_INSTANTIATE(double)
+
+}  // namespace zm
