@@ -1,0 +1,174 @@
+"""Benchmark of the mode-split step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case c2] [--dtype float32]
+
+Workload: the 0.5-degree plain FP32 step (720 x 320 x 50, dt = 900 s,
+45 barotropic and 45 nu_h substeps, seasonal wind via step_dyn), synthetic
+seeded inputs resident in HBM.  One "step" is one baroclinic step.  Prints one
+JSON line: SYPD of the whole step, the HBM roofline of the dominant kernel
+(the nu_h Laplacian substep) and a CPU baseline from the NumPy oracle.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SYPD at 0.5deg x 50L on 1 GPU: learned+mixed vs plain FP32; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_step_bytes(nx, ny, nz, n_nu, n_sub, word):
+    """B_alg = (8 n_nu + 86) F3 + 12 n_sub F2 (compulsory traffic of one step)."""
+    f3 = nx * ny * nz * word
+    f2 = nx * ny * word
+    return (8 * n_nu + 86) * f3 + 12 * n_sub * f2
+
+
+def read_pmc_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if present."""
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(seconds_budget=25.0):
+    """Oracle (NumPy float64, one core) on a bounded sample, scaled to SYPD at c2."""
+    from oracle import zm_oracle as O
+    from zhenmode_amd.cases import make_case
+
+    case = make_case("c1")
+    kw = {k: v for k, v in case.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
+    p = O.make_params(case.grid, case.physics, case.dt, **kw)
+    s = O.init_state(p, case.T_init, case.S_init)
+    t0 = time.perf_counter()
+    s = O.step(s, p)
+    t1 = time.perf_counter() - t0
+    # scale one c1 step to one c2 step by the algorithmic byte ratio
+    b1 = algorithmic_step_bytes(360, 160, 50, 40, 40, 8)
+    b2 = algorithmic_step_bytes(720, 320, 50, 45, 45, 8)
+    t_c2 = t1 * b2 / b1
+    return {"value": 900.0 / (365.0 * t_c2), "unit": "SYPD", "cores": 1, "kind": "port",
+            "sample": (f"1 oracle step (NumPy float64, single-threaded) of the 1-degree case 360x160x50 "
+                       f"took {t1:.1f} s; scaled to the 720x320x50 step by the ratio of algorithmic bytes "
+                       f"({b2 / b1:.2f}x)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--case", default="c2")
+    ap.add_argument("--dtype", default="float32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
+                                   "(latitude bands + RCCL halo exchange) and is not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from zhenmode_amd.cases import make_case
+    from zhenmode_amd.solver import make_solver_global
+
+    rank = int(os.environ.get("RANK", "0"))
+    case = make_case(args.case)
+    g = case.grid
+    kwargs = dict(case.solver_kwargs)
+    step, init_state, _, params, _, step_dyn = make_solver_global(
+        g, case.physics, case.dt, return_params=True, dynamic_forcing=True, dtype=args.dtype, **kwargs)
+    solver = step.solver
+    dev = solver.device
+    tdtype = solver.dtype
+    a = init_state(case.T_init, case.S_init)
+    b = solver.new_state()
+    # on-device seasonal wind: tau_x(t) = (0.1 + 0.05 sin(2 pi t / 365 d)) cos(3 lat)
+    from zhenmode_amd.solver import empty2
+    base = torch.as_tensor(np.cos(3.0 * np.radians(g.lat)), dtype=tdtype, device=dev)[:, None].expand(g.ny, g.nx).contiguous()
+    tau_x = empty2(g.nx, g.ny, tdtype, dev)
+    tau_store = tau_x.permute(1, 0)  # contiguous (ny, nx) storage
+    tau_y = empty2(g.nx, g.ny, tdtype, dev)
+    q = empty2(g.nx, g.ny, tdtype, dev)
+    tau_y.zero_()
+    q.zero_()
+
+    def one(t_index, src, dst):
+        t = t_index * case.dt
+        amp = 0.1 + (0.05 * np.sin(2.0 * np.pi * t / (365.0 * 86400.0)) if case.seasonal_wind else 0.0)
+        torch.mul(base, amp, out=tau_store)
+        return solver.step_dyn(src, tau_x, tau_y, q, out=dst)
+
+    n = 0
+    for _ in range(args.warmup):
+        one(n, a, b)
+        a, b = b, a
+        n += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.steps):
+        one(n, a, b)
+        a, b = b, a
+        n += 1
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    finite = bool(torch.isfinite(a.u).all() and torch.isfinite(a.T).all() and torch.isfinite(a.eta).all())
+
+    word = 4 if args.dtype == "float32" else 8
+    n3 = g.nx * g.ny * g.nz
+    nu_h = solver.nu_substeps_per_launch()
+    nu_ms = solver.time_kernel(5 if nu_h > 1 else 0, 50)
+    nu_bytes = nu_h * 4 * n3 * word
+    nu_single_ms = solver.time_kernel(0, 50)
+    achieved = nu_bytes / (nu_ms * 1e-3) / 1e9
+    step_bytes = algorithmic_step_bytes(g.nx, g.ny, g.nz, params.n_nu, params.n_subcyc, word)
+    sypd = case.dt / (365.0 * ms_per_step * 1e-3)
+    out = {
+        "metric": METRIC, "value": sypd, "unit": "SYPD", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if word == 4 else "f64", "data": "synthetic",
+        "config": {"workload": f"{args.case}: plain mode-split step {g.nx}x{g.ny}x{g.nz} "
+                               f"(0.5 deg), dt={case.dt:g}s, n_sub={params.n_subcyc}, n_nu={params.n_nu}, "
+                               f"seasonal wind via step_dyn, {args.dtype}",
+                   "sim_days_per_s": case.dt / 86400.0 / (ms_per_step * 1e-3),
+                   "event_ms_per_step": ev_ms, "finite": finite},
+        "roofline": {"bound": "hbm",
+                     "kernel": (f"k_nu_block ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
+                                else "k_nu_substep (one nu_h Laplacian substep of u and v)"),
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_pmc_traffic(f"nu_block_{args.case}_pmc.txt"),
+                     "bytes_per_launch": nu_bytes, "launch_ms": nu_ms,
+                     "single_substep_launch_ms": nu_single_ms,
+                     "single_substep_GBps": 4 * n3 * word / (nu_single_ms * 1e-3) / 1e9},
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    if rank == 0 and not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

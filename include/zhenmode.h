@@ -64,6 +64,7 @@ typedef struct zm_config {
   const double* d2z_denom;      /* [nz-2] */
   const double* dz_denom_interior; /* [nz-2] */
   const double* cap_weights;    /* [polar_cap_rows + polar_cap_taper] */
+  const int32_t* dealias_mask;  /* [nx] 1 = longitude wavenumber kept (factory.py:529-533) */
   const int32_t* nwet;          /* [ny*nx] wet levels per column */
   const double* wet_mask;       /* [ny*nx] */
   const double* tau_x;          /* [ny*nx] default forcing */
@@ -94,7 +95,8 @@ typedef struct zm_state {
 #define ZM_STAGE_NONLINEAR 1   /* _explicit_full_step, step.py:249-320 */
 #define ZM_STAGE_BAROTROPIC 2  /* mode-split block, step.py:393-428 */
 #define ZM_STAGE_TRACER_RES 3  /* _compute_tracer_residual -> out.T, out.S */
-#define ZM_STAGE_MOMENTUM_RES 4 /* _compute_momentum_residual -> out.u, out.v */
+#define ZM_STAGE_MOMENTUM_RES 4 /* _compute_momentum_residual + r_bot u bm (the drag strip of
+                                  step.py:273-277) -> out.u, out.v */
 
 /* Replaces make_solver_global's parameter freezing and jit compilation
  * (factory.py:30-718).  Returns ZM_EINVAL with a message for an invalid or
@@ -125,10 +127,17 @@ int zm_set_graph(zm_handle* h, int32_t enabled);
 
 /* Average duration [ms] of one launch of a step kernel, timed with HIP events
  * around `reps` back-to-back launches on the handle's own stream.
- * which: 0 = nu_h Laplacian substep, 1 = tracer residual, 2 = momentum
- * residual, 3 = x de-alias, 4 = one barotropic substep (2 launches),
- * 5 = whole step. */
+ * which: 0 = nu_h Laplacian substep (one substep per launch), 1 = tracer
+ * residual, 2 = momentum residual, 3 = x de-alias, 4 = one barotropic
+ * substep (2 launches), 5 = temporally blocked nu_h launch
+ * (zm_nu_substeps_per_launch substeps), 6 = plain copy of one 3-D field
+ * (bandwidth calibration), 7 = linear-step finish, 8 = convection,
+ * 9 = Fz column scan, 10 = unit D1 pair, 11 = momentum advection,
+ * 12 = final masks. */
 int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch);
+
+/* Number of nu_h substeps one blocked launch performs (1 = unblocked). */
+int32_t zm_nu_substeps_per_launch(const zm_handle* h);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,7 @@ class ZmConfig(ctypes.Structure):
         ("cos_lat", _D), ("dx_row", _D), ("f_row", _D),
         ("dz_node", _D), ("dz_iface", _D), ("dz", _D), ("dz_norm", _D),
         ("d2z_hm", _D), ("d2z_hp", _D), ("d2z_denom", _D), ("dz_denom_interior", _D),
-        ("cap_weights", _D),
+        ("cap_weights", _D), ("dealias_mask", _I),
         ("nwet", _I), ("wet_mask", _D),
         ("tau_x", _D), ("tau_y", _D), ("q_heat", _D), ("t_atm", _D),
         ("s_ref_surf", _D), ("coastal_restore_coef", _D), ("coastal_restore_T", _D),
@@ -50,7 +50,7 @@ class ZmState(ctypes.Structure):
 
 
 EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", "zm_step_dyn",
-           "zm_diagnostics", "zm_stage", "zm_set_graph", "zm_time_kernel")
+           "zm_diagnostics", "zm_stage", "zm_set_graph", "zm_time_kernel", "zm_nu_substeps_per_launch")
 
 _lib = None
 
@@ -84,6 +84,8 @@ def load():
     lib.zm_set_graph.restype = ctypes.c_int
     lib.zm_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     lib.zm_time_kernel.restype = ctypes.c_int
+    lib.zm_nu_substeps_per_launch.argtypes = [vp]
+    lib.zm_nu_substeps_per_launch.restype = ctypes.c_int32
     _lib = lib
     return lib
 

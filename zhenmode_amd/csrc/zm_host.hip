@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -46,6 +47,7 @@ struct SolverBase {
   virtual void diagnostics(const zm_state& in, void* rho, void* p, void* w, hipStream_t s) = 0;
   virtual void stage(int which, const zm_state& in, const zm_state& out, hipStream_t s) = 0;
   virtual double time_kernel(int which, int reps) = 0;
+  virtual int nu_substeps_per_launch() const = 0;
   bool use_graph = true;
 };
 
@@ -55,8 +57,10 @@ struct Solver : SolverBase {
   zm::Phys<real> P{};
   int nx = 0, ny = 0, nz = 0;
   long long n2 = 0, n3 = 0;
-  int n_sub = 0, n_nu = 0, N1 = 1, N2 = 1, keep = 0;
+  int n_sub = 0, n_nu = 0, N1 = 1, N2 = 1;
+  int* kept = nullptr;
   real dt_bt = 0, dtH = 0, drag = 0, decay_bot = 0, dts = 0;
+  int nu_block = 1;
   std::vector<void*> allocs;
   real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fden = nullptr;
   // workspace
@@ -227,13 +231,20 @@ struct Solver : SolverBase {
     drag = (real)(1.0 / (1.0 + c.r_bot * c.dt_bt));
     decay_bot = (real)std::exp(-c.r_bot * c.dt);
     dts = (real)((c.dt / 2.0) / n_nu);
+    // substeps fused per launch of the nu_h sub-cycle (1 = one launch per substep)
+    nu_block = zm::nu_block_max<real>();
+    if (const char* env = std::getenv("ZHENMODE_NU_BLOCK")) nu_block = std::max(1, std::min(nu_block, std::atoi(env)));
 
     // DFT factorisation nx = N1 * N2 with N1 the largest divisor <= sqrt(nx)
     N1 = 1;
     for (int d = 1; d * d <= nx; ++d)
       if (nx % d == 0) N1 = d;
     N2 = nx / N1;
-    keep = std::max(1, (int)((nx / 2) * 2.0 / 3.0));
+    {
+      std::vector<int> km(c.dealias_mask, c.dealias_mask + nx);
+      kept = dev_alloc<int>(nx);
+      ZM_HIP(hipMemcpy(kept, km.data(), nx * sizeof(int), hipMemcpyHostToDevice));
+    }
     std::vector<double> wr(nx), wi(nx);
     for (int e = 0; e < nx; ++e) {
       const double a = -2.0 * M_PI * (double)e / (double)nx;
@@ -264,13 +275,22 @@ struct Solver : SolverBase {
     if (bih_t) zm::launch_d1_unit(g, T0, S0, DT, DS, s);
     const real* su = u0;
     const real* sv = v0;
-    for (int n = 0; n < n_nu; ++n) {
-      real* du = (n % 2 == 0) ? nAu : nBu;
-      real* dv = (n % 2 == 0) ? nAv : nBv;
-      zm::launch_nu_substep(g, su, sv, du, dv, (n == 0 && bih_u) ? Lu : (real*)nullptr,
-                            (n == 0 && bih_u) ? Lv : (real*)nullptr, P.nu_h, dts, s);
+    int n = 0, flip = 0;
+    while (n < n_nu) {
+      real* du = flip ? nBu : nAu;
+      real* dv = flip ? nBv : nAv;
+      if ((n == 0 && bih_u) || nu_block <= 1) {
+        zm::launch_nu_substep(g, su, sv, du, dv, (n == 0 && bih_u) ? Lu : (real*)nullptr,
+                              (n == 0 && bih_u) ? Lv : (real*)nullptr, P.nu_h, dts, s);
+        n += 1;
+      } else {
+        const int h = std::min(nu_block, n_nu - n);
+        zm::launch_nu_block(g, su, sv, du, dv, P.nu_h, dts, h, s);
+        n += h;
+      }
       su = du;
       sv = dv;
+      flip ^= 1;
     }
     zm::launch_linear_finish(g, P, u0, v0, T0, S0, su, sv, Lu, Lv, DT, DS, uo, vo, To, So, s);
   }
@@ -286,7 +306,7 @@ struct Solver : SolverBase {
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
                       (real*)nullptr, s);
     zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
-    zm::launch_dealias_x(nx, N1, N2, keep, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+    zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
     zm::launch_mom_residual(g, Pf, 0, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, up, vp, decay_bot, s);
     // tracer residual #2 at (u_pred, v_pred, T_pred, S_pred) -> T_new
     zm::launch_fz(g, up, vp, Fz2, s);
@@ -412,6 +432,7 @@ struct Solver : SolverBase {
         zm::launch_copy(Ti, T3, n3, s);
         zm::launch_copy(Si, S3, n3, s);
         barotropic(P, ei, s);
+        zm::launch_cap_means(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, s);
         zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, ui, vi, Ti, Si, (real*)out.u,
                          (real*)out.v, (real*)out.T, (real*)out.S, s);
         zm::launch_copy((const real*)etaw, (real*)out.eta, n2, s);
@@ -428,7 +449,7 @@ struct Solver : SolverBase {
         zm::launch_bt_rho(g, P, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
                           (real*)nullptr, s);
         zm::launch_mom_adv(g, ui, vi, Fz1, au, av, s);
-        zm::launch_dealias_x(nx, N1, N2, keep, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+        zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
         zm::launch_mom_residual(g, P, 0, axu, axv, ui, vi, pbc, ei, Frx, Fry, (real*)out.u, (real*)out.v, tmpA,
                                 tmpB, decay_bot, s);
         break;
@@ -437,6 +458,8 @@ struct Solver : SolverBase {
     }
     ZM_HIP(hipGetLastError());
   }
+
+  int nu_substeps_per_launch() const override { return nu_block; }
 
   double time_kernel(int which, int reps) override {
     hipEvent_t a, b;
@@ -457,7 +480,32 @@ struct Solver : SolverBase {
                                   s);
           break;
         case 3:
-          zm::launch_dealias_x(nx, N1, N2, keep, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+          zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+          break;
+        case 5:
+          zm::launch_nu_block(g, (r % 2) ? nBu : nAu, (r % 2) ? nBv : nAv, (r % 2) ? nAu : nBu,
+                              (r % 2) ? nAv : nBv, P.nu_h, dts, std::max(1, nu_block), s);
+          break;
+        case 6:
+          zm::launch_copy((const real*)((r % 2) ? tmpB : tmpA), (r % 2) ? tmpA : tmpB, n3, s);
+          break;
+        case 7:
+          zm::launch_linear_finish(g, P, u1, v1, T1, S1, nAu, nAv, Lu, Lv, DT, DS, u3, v3, T3, S3, s);
+          break;
+        case 8:
+          zm::launch_conv(g, P, T1, S1, cT, cS, gate, s);
+          break;
+        case 9:
+          zm::launch_fz(g, u1, v1, Fz1, s);
+          break;
+        case 10:
+          zm::launch_d1_unit(g, T1, S1, DT, DS, s);
+          break;
+        case 11:
+          zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
+          break;
+        case 12:
+          zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, u1, v1, T1, S1, tmpA, tmpB, u2, v2, s);
           break;
         case 4:
           zm::launch_bt_eta(g, P, etaw, ubt, vbt, dtH, s);
@@ -492,7 +540,8 @@ void validate(const zm_config& c) {
   if (c.polar_cap_rows < 0 || c.polar_cap_taper < 0) bad("polar cap rows must be >= 0");
   if (c.polar_cap_rows > 0 && 2 * (c.polar_cap_rows + c.polar_cap_taper) > c.ny)
     bad("polar cap bands must not overlap");
-  const void* req[] = {c.cos_lat, c.dx_row, c.f_row, c.dz_node, c.dz_iface, c.dz, c.dz_norm, c.nwet, c.wet_mask};
+  const void* req[] = {c.cos_lat, c.dx_row, c.f_row, c.dz_node, c.dz_iface, c.dz, c.dz_norm, c.nwet, c.wet_mask,
+                       c.dealias_mask};
   for (const void* p : req)
     if (!p) bad("missing required array");
   if (c.nz > 2 && (!c.d2z_hm || !c.d2z_hp || !c.d2z_denom || !c.dz_denom_interior)) bad("missing vertical arrays");
@@ -600,6 +649,8 @@ int zm_set_graph(zm_handle* h, int32_t enabled) {
   h->solver->use_graph = enabled != 0;
   return ZM_OK;
 }
+
+int32_t zm_nu_substeps_per_launch(const zm_handle* h) { return h ? h->solver->nu_substeps_per_launch() : 0; }
 
 int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch) {
   if (!h || reps < 1 || !ms_per_launch) return fail(h, "invalid zm_time_kernel arguments", ZM_EINVAL);

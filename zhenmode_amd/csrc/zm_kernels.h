@@ -10,6 +10,11 @@ template <typename real>
 void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,
                        real* lv, real nu_h, real dts, hipStream_t s);
 template <typename real>
+int nu_block_max();
+template <typename real>
+void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
+                     int nsub, hipStream_t s);
+template <typename real>
 void launch_d1_unit(const Geo<real>& g, const real* T, const real* S, real* dT, real* dS, hipStream_t s);
 template <typename real>
 void launch_linear_finish(const Geo<real>& g, const Phys<real>& P, const real* u0, const real* v0,
@@ -36,7 +41,7 @@ template <typename real>
 void launch_mom_adv(const Geo<real>& g, const real* u, const real* v, const real* Fz, real* au, real* av,
                     hipStream_t s);
 template <typename real>
-void launch_dealias_x(int N, int N1, int N2, int keep, int nrows, const real* twr, const real* twi,
+void launch_dealias_x(int N, int N1, int N2, const int* kept, int nrows, const real* twr, const real* twi,
                       const real* ina, const real* inb, real* outa, real* outb, hipStream_t s);
 template <typename real>
 void launch_mom_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* ax, const real* ay,

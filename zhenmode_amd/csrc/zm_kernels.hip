@@ -119,13 +119,28 @@ __device__ __forceinline__ void cell_of(long long c, int n2, int nx, int& i, int
   i = r - j * nx;
 }
 
-#define ZM_CELL_PROLOGUE                                                     \
-  const long long c = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x; \
-  if (c >= g.n3) return;                                                     \
-  int i, j, k;                                                               \
-  cell_of(c, g.n2, g.nx, i, j, k);                                           \
-  const Nb n = make_nb(g, i, j);                                             \
-  const long long plane = (long long)k * g.n2;
+// Pointwise 3-D kernels run on 64 x 4 (i x j) tiles of one level, so the
+// north/south neighbour rows of a tile's interior come from the same
+// workgroup's L1; tiles are numbered i-fastest and mapped XCD-contiguously.
+constexpr int kTileI = 64, kTileJ = 4;
+
+#define ZM_TILE_COORDS                                                      \
+  const int ntx_ = (g.nx + kTileI - 1) / kTileI;                            \
+  const int nty_ = (g.ny + kTileJ - 1) / kTileJ;                            \
+  long long t_ = xcd_tile(blockIdx.x, gridDim.x);                           \
+  const int tx_ = (int)(t_ % ntx_);                                         \
+  t_ /= ntx_;                                                               \
+  const int ty_ = (int)(t_ % nty_);                                         \
+  const int k = (int)(t_ / nty_);                                           \
+  const int i = tx_ * kTileI + (int)(threadIdx.x % kTileI);                 \
+  const int j = ty_ * kTileJ + (int)(threadIdx.x / kTileI);                 \
+  if (i >= g.nx || j >= g.ny) return;                                       \
+  const long long plane = (long long)k * g.n2;                              \
+  const long long c = plane + (long long)j * g.nx + i;
+
+#define ZM_CELL_PROLOGUE \
+  ZM_TILE_COORDS         \
+  const Nb n = make_nb(g, i, j);
 
 // ---------------------------------------------------------------------------
 // nu_h sub-cycle: one explicit Laplacian substep of u and v
@@ -147,6 +162,188 @@ __global__ __launch_bounds__(256) void k_nu_substep(Geo<real> g, const real* __r
   }
 }
 
+// Cross-lane moves (DPP wave shifts): lane l receives lane l+1 (from_right)
+// or lane l-1 (from_left); lanes shifted in from outside the wave read 0.
+__device__ __forceinline__ int dpp_from_right(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_from_left(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ float dpp_from_right(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
+__device__ __forceinline__ float dpp_from_left(float x) { return __int_as_float(dpp_from_left(__float_as_int(x))); }
+__device__ __forceinline__ double dpp_from_right(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_from_right((int)(b & 0xffffffffll)), hi = dpp_from_right((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double dpp_from_left(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_from_left((int)(b & 0xffffffffll)), hi = dpp_from_left((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ unsigned long long dpp_from_right(unsigned long long x) {
+  const int lo = dpp_from_right((int)(x & 0xffffffffull)), hi = dpp_from_right((int)(x >> 32));
+  return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
+}
+__device__ __forceinline__ unsigned long long dpp_from_left(unsigned long long x) {
+  const int lo = dpp_from_left((int)(x & 0xffffffffull)), hi = dpp_from_left((int)(x >> 32));
+  return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
+}
+
+__device__ __forceinline__ float readlane(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double readlane(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Temporally blocked nu_h sub-cycle (register strips).  One wave owns a tile
+// of 64 longitudes x H latitude rows of one level; lane l holds its column's
+// H values of u and v in registers, east/west neighbours come from the
+// adjacent lanes (DPP), north/south from the neighbouring registers.  `nsub`
+// (<= HALO) substeps run per launch; each substep invalidates one more cell
+// at every tile edge, so the (64 - 2 HALO) x (H - 2 HALO) centre is exact.
+// Every node receives exactly the arithmetic of lap_h / k_nu_substep; rows
+// whose 64 nodes have all four faces open skip the face selects (the
+// expression is the same one with every gate true).
+template <typename real>
+__device__ __forceinline__ real lap_strip(real x, real xe, real xw, real xn, real xs, bool fe_on, bool fw_on,
+                                          bool fn_on, bool fs_on, real id2, real idy2, real hidy, real tr) {
+  const real fe = fe_on ? (xe - x) : real(0), fw = fw_on ? (x - xw) : real(0);
+  const real fn = fn_on ? (xn - x) : real(0), fs = fs_on ? (x - xs) : real(0);
+  return (fe - fw) * id2 + (fn - fs) * idy2 + (-((xn - xs) * hidy) * tr);
+}
+
+template <typename real>
+__device__ __forceinline__ real lap_open(real x, real xe, real xw, real xn, real xs, real id2, real idy2,
+                                         real hidy, real tr) {
+  return ((xe - x) - (x - xw)) * id2 + ((xn - x) - (x - xs)) * idy2 + (-((xn - xs) * hidy) * tr);
+}
+
+template <typename real, int H, int HALO>
+__global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __restrict__ u,
+                                                  const real* __restrict__ v, real* __restrict__ uo,
+                                                  real* __restrict__ vo, real nu_h, real dts, int nsub) {
+  constexpr int OX = 64 - 2 * HALO, OY = H - 2 * HALO;
+  static_assert(H <= 32, "wet bits are kept in one 32-bit word");
+  const int lane = threadIdx.x & 63;
+  const int ntx = (g.nx + OX - 1) / OX, nty = (g.ny + OY - 1) / OY;
+  const long long ntiles = (long long)ntx * nty * g.nz;
+  long long t = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (t >= ntiles) return;  // uniform per wave
+  // the tile index is uniform per wave: keep it (and what derives from it) in SGPRs
+  const int tx = __builtin_amdgcn_readfirstlane((int)(t % ntx));
+  t /= ntx;
+  const int ty = __builtin_amdgcn_readfirstlane((int)(t % nty));
+  const int k = __builtin_amdgcn_readfirstlane((int)(t / nty));
+  const int i0 = tx * OX - HALO, j0 = ty * OY - HALO;
+  int i = (i0 + lane) % g.nx;
+  if (i < 0) i += g.nx;
+  const long long plane = (long long)k * g.n2;
+  const real* __restrict__ up = u + plane;  // uniform bases: 32-bit per-lane offsets
+  const real* __restrict__ vp = v + plane;
+  real U[H], V[H];
+  int NW[H];
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    int j = j0 + r;
+    j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);  // out-of-domain rows read a copy; never used by valid nodes
+    const int c2 = j * g.nx + i;
+    U[r] = up[c2];
+    V[r] = vp[c2];
+    NW[r] = g.nwet[c2];
+  }
+  unsigned int mc = 0, wallN = 0, wallS = 0, inside = 0;
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    const int j = j0 + r;
+    const bool in = j >= 0 && j < g.ny;
+    if (in && k < NW[r]) mc |= 1u << r;
+    if (in) inside |= 1u << r;
+    if (j == g.ny - 1) wallN |= 1u << r;
+    if (j == 0) wallS |= 1u << r;
+  }
+  // face gates per row bit: east/west from the neighbouring lanes, north/south
+  // from the neighbouring rows (the wall row is its own mirror neighbour)
+  unsigned int me = mc & (unsigned int)dpp_from_right((int)mc);
+  unsigned int mw = mc & (unsigned int)dpp_from_left((int)mc);
+  unsigned int mn = mc & ((mc >> 1) | wallN);
+  unsigned int ms = mc & ((mc << 1) | wallS);
+  // rows where every interior lane has all four faces open (uniform)
+  unsigned int notfull = (lane == 0 || lane == 63) ? 0u : ~(me & mw & mn & ms);
+  for (int o = 1; o < 64; o <<= 1) notfull |= (unsigned int)__shfl_xor((int)notfull, o, 64);
+  unsigned int open_rows = __builtin_amdgcn_readfirstlane(~notfull) & inside;
+  wallN = __builtin_amdgcn_readfirstlane(wallN);
+  wallS = __builtin_amdgcn_readfirstlane(wallS);
+  const real idy2 = g.idy2, hidy = g.hidy;
+  // lane r holds the metrics of strip row r (read back with readlane)
+  real lane_idx2, lane_tan;
+  {
+    int j = j0 + (lane < H ? lane : H - 1);
+    j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
+    lane_idx2 = g.idx2[j];
+    lane_tan = g.tanR[j];
+  }
+  for (int s = 0; s < nsub; ++s) {
+    // keep one copy of the row metrics instead of H hoisted scalars
+    asm volatile("" : "+v"(lane_idx2), "+v"(lane_tan), "+v"(me), "+v"(mw), "+v"(mn), "+v"(ms), "+s"(wallN),
+                 "+s"(wallS), "+s"(open_rows));
+    real pu = U[0], pv = V[0];  // old values of row r-1
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const real id2 = readlane(lane_idx2, r), tr = readlane(lane_tan, r);
+      const bool atN = (wallN >> r) & 1u, atS = (wallS >> r) & 1u;
+      const int rn = r + 1 < H ? r + 1 : r;
+      // ordering barrier: keep each row's neighbour moves next to its update
+      asm volatile("" : "+v"(U[r]), "+v"(V[r]));
+      const real x = U[r], y = V[r];
+      const real xn = atN ? x : U[rn], xs = atS ? x : pu;
+      const real yn = atN ? y : V[rn], ys = atS ? y : pv;
+      const real xe = dpp_from_right(x), xw = dpp_from_left(x);
+      const real ye = dpp_from_right(y), yw = dpp_from_left(y);
+      real Lx, Ly;
+      if ((open_rows >> r) & 1u) {
+        Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
+        Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+      } else {
+        const bool fe_on = (me >> r) & 1u, fw_on = (mw >> r) & 1u;
+        const bool fn_on = (mn >> r) & 1u, fs_on = (ms >> r) & 1u;
+        Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+        Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+      }
+      pu = x;
+      pv = y;
+      U[r] = x + nu_h * Lx * dts;
+      V[r] = y + nu_h * Ly * dts;
+    }
+  }
+  const int io = tx * OX + (lane - HALO);
+  real* __restrict__ uop = uo + plane;
+  real* __restrict__ vop = vo + plane;
+  if (lane >= HALO && lane < 64 - HALO && io < g.nx) {
+#pragma unroll
+    for (int r = HALO; r < H - HALO; ++r) {
+      const int j = j0 + r;
+      if (j < g.ny) {
+        const int o = j * g.nx + io;
+        uop[o] = U[r];
+        vop[o] = V[r];
+      }
+    }
+  }
+}
+
+template <typename real>
+struct NuTile;
+template <>
+struct NuTile<float> {
+  static constexpr int H = 32, HALO = 4;
+};
+template <>
+struct NuTile<double> {
+  static constexpr int H = 20, HALO = 4;
+};
+
 // Unit-coefficient D1 of T and S (first half of _horizontal_biharmonic_tracer).
 template <typename real>
 __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __restrict__ T,
@@ -155,6 +352,80 @@ __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __rest
   ZM_CELL_PROLOGUE
   dT[c] = d1_flux(T, plane, n, k, g, (const real*)nullptr);
   dS[c] = d1_flux(S, plane, n, k, g, (const real*)nullptr);
+}
+
+
+// ---------------------------------------------------------------------------
+// Column-marching form.  A thread owns one (i, j) column over a segment of
+// kSeg levels; vertical neighbours come from a register window, the column's
+// 2-D data (wet counts, coefficients, metrics, forcing) is loaded once, and
+// each level only loads its horizontal neighbours.  The operators below take
+// explicit neighbour values and evaluate the same expressions as lap_h,
+// d1_flux and d2dz2.
+// ---------------------------------------------------------------------------
+constexpr int kSeg = 10;
+
+#define ZM_COLUMN_PROLOGUE                                                   \
+  const int ntx_ = (g.nx + kTileI - 1) / kTileI;                             \
+  const int nty_ = (g.ny + kTileJ - 1) / kTileJ;                             \
+  long long t_ = xcd_tile(blockIdx.x, gridDim.x);                            \
+  const int tx_ = (int)(t_ % ntx_);                                          \
+  t_ /= ntx_;                                                                \
+  const int ty_ = (int)(t_ % nty_);                                          \
+  const int seg_ = (int)(t_ / nty_);                                         \
+  const int i = tx_ * kTileI + (int)(threadIdx.x % kTileI);                  \
+  const int j = ty_ * kTileJ + (int)(threadIdx.x / kTileI);                  \
+  if (i >= g.nx || j >= g.ny) return;                                        \
+  const int k0 = seg_ * kSeg;                                                \
+  const int k1 = k0 + kSeg < g.nz ? k0 + kSeg : g.nz;                        \
+  const Nb n = make_nb(g, i, j);                                             \
+  const int c2 = n.c2;                                                       \
+  const long long n2 = g.n2;                                                 \
+  const int kb = n.nwC > 0 ? n.nwC - 1 : 0;
+
+template <typename real>
+__device__ __forceinline__ real lap_vals(real c, real e, real w, real nn, real s, bool ge, bool gw, bool gn,
+                                         bool gs, real idx2, real idy2, real hidy, real tr) {
+  const real fe = ge ? (e - c) : real(0), fw = gw ? (c - w) : real(0);
+  const real fn = gn ? (nn - c) : real(0), fs = gs ? (c - s) : real(0);
+  return (fe - fw) * idx2 + (fn - fs) * idy2 + (-((nn - s) * hidy) * tr);
+}
+
+template <typename real>
+__device__ __forceinline__ real d1_vals(real c, real e, real w, real nn, real s, bool ge, bool gw, bool gn,
+                                        bool gs, real kfe, real kfw, real kfn, real kfs, real idx, real idy,
+                                        real cfN, real cfS, real idyc) {
+  const real zE = ge ? kfe * (e - c) * idx : real(0);
+  const real zW = gw ? kfw * (c - w) * idx : real(0);
+  const real mN = gn ? kfn * (nn - c) * idy * cfN : real(0);
+  const real mS = gs ? kfs * (c - s) * idy * cfS : real(0);
+  return (zE - zW) * idx + (mN - mS) * idyc;
+}
+
+// node-form d2/dz2 with the seafloor ghost fill, from the window (k-1, k, k+1);
+// ghost nodes below the deepest wet level (k > kb) return 0.
+template <typename real>
+__device__ __forceinline__ real d2z_win(real xm, real xc, real xp, int k, int kb, const Geo<real>& g) {
+  if (k == 0) return kb >= 1 ? real(2) * (xp - xc) * g.ih0top2 : real(0);
+  if (k == g.nz - 1) return kb >= g.nz - 1 ? real(2) * (xm - xc) * g.ih0bot2 : real(0);
+  if (k > kb) return real(0);
+  const real up = k < kb ? xp : xc;
+  return (up * g.hm[k - 1] + xm * g.hp[k - 1] - xc * g.hmp[k - 1]) * g.iden[k - 1];
+}
+
+// one-sided/centered d/dz of the ghost-filled column (numerics/vertical.py:267-272)
+template <typename real>
+__device__ __forceinline__ real ddz_win(real xm, real xc, real xp, int k, int kb, const Geo<real>& g) {
+  if (k == 0) return ((kb >= 1 ? xp : xc) - xc) * g.idz_top;
+  if (k == g.nz - 1) return kb >= g.nz - 1 ? (xc - xm) * g.idz_bot : real(0);
+  if (k > kb) return real(0);
+  return ((k < kb ? xp : xc) - xm) * g.iddz[k - 1];
+}
+
+template <typename real>
+static inline unsigned segs3(const Geo<real>& g) {
+  return (unsigned)((long long)((g.nx + kTileI - 1) / kTileI) * ((g.ny + kTileJ - 1) / kTileJ) *
+                    ((g.nz + kSeg - 1) / kSeg));
 }
 
 // Remainder of _linear_half_step after the nu_h sub-cycle (step.py:111-173):
@@ -166,37 +437,65 @@ __global__ __launch_bounds__(256) void k_linear_finish(
     const real* __restrict__ vnu, const real* __restrict__ Lu0, const real* __restrict__ Lv0,
     const real* __restrict__ DT0, const real* __restrict__ DS0, real* __restrict__ uo,
     real* __restrict__ vo, real* __restrict__ To, real* __restrict__ So) {
-  ZM_CELL_PROLOGUE
+  ZM_COLUMN_PROLOGUE
   const real dth = P.dth;
-  real u = unu[c], v = vnu[c];
-  if (P.nu_bi > real(0)) {
-    u = u - P.nu_bi * lap_h(Lu0, plane, n, k, g) * dth;
-    v = v - P.nu_bi * lap_h(Lv0, plane, n, k, g) * dth;
-  }
-  u = u + P.nu_v * d2dz2(u0, n.c2, g.n2, k, n.nwC, g) * dth;
-  v = v + P.nu_v * d2dz2(v0, n.c2, g.n2, k, n.nwC, g) * dth;
-  const real t0 = T0[c], s0 = S0[c];
-  real T = t0 + d1_flux(T0, plane, n, k, g, P.kh2) * dth;
-  real S = s0 + d1_flux(S0, plane, n, k, g, P.kh2) * dth;
-  if (P.kappa_bi > real(0)) {
-    T = T - P.kappa_bi * d1_flux(DT0, plane, n, k, g, (const real*)nullptr) * dth;
-    S = S - P.kappa_bi * d1_flux(DS0, plane, n, k, g, (const real*)nullptr) * dth;
-  }
-  const real kv = P.kv2[n.c2];
-  T = T + kv * d2dz2(T0, n.c2, g.n2, k, n.nwC, g) * dth;
-  S = S + kv * d2dz2(S0, n.c2, g.n2, k, n.nwC, g) * dth;
-  const bool wc = k < n.nwC;
-  if (!wc) {
-    u = u0[c];
-    v = v0[c];
-    T = t0;
-    S = s0;
-  }
+  const real idx = g.idx[j], idx2 = g.idx2[j], tr = g.tanR[j], idyc = g.idyc[j];
+  const real cfN = g.cfn[j], cfS = n.hasS ? g.cfn[j - 1] : real(0);
+  const real kc = P.kh2[c2];
+  const real kfe = real(0.5) * (kc + P.kh2[n.e2]), kfw = real(0.5) * (P.kh2[n.w2] + kc);
+  const real kfn = real(0.5) * (kc + P.kh2[n.n2e]), kfs = real(0.5) * (P.kh2[n.s2e] + kc);
+  const real kv = P.kv2[c2];
   const real cr = P.rot_c[j], sr = P.rot_s[j];
-  uo[c] = cr * u + sr * v;
-  vo[c] = -sr * u + cr * v;
-  To[c] = T;
-  So[c] = S;
+  const bool bu = P.nu_bi > real(0), bt = P.kappa_bi > real(0);
+  long long o = (long long)k0 * n2 + c2;
+  real um = k0 > 0 ? u0[o - n2] : real(0), uc = u0[o];
+  real vm = k0 > 0 ? v0[o - n2] : real(0), vc = v0[o];
+  real Tm = k0 > 0 ? T0[o - n2] : real(0), Tc = T0[o];
+  real Sm = k0 > 0 ? S0[o - n2] : real(0), Sc = S0[o];
+  for (int k = k0; k < k1; ++k, o += n2) {
+    const bool more = k + 1 < g.nz;
+    const real up = more ? u0[o + n2] : uc, vp = more ? v0[o + n2] : vc;
+    const real Tp = more ? T0[o + n2] : Tc, Sp = more ? S0[o + n2] : Sc;
+    const long long pl = o - c2;
+    const bool wc = k < n.nwC;
+    const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
+    const bool gnE = wc && k < n.nwN, gsE = wc && k < n.nwS;  // mirror ghost rows
+    const bool gn = n.hasN && gnE, gs = n.hasS && gsE;        // closed walls
+    real u = unu[o], v = vnu[o];
+    if (bu) {
+      u = u - P.nu_bi * lap_vals(Lu0[o], Lu0[pl + n.e2], Lu0[pl + n.w2], Lu0[pl + n.n2e], Lu0[pl + n.s2e], ge, gw, gnE,
+                                 gsE, idx2, g.idy2, g.hidy, tr) * dth;
+      v = v - P.nu_bi * lap_vals(Lv0[o], Lv0[pl + n.e2], Lv0[pl + n.w2], Lv0[pl + n.n2e], Lv0[pl + n.s2e], ge, gw, gnE,
+                                 gsE, idx2, g.idy2, g.hidy, tr) * dth;
+    }
+    u = u + P.nu_v * d2z_win(um, uc, up, k, kb, g) * dth;
+    v = v + P.nu_v * d2z_win(vm, vc, vp, k, kb, g) * dth;
+    real T = Tc + d1_vals(Tc, T0[pl + n.e2], T0[pl + n.w2], T0[pl + n.n2e], T0[pl + n.s2e], ge, gw, gn, gs, kfe, kfw,
+                          kfn, kfs, idx, g.idy, cfN, cfS, idyc) * dth;
+    real S = Sc + d1_vals(Sc, S0[pl + n.e2], S0[pl + n.w2], S0[pl + n.n2e], S0[pl + n.s2e], ge, gw, gn, gs, kfe, kfw,
+                          kfn, kfs, idx, g.idy, cfN, cfS, idyc) * dth;
+    if (bt) {
+      const real one = real(1);
+      T = T - P.kappa_bi * d1_vals(DT0[o], DT0[pl + n.e2], DT0[pl + n.w2], DT0[pl + n.n2e], DT0[pl + n.s2e], ge, gw,
+                                   gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
+      S = S - P.kappa_bi * d1_vals(DS0[o], DS0[pl + n.e2], DS0[pl + n.w2], DS0[pl + n.n2e], DS0[pl + n.s2e], ge, gw,
+                                   gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
+    }
+    T = T + kv * d2z_win(Tm, Tc, Tp, k, kb, g) * dth;
+    S = S + kv * d2z_win(Sm, Sc, Sp, k, kb, g) * dth;
+    if (!wc) {
+      u = uc;
+      v = vc;
+      T = Tc;
+      S = Sc;
+    }
+    uo[o] = cr * u + sr * v;
+    vo[o] = -sr * u + cr * v;
+    To[o] = T;
+    So[o] = S;
+    um = uc; uc = up; vm = vc; vc = vp;
+    Tm = Tc; Tc = Tp; Sm = Sc; Sc = Sp;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -356,51 +655,90 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
     const real* __restrict__ cT, const real* __restrict__ cS, const int* __restrict__ gate,
     real* __restrict__ dT1, real* __restrict__ dS1, const real* __restrict__ Tb,
     const real* __restrict__ Sb, real* __restrict__ To, real* __restrict__ So) {
-  ZM_CELL_PROLOGUE
-  const bool wc = k < n.nwC;
-  const real advT = adv_scalar(T, u, v, Fz, plane, n, k, g);
-  const real advS = adv_scalar(S, u, v, Fz, plane, n, k, g);
-  const real dhT = d1_flux(T, plane, n, k, g, P.kh2);
-  const real dhS = d1_flux(S, plane, n, k, g, P.kh2);
-  const real kv = P.kv2[n.c2];
-  const real dvT = kv * d2dz2(T, n.c2, g.n2, k, n.nwC, g);
-  const real dvS = kv * d2dz2(S, n.c2, g.n2, k, n.nwC, g);
-  const bool gated = gate[n.c2] != 0;
-  const real convT = gated ? cT[c] : real(0);
-  const real convS = gated ? cS[c] : real(0);
-  real dTdt = 0, dSdt = 0;
-  if (wc) {
-    dTdt = advT + dhT + dvT;
-    dSdt = advS + dhS + dvS;
-    if (k == 0) {
-      const real sst = T[n.c2], sss = S[n.c2];
-      const real hf = P.heat_factor;
-      const real ta = P.t_atm[n.c2];
-      dTdt = dTdt + P.q_heat[n.c2] * hf;
-      dTdt = dTdt + P.lambda_bulk * (ta - sst) * hf;
-      dTdt = dTdt + P.cbl[n.c2] * (ta - sst) * hf;
-      dTdt = dTdt + convT;
-      dTdt = dTdt + P.crc[n.c2] * (P.crT[n.c2] - sst);
-      dSdt = dSdt + convS;
-      dSdt = dSdt + P.restore_coef_S * (P.s_ref2[n.c2] - sss);
-      dSdt = dSdt + P.ice_salt_flux * (sst <= P.ice_freeze ? real(1) : real(0));
-    } else {
-      dTdt = dTdt + convT;
-      dSdt = dSdt + convS;
+  ZM_COLUMN_PROLOGUE
+  const real idx = g.idx[j], idyc = g.idyc[j];
+  const real cfN = g.cfn[j], cfS = n.hasS ? g.cfn[j - 1] : real(0);
+  const real kc = P.kh2[c2];
+  const real kfe = real(0.5) * (kc + P.kh2[n.e2]), kfw = real(0.5) * (P.kh2[n.w2] + kc);
+  const real kfn = real(0.5) * (kc + P.kh2[n.n2e]), kfs = real(0.5) * (P.kh2[n.s2e] + kc);
+  const real kv = P.kv2[c2];
+  const bool gated = gate[c2] != 0;
+  const real h = real(0.5);
+  long long o = (long long)k0 * n2 + c2;
+  real Tm = k0 > 0 ? T[o - n2] : real(0), Tc = T[o];
+  real Sm = k0 > 0 ? S[o - n2] : real(0), Sc = S[o];
+  real fz0 = Fz[o];
+  for (int k = k0; k < k1; ++k, o += n2) {
+    const bool more = k + 1 < g.nz;
+    const real Tp = more ? T[o + n2] : Tc, Sp = more ? S[o + n2] : Sc;
+    const real fz1 = Fz[o + n2];
+    const long long pl = o - c2;
+    const bool wc = k < n.nwC;
+    const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
+    const bool gn = n.hasN && wc && k < n.nwN, gs = n.hasS && wc && k < n.nwS;
+    const real TE = T[pl + n.e2], TW = T[pl + n.w2], TN = T[pl + n.n2e], TS = T[pl + n.s2e];
+    const real SE = S[pl + n.e2], SW = S[pl + n.w2], SN = S[pl + n.n2e], SS = S[pl + n.s2e];
+    const real dhT = d1_vals(Tc, TE, TW, TN, TS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
+    const real dhS = d1_vals(Sc, SE, SW, SN, SS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
+    const real dvT = kv * d2z_win(Tm, Tc, Tp, k, kb, g);
+    const real dvS = kv * d2z_win(Sm, Sc, Sp, k, kb, g);
+    real dTdt = 0, dSdt = 0;
+    if (wc) {
+      // centred face-gated horizontal fluxes (transport.py:224-290)
+      const real uc = u[o], vc = v[o];
+      const real ufe = ge ? h * (uc + u[pl + n.e2]) : real(0);
+      const real ufw = gw ? h * (u[pl + n.w2] + uc) : real(0);
+      const real vfn = gn ? h * (vc + v[pl + n.n2e]) : real(0);
+      const real vfs = gs ? h * (v[pl + n.s2e] + vc) : real(0);
+      const real hT = (ufe * (h * (Tc + TE)) - ufw * (h * (TW + Tc))) * idx +
+                      (vfn * (h * (Tc + TN)) * cfN - vfs * (h * (TS + Tc)) * cfS) * idyc;
+      const real hS = (ufe * (h * (Sc + SE)) - ufw * (h * (SW + Sc))) * idx +
+                      (vfn * (h * (Sc + SN)) * cfN - vfs * (h * (SS + Sc)) * cfS) * idyc;
+      // donor-cell vertical flux on Fz (transport.py:444-475)
+      const real upT = k == 0 ? fz0 * Tc : fz0 * (fz0 > real(0) ? Tm : Tc);
+      const real upS = k == 0 ? fz0 * Sc : fz0 * (fz0 > real(0) ? Sm : Sc);
+      real dnT = 0, dnS = 0;
+      if (k + 1 < n.nwC) {
+        dnT = fz1 * (fz1 > real(0) ? Tc : Tp);
+        dnS = fz1 * (fz1 > real(0) ? Sc : Sp);
+      }
+      const real advT = -(hT + (dnT - upT) * g.idzn[k]);
+      const real advS = -(hS + (dnS - upS) * g.idzn[k]);
+      const real convT = gated ? cT[o] : real(0);
+      const real convS = gated ? cS[o] : real(0);
+      dTdt = advT + dhT + dvT;
+      dSdt = advS + dhS + dvS;
+      if (k == 0) {
+        const real hf = P.heat_factor;
+        const real ta = P.t_atm[c2];
+        dTdt = dTdt + P.q_heat[c2] * hf;
+        dTdt = dTdt + P.lambda_bulk * (ta - Tc) * hf;
+        dTdt = dTdt + P.cbl[c2] * (ta - Tc) * hf;
+        dTdt = dTdt + convT;
+        dTdt = dTdt + P.crc[c2] * (P.crT[c2] - Tc);
+        dSdt = dSdt + convS;
+        dSdt = dSdt + P.restore_coef_S * (P.s_ref2[c2] - Sc);
+        dSdt = dSdt + P.ice_salt_flux * (Tc <= P.ice_freeze ? real(1) : real(0));
+      } else {
+        dTdt = dTdt + convT;
+        dSdt = dSdt + convS;
+      }
     }
-  }
-  dTdt = dTdt - dhT;
-  dSdt = dSdt - dhS;
-  dTdt = dTdt - dvT;
-  dSdt = dSdt - dvS;
-  if (mode == 0) {
-    dT1[c] = dTdt;
-    dS1[c] = dSdt;
-    To[c] = T[c] + dTdt * P.dt;
-    So[c] = S[c] + dSdt * P.dt;
-  } else {
-    To[c] = Tb[c] + real(0.5) * (dT1[c] + dTdt) * P.dt;
-    So[c] = Sb[c] + real(0.5) * (dS1[c] + dSdt) * P.dt;
+    dTdt = dTdt - dhT;
+    dSdt = dSdt - dhS;
+    dTdt = dTdt - dvT;
+    dSdt = dSdt - dvS;
+    if (mode == 0) {
+      dT1[o] = dTdt;
+      dS1[o] = dSdt;
+      To[o] = Tc + dTdt * P.dt;
+      So[o] = Sc + dSdt * P.dt;
+    } else {
+      To[o] = Tb[o] + real(0.5) * (dT1[o] + dTdt) * P.dt;
+      So[o] = Sb[o] + real(0.5) * (dS1[o] + dSdt) * P.dt;
+    }
+    Tm = Tc; Tc = Tp; Sm = Sc; Sc = Sp;
+    fz0 = fz1;
   }
 }
 
@@ -507,17 +845,29 @@ template <typename real>
 __global__ __launch_bounds__(256) void k_mom_adv(Geo<real> g, const real* __restrict__ u,
                                                  const real* __restrict__ v, const real* __restrict__ Fz,
                                                  real* __restrict__ au, real* __restrict__ av) {
-  ZM_CELL_PROLOGUE
-  const bool wc = k < n.nwC;
-  const real w = wc ? -Fz[c] : real(0);
-  const real uc = u[c], vc = v[c];
-  real dudx, dudy, dvdx, dvdy;
-  grad_gated(u, plane, n, k, g, dudx, dudy);
-  grad_gated(v, plane, n, k, g, dvdx, dvdy);
-  const real dudz = d_dz_filled(u, n.c2, g.n2, k, n.nwC, g);
-  const real dvdz = d_dz_filled(v, n.c2, g.n2, k, n.nwC, g);
-  au[c] = -(uc * dudx + vc * dudy + w * dudz);
-  av[c] = -(uc * dvdx + vc * dvdy + w * dvdz);
+  ZM_COLUMN_PROLOGUE
+  const real hidx = real(0.5) * g.idx[j], hidy = real(0.5) * g.idy;
+  long long o = (long long)k0 * n2 + c2;
+  real um = k0 > 0 ? u[o - n2] : real(0), uc = u[o];
+  real vm = k0 > 0 ? v[o - n2] : real(0), vc = v[o];
+  for (int k = k0; k < k1; ++k, o += n2) {
+    const bool more = k + 1 < g.nz;
+    const real up = more ? u[o + n2] : uc, vp = more ? v[o + n2] : vc;
+    const long long pl = o - c2;
+    const bool wc = k < n.nwC;
+    const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
+    const bool gn = wc && k < n.nwN, gs = wc && k < n.nwS;
+    const real w = wc ? -Fz[o] : real(0);
+    const real dudx = hidx * ((ge ? u[pl + n.e2] - uc : real(0)) + (gw ? uc - u[pl + n.w2] : real(0)));
+    const real dudy = hidy * ((gn ? u[pl + n.n2e] - uc : real(0)) + (gs ? uc - u[pl + n.s2e] : real(0)));
+    const real dvdx = hidx * ((ge ? v[pl + n.e2] - vc : real(0)) + (gw ? vc - v[pl + n.w2] : real(0)));
+    const real dvdy = hidy * ((gn ? v[pl + n.n2e] - vc : real(0)) + (gs ? vc - v[pl + n.s2e] : real(0)));
+    const real dudz = ddz_win(um, uc, up, k, kb, g);
+    const real dvdz = ddz_win(vm, vc, vp, k, kb, g);
+    au[o] = -(uc * dudx + vc * dudy + w * dudz);
+    av[o] = -(uc * dvdx + vc * dvdy + w * dvdz);
+    um = uc; uc = up; vm = vc; vc = vp;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -525,10 +875,11 @@ __global__ __launch_bounds__(256) void k_mom_adv(Geo<real> g, const real* __rest
 // Re(IFFT(mask * FFT(row))).  The projection is real, so one complex row
 // z = a + i b carries the a row of u-advection and the b row of v-advection.
 // DFT of length N = N1 N2 by the four-step factorisation in LDS; the
-// twiddle table holds exp(-2 pi i e / N), e in [0, N).
+// twiddle table holds exp(-2 pi i e / N), e in [0, N); kept[k] != 0 marks the
+// retained wavenumbers (the mask of factory.py:529-533, computed on the host).
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ __launch_bounds__(256) void k_dealias_x(int N, int N1, int N2, int keep, int nrows,
+__global__ __launch_bounds__(256) void k_dealias_x(int N, int N1, int N2, const int* __restrict__ kept, int nrows,
                                                    const real* __restrict__ twr,
                                                    const real* __restrict__ twi,
                                                    const real* __restrict__ ina,
@@ -573,7 +924,7 @@ __global__ __launch_bounds__(256) void k_dealias_x(int N, int N1, int N2, int ke
   for (int t = threadIdx.x; t < N; t += blockDim.x) {
     const int k2 = t / N1, k1 = t - k2 * N1;
     real sr = 0, si = 0;
-    if (t <= keep || t >= N - keep) {
+    if (kept[t]) {
       const int step = N1 * k2;
       int e = 0;
       for (int n2 = 0; n2 < N2; ++n2) {
@@ -640,87 +991,109 @@ __global__ __launch_bounds__(256) void k_mom_residual(
     const real* __restrict__ eta, const real* __restrict__ Frx, const real* __restrict__ Fry,
     real* __restrict__ du1, real* __restrict__ dv1, real* __restrict__ uo, real* __restrict__ vo,
     real decay_bot) {
-  ZM_CELL_PROLOGUE
-  const bool wc = k < n.nwC;
-  const real uc = u[c], vc = v[c];
+  ZM_COLUMN_PROLOGUE
   const real f = g.frow[j];
+  const real idx = g.idx[j], idx2 = g.idx2[j], tr = g.tanR[j], idyc = g.idyc[j];
+  const real cfn = g.cfn[j], cfs = g.cfs[j];
   // y binomial [1,4,6,4,1]/16 with 2-row edge padding
   const int jm2 = j >= 2 ? j - 2 : 0, jm1 = j >= 1 ? j - 1 : 0;
   const int jp1 = j + 1 < g.ny ? j + 1 : g.ny - 1, jp2 = j + 2 < g.ny ? j + 2 : g.ny - 1;
-  const long long rb = plane + i;
-  auto binom = [&](const real* a) {
-    return (a[rb + (long long)jm2 * g.nx] + real(4) * a[rb + (long long)jm1 * g.nx] +
-            real(6) * a[rb + (long long)j * g.nx] + real(4) * a[rb + (long long)jp1 * g.nx] +
-            a[rb + (long long)jp2 * g.nx]) / real(16);
-  };
-  const real advu = wc ? binom(ax) : real(0);
-  const real advv = wc ? binom(ay) : real(0);
-  // hydrostatic pressure p = rho0 g eta + p_bc at the five points
+  const int rm2 = (jm2 - j) * g.nx, rm1 = (jm1 - j) * g.nx, rp1 = (jp1 - j) * g.nx, rp2 = (jp2 - j) * g.nx;
+  // column constants: eta at the five points, 2-D forcing
   const real rg = real(kRho0 * kG);
-  const real pc = rg * eta[n.c2] + pbc[plane + n.c2];
-  const real pe = rg * eta[n.e2] + pbc[plane + n.e2];
-  const real pw = rg * eta[n.w2] + pbc[plane + n.w2];
-  const real pn = n.hasN ? rg * eta[n.c2 + g.nx] + pbc[plane + n.c2 + g.nx] : real(0);
-  const real ps = n.hasS ? rg * eta[n.c2 - g.nx] + pbc[plane + n.c2 - g.nx] : real(0);
-  real gpx, gpy;
-  grad_cons(pc, pe, pw, pn, ps, n, k, g, gpx, gpy);
-  const real pgfx = -gpx / real(kRho0), pgfy = -gpy / real(kRho0);
-  const real lapu = lap_h(u, plane, n, k, g), lapv = lap_h(v, plane, n, k, g);
-  const real dvu = P.nu_v * d2dz2(u, n.c2, g.n2, k, n.nwC, g);
-  const real dvv = P.nu_v * d2dz2(v, n.c2, g.n2, k, n.nwC, g);
-  real dudt = 0, dvdt = 0;
-  if (wc) {
-    dudt = advu + f * vc + pgfx + P.nu_h * lapu + dvu;
-    dvdt = advv + (-f) * uc + pgfy + P.nu_h * lapv + dvv;
-    if (k == 0) {
-      dudt = dudt + P.tau_x[n.c2] * P.wind_factor;
-      dvdt = dvdt + P.tau_y[n.c2] * P.wind_factor;
+  const real eC = eta[c2], eE = eta[n.e2], eW = eta[n.w2];
+  const real eN = n.hasN ? eta[c2 + g.nx] : real(0), eS = n.hasS ? eta[c2 - g.nx] : real(0);
+  const real frx = Frx[c2], fry = Fry[c2];
+  const real tx = P.tau_x[c2], ty = P.tau_y[c2];
+  const real sfx = tx * P.wind_factor, sfy = ty * P.wind_factor;
+  const real bfx = tx * P.inv_rho0H, bfy = ty * P.inv_rho0H;
+  const real half = real(0.5);
+  long long o = (long long)k0 * n2 + c2;
+  real um = k0 > 0 ? u[o - n2] : real(0), uc = u[o];
+  real vm = k0 > 0 ? v[o - n2] : real(0), vc = v[o];
+  for (int k = k0; k < k1; ++k, o += n2) {
+    const bool more = k + 1 < g.nz;
+    const real up = more ? u[o + n2] : uc, vp = more ? v[o + n2] : vc;
+    const long long pl = o - c2;
+    const bool wc = k < n.nwC;
+    const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
+    const bool gnE = wc && k < n.nwN, gsE = wc && k < n.nwS;
+    const bool gn = n.hasN && gnE, gs = n.hasS && gsE;
+    real advu = 0, advv = 0;
+    if (wc) {
+      advu = (ax[o + rm2] + real(4) * ax[o + rm1] + real(6) * ax[o] + real(4) * ax[o + rp1] + ax[o + rp2]) /
+             real(16);
+      advv = (ay[o + rm2] + real(4) * ay[o + rm1] + real(6) * ay[o] + real(4) * ay[o + rp1] + ay[o + rp2]) /
+             real(16);
     }
-    if (k == g.nz - 1) {
-      dudt = dudt + (-P.r_bot) * uc;
-      dvdt = dvdt + (-P.r_bot) * vc;
+    // hydrostatic pressure p = rho0 g eta + p_bc at the five points
+    const real pc = rg * eC + pbc[o];
+    const real pe = rg * eE + pbc[pl + n.e2];
+    const real pw = rg * eW + pbc[pl + n.w2];
+    const real pn = n.hasN ? rg * eN + pbc[o + g.nx] : real(0);
+    const real ps = n.hasS ? rg * eS + pbc[o - g.nx] : real(0);
+    const real gpx = idx * half * ((ge ? pe - pc : real(0)) + (gw ? pc - pw : real(0)));
+    const real gpy = idyc * half * ((gn ? (pn - pc) * cfn : real(0)) + (gs ? (pc - ps) * cfs : real(0)));
+    const real pgfx = -gpx / real(kRho0), pgfy = -gpy / real(kRho0);
+    const real lapu = lap_vals(uc, u[pl + n.e2], u[pl + n.w2], u[pl + n.n2e], u[pl + n.s2e], ge, gw, gnE, gsE, idx2,
+                               g.idy2, g.hidy, tr);
+    const real lapv = lap_vals(vc, v[pl + n.e2], v[pl + n.w2], v[pl + n.n2e], v[pl + n.s2e], ge, gw, gnE, gsE, idx2,
+                               g.idy2, g.hidy, tr);
+    const real dvu = P.nu_v * d2z_win(um, uc, up, k, kb, g);
+    const real dvv = P.nu_v * d2z_win(vm, vc, vp, k, kb, g);
+    real dudt = 0, dvdt = 0;
+    if (wc) {
+      dudt = advu + f * vc + pgfx + P.nu_h * lapu + dvu;
+      dvdt = advv + (-f) * uc + pgfy + P.nu_h * lapv + dvv;
+      if (k == 0) {
+        dudt = dudt + sfx;
+        dvdt = dvdt + sfy;
+      }
+      if (k == g.nz - 1) {
+        dudt = dudt + (-P.r_bot) * uc;
+        dvdt = dvdt + (-P.r_bot) * vc;
+      }
     }
-  }
-  dudt = dudt - P.nu_h * lapu;
-  dvdt = dvdt - P.nu_h * lapv;
-  dudt = dudt - dvu;
-  dvdt = dvdt - dvv;
-  dudt = dudt - f * vc;
-  dvdt = dvdt + f * uc;
-  // cancel the eta part with the same layer-gated operator
-  const real en = n.hasN ? eta[n.c2 + g.nx] : real(0);
-  const real es = n.hasS ? eta[n.c2 - g.nx] : real(0);
-  real gex, gey;
-  grad_cons(eta[n.c2], eta[n.e2], eta[n.w2], en, es, n, k, g, gex, gey);
-  dudt = dudt + real(kG) * gex;
-  dvdt = dvdt + real(kG) * gey;
-  dudt = dudt - Frx[n.c2];
-  dvdt = dvdt - Fry[n.c2];
-  dudt = dudt - P.tau_x[n.c2] * P.inv_rho0H;
-  dvdt = dvdt - P.tau_y[n.c2] * P.inv_rho0H;
-  if (!wc) {
-    dudt = 0;
-    dvdt = 0;
-  }
-  const bool bm = wc && k == g.nz - 1;
-  if (bm) {
-    dudt = dudt + P.r_bot * uc;
-    dvdt = dvdt + P.r_bot * vc;
-  }
-  if (mode == 0) {
-    du1[c] = dudt;
-    dv1[c] = dvdt;
-    uo[c] = uc + dudt * P.dt;
-    vo[c] = vc + dvdt * P.dt;
-  } else {
-    real un = uc + real(0.5) * (du1[c] + dudt) * P.dt;
-    real vn = vc + real(0.5) * (dv1[c] + dvdt) * P.dt;
+    dudt = dudt - P.nu_h * lapu;
+    dvdt = dvdt - P.nu_h * lapv;
+    dudt = dudt - dvu;
+    dvdt = dvdt - dvv;
+    dudt = dudt - f * vc;
+    dvdt = dvdt + f * uc;
+    // cancel the eta part with the same layer-gated operator
+    const real gex = idx * half * ((ge ? eE - eC : real(0)) + (gw ? eC - eW : real(0)));
+    const real gey = idyc * half * ((gn ? (eN - eC) * cfn : real(0)) + (gs ? (eC - eS) * cfs : real(0)));
+    dudt = dudt + real(kG) * gex;
+    dvdt = dvdt + real(kG) * gey;
+    dudt = dudt - frx;
+    dvdt = dvdt - fry;
+    dudt = dudt - bfx;
+    dvdt = dvdt - bfy;
+    if (!wc) {
+      dudt = 0;
+      dvdt = 0;
+    }
+    const bool bm = wc && k == g.nz - 1;
     if (bm) {
-      un = un * decay_bot;
-      vn = vn * decay_bot;
+      dudt = dudt + P.r_bot * uc;
+      dvdt = dvdt + P.r_bot * vc;
     }
-    uo[c] = un;
-    vo[c] = vn;
+    if (mode == 0) {
+      du1[o] = dudt;
+      dv1[o] = dvdt;
+      uo[o] = uc + dudt * P.dt;
+      vo[o] = vc + dvdt * P.dt;
+    } else {
+      real un = uc + real(0.5) * (du1[o] + dudt) * P.dt;
+      real vn = vc + real(0.5) * (dv1[o] + dvdt) * P.dt;
+      if (bm) {
+        un = un * decay_bot;
+        vn = vn * decay_bot;
+      }
+      uo[o] = un;
+      vo[o] = vn;
+    }
+    um = uc; uc = up; vm = vc; vc = vp;
   }
 }
 
@@ -919,10 +1292,7 @@ __global__ __launch_bounds__(256) void k_final(Geo<real> g, Phys<real> P, const 
                                                const real* __restrict__ Tin, const real* __restrict__ Sin,
                                                real* __restrict__ uo, real* __restrict__ vo,
                                                real* __restrict__ To, real* __restrict__ So) {
-  const long long c = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (c >= g.n3) return;
-  int i, j, k;
-  cell_of(c, g.n2, g.nx, i, j, k);
+  ZM_TILE_COORDS
   const int c2 = j * g.nx + i;
   const bool wc = k < g.nwet[c2];
   real u = u3[c] + (ubt[c2] - ubt0[c2]);
@@ -959,10 +1329,7 @@ __global__ __launch_bounds__(256) void k_diag(Geo<real> g, Phys<real> P, const r
                                               const real* __restrict__ pbc, const real* __restrict__ Fz,
                                               real* __restrict__ rho, real* __restrict__ pres,
                                               real* __restrict__ w) {
-  const long long c = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (c >= g.n3) return;
-  int i, j, k;
-  cell_of(c, g.n2, g.nx, i, j, k);
+  ZM_TILE_COORDS
   const int c2 = j * g.nx + i;
   rho[c] = real(kRho0) + rho_anom(T[c], S[c], P);
   pres[c] = real(kRho0 * kG) * eta[c2] + pbc[c];
@@ -981,15 +1348,35 @@ __global__ __launch_bounds__(256) void k_copy(const real* __restrict__ a, real* 
 static inline unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 template <typename real>
+static inline unsigned tiles3(const Geo<real>& g) {
+  return (unsigned)((long long)((g.nx + kTileI - 1) / kTileI) * ((g.ny + kTileJ - 1) / kTileJ) * g.nz);
+}
+
+template <typename real>
 void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,
                        real* lv, real nu_h, real dts, hipStream_t s) {
-  hipLaunchKernelGGL(k_nu_substep<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, u, v, uo, vo, lu,
+  hipLaunchKernelGGL(k_nu_substep<real>, dim3(tiles3(g)), dim3(256), 0, s, g, u, v, uo, vo, lu,
                      lv, nu_h, dts);
 }
 
 template <typename real>
+int nu_block_max() {
+  return NuTile<real>::HALO;
+}
+
+template <typename real>
+void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
+                     int nsub, hipStream_t s) {
+  using T = NuTile<real>;
+  constexpr int OX = 64 - 2 * T::HALO, OY = T::H - 2 * T::HALO;
+  const long long tiles = (long long)((g.nx + OX - 1) / OX) * ((g.ny + OY - 1) / OY) * g.nz;
+  hipLaunchKernelGGL((k_nu_strip<real, T::H, T::HALO>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, g, u,
+                     v, uo, vo, nu_h, dts, nsub);
+}
+
+template <typename real>
 void launch_d1_unit(const Geo<real>& g, const real* T, const real* S, real* dT, real* dS, hipStream_t s) {
-  hipLaunchKernelGGL(k_d1_unit<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, T, S, dT, dS);
+  hipLaunchKernelGGL(k_d1_unit<real>, dim3(tiles3(g)), dim3(256), 0, s, g, T, S, dT, dS);
 }
 
 template <typename real>
@@ -997,7 +1384,7 @@ void launch_linear_finish(const Geo<real>& g, const Phys<real>& P, const real* u
                           const real* T0, const real* S0, const real* unu, const real* vnu, const real* Lu0,
                           const real* Lv0, const real* DT0, const real* DS0, real* uo, real* vo, real* To,
                           real* So, hipStream_t s) {
-  hipLaunchKernelGGL(k_linear_finish<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, u0, v0, T0,
+  hipLaunchKernelGGL(k_linear_finish<real>, dim3(segs3(g)), dim3(256), 0, s, g, P, u0, v0, T0,
                      S0, unu, vnu, Lu0, Lv0, DT0, DS0, uo, vo, To, So);
 }
 
@@ -1018,7 +1405,7 @@ void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, c
                             const real* T, const real* S, const real* Fz, const real* cT, const real* cS,
                             const int* gate, real* dT1, real* dS1, const real* Tb, const real* Sb, real* To,
                             real* So, hipStream_t s) {
-  hipLaunchKernelGGL(k_tracer_residual<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, mode, u, v,
+  hipLaunchKernelGGL(k_tracer_residual<real>, dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
                      T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
 }
 
@@ -1038,14 +1425,14 @@ void launch_bt_rho(const Geo<real>& g, const Phys<real>& P, const real* pbc, rea
 template <typename real>
 void launch_mom_adv(const Geo<real>& g, const real* u, const real* v, const real* Fz, real* au, real* av,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_mom_adv<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, u, v, Fz, au, av);
+  hipLaunchKernelGGL(k_mom_adv<real>, dim3(segs3(g)), dim3(256), 0, s, g, u, v, Fz, au, av);
 }
 
 template <typename real>
-void launch_dealias_x(int N, int N1, int N2, int keep, int nrows, const real* twr, const real* twi,
+void launch_dealias_x(int N, int N1, int N2, const int* kept, int nrows, const real* twr, const real* twi,
                       const real* ina, const real* inb, real* outa, real* outb, hipStream_t s) {
   const size_t lds = (size_t)6 * N * sizeof(real);
-  hipLaunchKernelGGL(k_dealias_x<real>, dim3(nrows), dim3(256), lds, s, N, N1, N2, keep, nrows, twr, twi, ina,
+  hipLaunchKernelGGL(k_dealias_x<real>, dim3(nrows), dim3(256), lds, s, N, N1, N2, kept, nrows, twr, twi, ina,
                      inb, outa, outb);
 }
 
@@ -1054,7 +1441,7 @@ void launch_mom_residual(const Geo<real>& g, const Phys<real>& P, int mode, cons
                          const real* u, const real* v, const real* pbc, const real* eta, const real* Frx,
                          const real* Fry, real* du1, real* dv1, real* uo, real* vo, real decay_bot,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_mom_residual<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, mode, ax, ay, u,
+  hipLaunchKernelGGL(k_mom_residual<real>, dim3(segs3(g)), dim3(256), 0, s, g, P, mode, ax, ay, u,
                      v, pbc, eta, Frx, Fry, du1, dv1, uo, vo, decay_bot);
 }
 
@@ -1086,14 +1473,14 @@ void launch_final(const Geo<real>& g, const Phys<real>& P, const real* u3, const
                   const real* S3, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
                   const real* means, const real* uin, const real* vin, const real* Tin, const real* Sin,
                   real* uo, real* vo, real* To, real* So, hipStream_t s) {
-  hipLaunchKernelGGL(k_final<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, u3, v3, T3, S3, ubt,
+  hipLaunchKernelGGL(k_final<real>, dim3(tiles3(g)), dim3(256), 0, s, g, P, u3, v3, T3, S3, ubt,
                      ubt0, vbt, vbt0, means, uin, vin, Tin, Sin, uo, vo, To, So);
 }
 
 template <typename real>
 void launch_diag(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, const real* eta,
                  const real* pbc, const real* Fz, real* rho, real* pres, real* w, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag<real>, dim3(blocks_for(g.n3, 256)), dim3(256), 0, s, g, P, T, S, eta, pbc, Fz, rho,
+  hipLaunchKernelGGL(k_diag<real>, dim3(tiles3(g)), dim3(256), 0, s, g, P, T, S, eta, pbc, Fz, rho,
                      pres, w);
 }
 
@@ -1106,6 +1493,9 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
   template void launch_nu_substep<real>(const Geo<real>&, const real*, const real*, real*, real*, real*,    \
                                         real*, real, real, hipStream_t);                                      \
   template void launch_d1_unit<real>(const Geo<real>&, const real*, const real*, real*, real*, hipStream_t); \
+  template int nu_block_max<real>();                                                                         \
+  template void launch_nu_block<real>(const Geo<real>&, const real*, const real*, real*, real*, real, real,  \
+                                      int, hipStream_t);                                                     \
   template void launch_linear_finish<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,   \
                                            const real*, const real*, const real*, const real*, const real*, \
                                            const real*, const real*, const real*, real*, real*, real*,      \
@@ -1123,7 +1513,7 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                     const real*, const real*, real*, real*, hipStream_t);                    \
   template void launch_mom_adv<real>(const Geo<real>&, const real*, const real*, const real*, real*, real*,  \
                                      hipStream_t);                                                           \
-  template void launch_dealias_x<real>(int, int, int, int, int, const real*, const real*, const real*,      \
+  template void launch_dealias_x<real>(int, int, int, const int*, int, const real*, const real*, const real*, \
                                        const real*, real*, real*, hipStream_t);                              \
   template void launch_mom_residual<real>(const Geo<real>&, const Phys<real>&, int, const real*,            \
                                           const real*, const real*, const real*, const real*, const real*,  \

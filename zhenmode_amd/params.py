@@ -188,6 +188,10 @@ def freeze(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
     p.polar_cap_rows, p.polar_cap_taper = int(polar_cap_rows), int(polar_cap_taper)
     p.cap_weights = cap_weights(p.polar_cap_rows, p.polar_cap_taper)
     p.dealias_keep = max(1, int((nx // 2) * 2 / 3))
+    # Same float64 expression as the reference (factory.py:529-533): for some
+    # nx (90, 180, 360, 720) fftfreq(nx) * nx rounds the +-keep wavenumber just
+    # above keep, so that wavenumber is dropped; the mask is kept as computed.
+    p.dealias_mask = (np.abs(np.fft.fftfreq(nx) * nx) <= p.dealias_keep).astype(np.int32)
     p.mode_split = True
     p.n_subcyc = max(1, int(round(dt / dt_bt)))
     p.dt_bt = dt / p.n_subcyc
@@ -244,6 +248,9 @@ def make_config(p):
     c.d2z_hm, c.d2z_hp = dptr(p.d2z_hm), dptr(p.d2z_hp)
     c.d2z_denom, c.dz_denom_interior = dptr(p.d2z_denom), dptr(p.dz_denom_interior)
     c.cap_weights = dptr(p.cap_weights)
+    mask = np.ascontiguousarray(p.dealias_mask.astype(np.int32))
+    keep.append(mask)
+    c.dealias_mask = mask.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     c.nwet = nwet.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     c.wet_mask = dptr(to_internal2(p.wet_mask))
     c.tau_x, c.tau_y = dptr(to_internal2(p.tau_x_2d)), dptr(to_internal2(p.tau_y_2d))

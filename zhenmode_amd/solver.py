@@ -165,6 +165,9 @@ class Solver:
     def set_graph(self, enabled):
         _lib.check(self.lib.zm_set_graph(self.handle, int(bool(enabled))), self.handle)
 
+    def nu_substeps_per_launch(self):
+        return int(self.lib.zm_nu_substeps_per_launch(self.handle))
+
     def time_kernel(self, which, reps):
         ms = ctypes.c_double()
         _lib.check(self.lib.zm_time_kernel(self.handle, int(which), int(reps), ctypes.byref(ms)), self.handle)
