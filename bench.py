@@ -1,6 +1,6 @@
 """Benchmark of the mode-split step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case c2] [--dtype float32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case g05deg] [--dtype float32]
 
 Workload: the 0.5-degree plain FP32 step (720 x 320 x 50, dt = 900 s,
 45 barotropic and 45 nu_h substeps, seasonal wind via step_dyn), synthetic
@@ -45,26 +45,34 @@ def read_pmc_traffic(name):
     return None
 
 
-def cpu_baseline(seconds_budget=25.0):
-    """Oracle (NumPy float64, one core) on a bounded sample, scaled to SYPD at c2."""
+def cpu_baseline():
+    """Oracle (NumPy float64, one core) on a bounded sample, scaled to SYPD of the 0.5-degree step.
+
+    The sample is one step of the same generator, time step and sub-cycle
+    counts at 2-degree horizontal resolution (180 x 80 x 50, 1/16 of the
+    cells); the time is scaled to the 720 x 320 x 50 step by the ratio of
+    algorithmic bytes, which is the cell ratio at equal sub-cycle counts.
+    """
     from oracle import zm_oracle as O
     from zhenmode_amd.cases import make_case
 
-    case = make_case("c1")
+    sx, sy = 180, 80
+    case = make_case("g05deg", nx=sx, ny=sy)
     kw = {k: v for k, v in case.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
     p = O.make_params(case.grid, case.physics, case.dt, **kw)
     s = O.init_state(p, case.T_init, case.S_init)
     t0 = time.perf_counter()
     s = O.step(s, p)
     t1 = time.perf_counter() - t0
-    # scale one c1 step to one c2 step by the algorithmic byte ratio
-    b1 = algorithmic_step_bytes(360, 160, 50, 40, 40, 8)
-    b2 = algorithmic_step_bytes(720, 320, 50, 45, 45, 8)
-    t_c2 = t1 * b2 / b1
-    return {"value": 900.0 / (365.0 * t_c2), "unit": "SYPD", "cores": 1, "kind": "port",
-            "sample": (f"1 oracle step (NumPy float64, single-threaded) of the 1-degree case 360x160x50 "
-                       f"took {t1:.1f} s; scaled to the 720x320x50 step by the ratio of algorithmic bytes "
-                       f"({b2 / b1:.2f}x)")}
+    nz = case.grid.nz
+    n_nu = max(1, int(p.n_subcyc if p.nu_nsub is None else p.nu_nsub))
+    b1 = algorithmic_step_bytes(sx, sy, nz, n_nu, p.n_subcyc, 8)
+    b2 = algorithmic_step_bytes(720, 320, nz, n_nu, p.n_subcyc, 8)
+    t_full = t1 * b2 / b1
+    return {"value": case.dt / (365.0 * t_full), "unit": "SYPD", "cores": 1, "kind": "port",
+            "sample": (f"1 oracle step (NumPy float64, single-threaded) of the 0.5-degree case's generator at "
+                       f"{sx}x{sy}x{nz} (dt={case.dt:g}s, n_sub={p.n_subcyc}, n_nu={n_nu}) took {t1:.1f} s; "
+                       f"scaled to 720x320x{nz} by the algorithmic-byte ratio {b2 / b1:.1f}x")}
 
 
 def main():
@@ -72,7 +80,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--case", default="c2")
+    ap.add_argument("--case", default="g05deg")
     ap.add_argument("--dtype", default="float32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -133,6 +141,23 @@ def main():
     ev_ms = ev0.elapsed_time(ev1) / args.steps
     finite = bool(torch.isfinite(a.u).all() and torch.isfinite(a.T).all() and torch.isfinite(a.eta).all())
 
+    # STREAM-like copy of 1 GiB between two fresh buffers: the measured HBM
+    # reference, and the known-size kernel that calibrates the FETCH_SIZE /
+    # WRITE_SIZE counters of a --pmc run of this script
+    copy_src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    copy_dst = torch.empty_like(copy_src)
+    torch.cuda.synchronize()
+    ce0, ce1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    copy_dst.copy_(copy_src)
+    ce0.record()
+    for _ in range(5):
+        copy_dst.copy_(copy_src)
+    ce1.record()
+    torch.cuda.synchronize()
+    copy_ms = ce0.elapsed_time(ce1) / 5
+    copy_gbps = 2.0 * copy_src.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del copy_src, copy_dst
+
     word = 4 if args.dtype == "float32" else 8
     n3 = g.nx * g.ny * g.nz
     nu_h = solver.nu_substeps_per_launch()
@@ -152,14 +177,15 @@ def main():
                    "sim_days_per_s": case.dt / 86400.0 / (ms_per_step * 1e-3),
                    "event_ms_per_step": ev_ms, "finite": finite},
         "roofline": {"bound": "hbm",
-                     "kernel": (f"k_nu_block ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
+                     "kernel": (f"k_nu_strip ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
                                 else "k_nu_substep (one nu_h Laplacian substep of u and v)"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_pmc_traffic(f"nu_block_{args.case}_pmc.txt"),
+                     "traffic": read_pmc_traffic(f"nu_strip_{args.case}_pmc.txt"),
                      "bytes_per_launch": nu_bytes, "launch_ms": nu_ms,
                      "single_substep_launch_ms": nu_single_ms,
-                     "single_substep_GBps": 4 * n3 * word / (nu_single_ms * 1e-3) / 1e9},
+                     "single_substep_GBps": 4 * n3 * word / (nu_single_ms * 1e-3) / 1e9,
+                     "copy_1GiB_GBps": copy_gbps},
         "step_roofline": {"algorithmic_bytes": step_bytes,
                           "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},

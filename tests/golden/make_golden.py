@@ -3,7 +3,7 @@
     python tests/golden/make_golden.py
 
 Outputs (float64, reference layout (nx, ny, nz)):
-  c0_8steps.npz   -- 90x40x12 case (4 degrees), initial state and the state
+  g4deg_8steps.npz   -- 90x40x12 case (4 degrees), initial state and the state
                      after 1 and 8 steps.
   land_1step.npz  -- 24x20x6 grid with land, shallow one-level columns and all
                      processes on: input state, every stage of one step.
@@ -22,20 +22,20 @@ from zhenmode_amd.cases import make_case
 HERE = Path(__file__).resolve().parent
 
 
-def c0_case():
-    case = make_case("c0")
+def g4deg_case():
+    case = make_case("g4deg")
     p = oracle_params(case.grid, case.physics, case.dt, case.solver_kwargs)
     return case, p, O.init_state(p, case.T_init, case.S_init)
 
 
 def main():
-    case, p, s = c0_case()
+    case, p, s = g4deg_case()
     out = {"init_" + f: getattr(s, f) for f in ("u", "v", "T", "S", "eta")}
     for n in range(8):
         s = O.step(s, p)
         if n in (0, 7):
             out.update({f"step{n + 1}_" + f: getattr(s, f) for f in ("u", "v", "T", "S", "eta")})
-    np.savez_compressed(HERE / "c0_8steps.npz", **out)
+    np.savez_compressed(HERE / "g4deg_8steps.npz", **out)
 
     grid, physics, dt, kwargs, T, S = land_case()
     p = oracle_params(grid, physics, dt, kwargs)

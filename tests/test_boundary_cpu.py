@@ -80,13 +80,13 @@ def test_host_freeze_matches_oracle_parameters():
     (dict(mode_split=True, surface_pressure_pa=np.zeros((90, 40))), "surface_pressure"),
 ])
 def test_invalid_or_unsupported_options_raise_value_error(kwargs, match):
-    case = make_case("c0")
+    case = make_case("g4deg")
     with pytest.raises(ValueError, match=match):
         freeze(case.grid, case.physics, case.dt, **kwargs)
 
 
 def test_negative_coefficients_raise_like_the_reference():
-    case = make_case("c0")
+    case = make_case("g4deg")
     with pytest.raises(ValueError, match="kappa_h"):
         freeze(case.grid, replace(case.physics, kappa_h=-1.0), case.dt, mode_split=True)
     with pytest.raises(ValueError, match="dt"):
@@ -94,10 +94,10 @@ def test_negative_coefficients_raise_like_the_reference():
 
 
 def test_case_generator_is_seeded_and_near_thirty_percent_land():
-    a, b = make_case("c0"), make_case("c0")
+    a, b = make_case("g4deg"), make_case("g4deg")
     np.testing.assert_array_equal(a.T_init, b.T_init)
     np.testing.assert_array_equal(a.grid.depth, b.grid.depth)
     assert 0.25 < a.land_fraction < 0.40
     assert a.grid.z[0] == 0.0 and a.grid.z[-1] == -4000.0 and a.grid.z[1] == pytest.approx(-10.0)
-    c1 = make_case("c1", nx=90, ny=40)
-    assert c1.grid.z[-1] == -5500.0 and c1.grid.nz == 50
+    g1deg = make_case("g1deg", nx=90, ny=40)
+    assert g1deg.grid.z[-1] == -5500.0 and g1deg.grid.nz == 50

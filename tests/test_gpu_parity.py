@@ -44,8 +44,8 @@ def check(got, ref, tol, label):
         assert e <= t, (label, f, e, t)
 
 
-def c0_solver(dtype, **extra):
-    case = make_case("c0")
+def g4deg_solver(dtype, **extra):
+    case = make_case("g4deg")
     step, init_state, diag, params, _ = make_solver_global(case.grid, case.physics, case.dt, return_params=True,
                                                            dtype=dtype, **case.solver_kwargs, **extra)
     return case, step, init_state, diag, params
@@ -53,9 +53,9 @@ def c0_solver(dtype, **extra):
 
 @pytest.mark.parametrize("dtype,tol", [("float64", TOL64), ("float32", TOL32)])
 def test_c0_eight_steps_match_golden_vectors(dtype, tol):
-    print(f"case c0 90x40x12 {dtype}", flush=True)
-    gold = np.load(GOLDEN / "c0_8steps.npz")
-    case, step, init_state, _, _ = c0_solver(dtype)
+    print(f"case g4deg 90x40x12 {dtype}", flush=True)
+    gold = np.load(GOLDEN / "g4deg_8steps.npz")
+    case, step, init_state, _, _ = g4deg_solver(dtype)
     s = init_state(case.T_init, case.S_init)
     check(s, {f: gold["init_" + f] for f in FIELDS}, TOL64 if dtype == "float64" else 1e-7, "init")
     s = step(s)
@@ -110,8 +110,8 @@ def test_residual_stages_match_oracle(dtype):
 
 
 def test_blocked_nu_subcycle_equals_single_substeps():
-    print("case c0 nu blocking float64", flush=True)
-    case = make_case("c0")
+    print("case g4deg nu blocking float64", flush=True)
+    case = make_case("g4deg")
     s = None
     outs = []
     for block in ("1", ""):
@@ -132,8 +132,8 @@ def test_blocked_nu_subcycle_equals_single_substeps():
 
 
 def test_graph_replays_match_eager_steps():
-    print("case c0 graph replay float32", flush=True)
-    case, step, init_state, _, _ = c0_solver("float32")
+    print("case g4deg graph replay float32", flush=True)
+    case, step, init_state, _, _ = g4deg_solver("float32")
     sol = step.solver
     s0 = init_state(case.T_init, case.S_init)
     sol.set_graph(False)
@@ -153,8 +153,8 @@ def test_graph_replays_match_eager_steps():
 
 
 def test_step_is_functional_and_step_dyn_matches_frozen_forcing():
-    print("case c0 functional float32", flush=True)
-    case = make_case("c0")
+    print("case g4deg functional float32", flush=True)
+    case = make_case("g4deg")
     step, init_state, _, step_dyn = make_solver_global(case.grid, case.physics, case.dt, dtype="float32",
                                                        dynamic_forcing=True, **case.solver_kwargs)
     s0 = init_state(case.T_init, case.S_init)
@@ -185,7 +185,7 @@ def test_diagnostics_match_oracle():
 
 
 def test_init_state_masks_land_and_ghost_like_the_reference():
-    case, _, init_state, _, params = c0_solver("float64")
+    case, _, init_state, _, params = g4deg_solver("float64")
     s = to_numpy_state(init_state(case.T_init, case.S_init))
     p = oracle_params(case.grid, case.physics, case.dt, case.solver_kwargs)
     ref = O.init_state(p, case.T_init, case.S_init)

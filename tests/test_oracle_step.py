@@ -8,7 +8,7 @@ import pytest
 
 from oracle import zm_oracle as O
 from tests.fixtures import land_case, oracle_params, perturbed_state, quiet_physics
-from tests.golden.make_golden import c0_case
+from tests.golden.make_golden import g4deg_case
 from zhenmode_amd.cases import make_case
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -16,8 +16,8 @@ FIELDS = ("u", "v", "T", "S", "eta")
 
 
 def test_c0_golden_vectors_reproduce():
-    case, p, s = c0_case()
-    gold = np.load(GOLDEN / "c0_8steps.npz")
+    case, p, s = g4deg_case()
+    gold = np.load(GOLDEN / "g4deg_8steps.npz")
     for f in FIELDS:
         np.testing.assert_array_equal(getattr(s, f), gold["init_" + f])
     s = O.step(s, p)
@@ -42,7 +42,7 @@ def test_resting_uniform_ocean_is_a_fixed_point():
     # A resting ocean at T_ref, S_ref everywhere has no force: every stage
     # returns it (u, v, eta to round-off zero; T, S to a few ulps of the cap
     # blend wb*x + (1-wb)*x).
-    case = make_case("c0")
+    case = make_case("g4deg")
     kw = dict(case.solver_kwargs)
     kw["forcing"] = None
     p = oracle_params(case.grid, case.physics, case.dt, kw)
@@ -79,7 +79,7 @@ def test_land_and_ghost_nodes_hold_their_prestep_values():
 
 
 def test_c0_run_stays_bounded_over_eight_steps():
-    gold = np.load(GOLDEN / "c0_8steps.npz")
+    gold = np.load(GOLDEN / "g4deg_8steps.npz")
     assert np.all(np.isfinite(gold["step8_u"]))
     assert np.max(np.abs(gold["step8_u"])) < 1.0
     assert np.max(np.abs(gold["step8_eta"])) < 1.0

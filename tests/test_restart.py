@@ -11,7 +11,7 @@ from zhenmode_amd.solver import JaxStateG
 
 
 def _setup(tmp_path, dtype="float32"):
-    case = make_case("c0")
+    case = make_case("g4deg")
     p = freeze(case.grid, case.physics, case.dt, dtype=dtype, **case.solver_kwargs)
     src = tmp_path / "code.py"
     src.write_text("print('step')\n")

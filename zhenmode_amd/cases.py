@@ -1,16 +1,18 @@
 """Seeded synthetic cases for the mode-split step (no data files needed).
 
-Five workloads, all on the reference's collocated lat-lon grid:
+Four single-grid workloads and one ensemble, all on the reference's collocated lat-lon grid:
 
-====  ===========================  =====  =======  =====  ======================
-name  grid (nx x ny x nz)          lat    dt [s]  n_sub  forcing
-====  ===========================  =====  =======  =====  ======================
-c0    90 x 40 x 12, H = 4000 m     +-78    600     30    wind
-c1    360 x 160 x 50, H = 5500 m   +-80   1200     40    wind + SST/SSS restore
-c2    720 x 320 x 50, H = 5500 m   +-80    900     45    seasonal wind + restore
-c3    24-member ensemble (2, 1, 0.75 deg)   1200     40    wind (+ kappa_v draw)
-c4    800 x 356 x 50, H = 5500 m   +-80    900     45    seasonal wind + restore
-====  ===========================  =====  =======  =====  ======================
+=======  ===========================  =====  ======  =====  ======================
+name     grid (nx x ny x nz)          lat    dt [s]  n_sub  forcing
+=======  ===========================  =====  ======  =====  ======================
+g4deg    90 x 40 x 12, H = 4000 m     +-78      600     30  wind
+g1deg    360 x 160 x 50, H = 5500 m   +-80     1200     40  wind + SST/SSS restore
+g05deg   720 x 320 x 50, H = 5500 m   +-80      900     45  seasonal wind + restore
+g045deg  800 x 356 x 50, H = 5500 m   +-80      900     45  seasonal wind + restore
+=======  ===========================  =====  ======  =====  ======================
+
+The 24-member uneven ensemble (2, 1 and 0.75 degrees, dt = 1200 s, 40
+substeps, per-member kappa_v draw) comes from ``ensemble_members``.
 
 Levels: z_0 = 0 and spacings 10 r^k with r chosen so that z_{nz-1} = -H.
 Bathymetry: 20 Gaussian elevation bumps on a flat abyssal plain of depth H,
@@ -20,7 +22,7 @@ latitude band, s = H / 4000 (so every config shares one coastline pattern);
 columns shallower than 50 m are land.
 Initial state: T = 2 + 23 exp(z/500) + N(0, 0.05), S = 34.7 + 0.3 tanh(z/300) +
 N(0, 0.01), u = v = eta = 0.  Wind: tau_x = 0.1 cos(3 lat) N/m^2.
-Restoring targets (c1, c2, c4): SST* = -1 + 28 cos^2(lat) + N(0, 0.2),
+Restoring targets (g1deg, g05deg, g045deg): SST* = -1 + 28 cos^2(lat) + N(0, 0.2),
 SSS* = 34.7 + 0.5 cos(2 lat) + N(0, 0.02), 30-day timescale, applied through the
 reference factory's ``S_ref_surf``/``sss_restore_days`` and
 ``coastal_restore_{mask,days,T}`` arguments (``factory.py:424-454``).
@@ -108,7 +110,7 @@ class Case:
     land_fraction: float = 0.0
 
     def wind(self, t_seconds):
-        """(tau_x, tau_y, q_heat) at time t; seasonal amplitude for c2/c4."""
+        """(tau_x, tau_y, q_heat) at time t; seasonal amplitude for g05deg/g045deg."""
         lat = np.radians(self.grid.lat)
         amp = 0.1 + (0.05 * np.sin(2.0 * np.pi * t_seconds / (365.0 * 86400.0)) if self.seasonal_wind else 0.0)
         nx, ny = self.grid.nx, self.grid.ny
@@ -117,10 +119,10 @@ class Case:
 
 
 _SPECS = {
-    "c0": dict(nx=90, ny=40, nz=12, lat=78.0, H=4000.0, dt=600.0, nsub=30, steps=8, restore=False, seasonal=False),
-    "c1": dict(nx=360, ny=160, nz=50, lat=80.0, H=5500.0, dt=1200.0, nsub=40, steps=72, restore=True, seasonal=False),
-    "c2": dict(nx=720, ny=320, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True),
-    "c4": dict(nx=800, ny=356, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True),
+    "g4deg": dict(seed_offset=147, nx=90, ny=40, nz=12, lat=78.0, H=4000.0, dt=600.0, nsub=30, steps=8, restore=False, seasonal=False),
+    "g1deg": dict(seed_offset=148, nx=360, ny=160, nz=50, lat=80.0, H=5500.0, dt=1200.0, nsub=40, steps=72, restore=True, seasonal=False),
+    "g05deg": dict(seed_offset=149, nx=720, ny=320, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True),
+    "g045deg": dict(seed_offset=151, nx=800, ny=356, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True),
 }
 
 
@@ -140,7 +142,7 @@ def make_case(name, *, nx=None, ny=None, nz=None, seed=FIELD_SEED):
     physics = replace(PhysicsConfig(), nu_h=5.0e6 * res ** 2, nu_v=1.0e-4, kappa_h=100.0,
                       kappa_v=1.0e-5, kappa_conv=0.05, nu_bi=0.5 * bound, kappa_bi=0.5 * bound,
                       r_bot=1.0e-3)
-    rng = np.random.default_rng(seed + sum(map(ord, name)))
+    rng = np.random.default_rng(seed + spec["seed_offset"])
     zz = z[None, None, :]
     T = 2.0 + 23.0 * np.exp(zz / 500.0) + rng.normal(0.0, 0.05, (nx, ny, nz))
     S = 34.7 + 0.3 * np.tanh(zz / 300.0) + rng.normal(0.0, 0.01, (nx, ny, nz))
