@@ -139,6 +139,74 @@ int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_lau
 /* Number of nu_h substeps one blocked launch performs (1 = unblocked). */
 int32_t zm_nu_substeps_per_launch(const zm_handle* h);
 
+/* ---- implicit vertical mixing and its learned surrogate -----------------
+ * No reference function: the reference's vertical diffusion and convection
+ * are explicit.  The mixing operator is the backward-Euler counterpart of its
+ * flux-form interface operator (_diffusive_interface_flux /
+ * _iface_flux_divergence, numerics/vertical.py:296-332) with the linear EOS
+ * (physics/eos.py:13-18):
+ *   kappa_{k+1/2} = kappa_v + kappa_conv [rho'_k > rho'_{k+1}] on wet interfaces,
+ *   (I - dt D_kappa) X_new = X for X = T, S; zero-flux ends; dry levels held.
+ * The handle supplies the grid (wet levels, dz_node, dz_iface, T_ref, S_ref). */
+
+/* Thomas solve per column (one column per lane, float64 elimination).
+ * T, S, T_out, S_out: 3-D device fields in the handle's dtype; outputs must not
+ * alias inputs.  only_columns: optional int32 [ny*nx] device mask; when given,
+ * only columns with a nonzero entry are solved (others untouched). */
+int zm_vertical_mix(zm_handle* h, const void* T, const void* S, void* T_out, void* S_out, double dt,
+                    double kappa_v, double kappa_conv, const int32_t* only_columns, void* stream);
+
+/* Surrogate input rows X[c][0..255] (bfloat16, c = j*nx + i, rows >= ny*nx,
+ * rows beyond ny*nx zero): T, S, u, v on nz levels (0 below the seafloor),
+ * N^2 at each level's lower interface, q_heat, the SSS-restoring salt flux,
+ * |tau|, zero padding; each feature f normalised as (x - mean[f]) * inv_std[f].
+ * tau_x / tau_y / q_heat may be NULL for the handle's frozen forcing. nz <= 50. */
+int zm_column_features(zm_handle* h, const zm_state* in, const void* tau_x, const void* tau_y,
+                       const void* q_heat, const float* mean, const float* inv_std, void* X, int64_t rows,
+                       void* stream);
+
+/* Apply surrogate outputs Y (float32 rows of ldy >= 2 nz: nz T then nz S
+ * values, normalised): dX = dt (y out_std + out_mean) on wet levels, shifted
+ * so that the float64 column integral sum dz_node dX is zero (heat and salt
+ * conserved exactly), X_new = X + dX into T_out/S_out.  Columns whose implicit
+ * residual max|X_new - dt D(X_new) - X| exceeds tol_T / tol_S get flags[c] = 1
+ * and are counted into *n_flagged (device int32, accumulated); the caller
+ * passes the same flags to zm_vertical_mix to replace them by the Thomas
+ * solution. */
+int zm_mlp_apply(zm_handle* h, const zm_state* in, const float* Y, int32_t ldy, const float* out_mean,
+                 const float* out_std, double dt, double kappa_v, double kappa_conv, double tol_T,
+                 double tol_S, void* T_out, void* S_out, int32_t* flags, int32_t* n_flagged, void* stream);
+
+/* C = A B^T, A (M x K) and B (N x K) bfloat16 row-major device arrays, float32
+ * accumulation on MFMA.  M and N multiples of 128, K a multiple of 32.
+ * epilogue 0: C_f32 = acc; 1: C_f32 = acc + bias[n];
+ * 2: Z = acc + bias[n] -> C_f32 (optional), C_bf16 = gelu(Z) (M x N) and
+ *    CT_bf16 = its transpose (N x M, optional);
+ * 3: C_bf16 = acc * gelu'(aux[m][n]) and CT_bf16 = its transpose (optional). */
+int zm_gemm_bf16_nt(int64_t M, int64_t N, int64_t K, const void* A, const void* B, int32_t epilogue,
+                    const float* bias, const float* aux, float* C_f32, void* C_bf16, void* CT_bf16,
+                    void* stream);
+
+/* Mean-squared-error gradient over the first n_out of ld entries per row:
+ * dY = 2 (Y - target) / (rows n_out) as bfloat16 rows (dY) and columns (dYT),
+ * *loss_sum += sum (Y - target)^2 (device float). */
+int zm_mse_grad(int64_t rows, int32_t ld, int32_t n_out, const float* Y, const float* target, void* dY,
+                void* dYT, float* loss_sum, void* stream);
+
+/* out[r] = sum_c in[r][c] over a rows x cols bfloat16 matrix (float32 sums). */
+int zm_rowsum_bf16(int64_t rows, int64_t cols, const void* in, float* out, void* stream);
+
+/* out (cols x rows) = transpose of in (rows x cols), bfloat16. */
+int zm_transpose_bf16(int64_t rows, int64_t cols, const void* in, void* out, void* stream);
+
+/* Adam step t (>= 1) on float32 master weights w (n_out x n_in) with gradient
+ * gw and moments m, v; writes the bfloat16 copies w_bf (n_out x n_in) and
+ * wT_bf (n_in x n_out).  b/gb/bm/bv (length n_out) update the bias when b is
+ * not NULL. */
+int zm_adam(int32_t n_out, int32_t n_in, float* w, const float* gw, float* m, float* v, void* w_bf,
+            void* wT_bf, float* b, const float* gb, float* bm, float* bv, float lr, float beta1, float beta2,
+            float eps, int32_t t, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,7 +50,12 @@ class ZmState(ctypes.Structure):
 
 
 EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", "zm_step_dyn",
-           "zm_diagnostics", "zm_stage", "zm_set_graph", "zm_time_kernel", "zm_nu_substeps_per_launch")
+           "zm_diagnostics", "zm_stage", "zm_set_graph", "zm_time_kernel", "zm_nu_substeps_per_launch",
+           "zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
+           "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam")
+
+# epilogues of zm_gemm_bf16_nt
+EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
 
 _lib = None
 
@@ -86,6 +91,19 @@ def load():
     lib.zm_time_kernel.restype = ctypes.c_int
     lib.zm_nu_substeps_per_launch.argtypes = [vp]
     lib.zm_nu_substeps_per_launch.restype = ctypes.c_int32
+    i64, i32, f32, f64 = ctypes.c_int64, ctypes.c_int32, ctypes.c_float, ctypes.c_double
+    lib.zm_vertical_mix.argtypes = [vp, vp, vp, vp, vp, f64, f64, f64, vp, vp]
+    lib.zm_column_features.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp, vp, vp, vp, vp, i64, vp]
+    lib.zm_mlp_apply.argtypes = [vp, ctypes.POINTER(ZmState), vp, i32, vp, vp, f64, f64, f64, f64, f64, vp, vp,
+                                 vp, vp, vp]
+    lib.zm_gemm_bf16_nt.argtypes = [i64, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp]
+    lib.zm_mse_grad.argtypes = [i64, i32, i32, vp, vp, vp, vp, vp, vp]
+    lib.zm_rowsum_bf16.argtypes = [i64, i64, vp, vp, vp]
+    lib.zm_transpose_bf16.argtypes = [i64, i64, vp, vp, vp]
+    lib.zm_adam.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
+    for name in ("zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
+                 "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam"):
+        getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
 

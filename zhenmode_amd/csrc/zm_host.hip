@@ -17,6 +17,7 @@
 
 #include "../../include/zhenmode.h"
 #include "zm_common.h"
+#include "zm_column.h"
 #include "zm_kernels.h"
 
 namespace {
@@ -48,6 +49,16 @@ struct SolverBase {
   virtual void stage(int which, const zm_state& in, const zm_state& out, hipStream_t s) = 0;
   virtual double time_kernel(int which, int reps) = 0;
   virtual int nu_substeps_per_launch() const = 0;
+  virtual void vertical_mix(const void* T, const void* S, void* To, void* So, double dt, double kv, double kc,
+                            const int* only, hipStream_t s) = 0;
+  virtual void column_features(const zm_state& in, const void* tau_x, const void* tau_y, const void* q_heat,
+                               const float* mean, const float* inv_std, void* X, long long rows,
+                               hipStream_t s) = 0;
+  virtual void mlp_apply(const zm_state& in, const float* Y, int ldy, const float* omean, const float* ostd,
+                         double dt, double kv, double kc, double tol_T, double tol_S, void* To, void* So, int* flags,
+                         int* count, hipStream_t s) = 0;
+  int nz_ = 0;
+  long long n2_ = 0;
   bool use_graph = true;
 };
 
@@ -63,6 +74,8 @@ struct Solver : SolverBase {
   int nu_block = 1;
   std::vector<void*> allocs;
   real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fden = nullptr;
+  double *dzn64 = nullptr, *dzi64 = nullptr;
+  double T_ref64 = 0, S_ref64 = 0;
   // workspace
   real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
   real *Fz1, *Fz2, *cT, *cS;
@@ -183,6 +196,14 @@ struct Solver : SolverBase {
     int* nw = dev_alloc<int>(n2);
     ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
     g.nwet = nw;
+    dzn64 = dev_alloc<double>(nz);
+    ZM_HIP(hipMemcpy(dzn64, c.dz_node, nz * sizeof(double), hipMemcpyHostToDevice));
+    dzi64 = dev_alloc<double>(nz > 1 ? nz - 1 : 1);
+    if (nz > 1) ZM_HIP(hipMemcpy(dzi64, c.dz_iface, (nz - 1) * sizeof(double), hipMemcpyHostToDevice));
+    T_ref64 = c.T_ref;
+    S_ref64 = c.S_ref;
+    nz_ = nz;
+    n2_ = n2;
 
     P.dt = (real)c.dt;
     P.dth = (real)(c.dt / 2.0);
@@ -461,6 +482,42 @@ struct Solver : SolverBase {
 
   int nu_substeps_per_launch() const override { return nu_block; }
 
+  // ---- column operators (implicit mixing, surrogate) ----------------------
+  zm::MixParams mix(double dt, double kv, double kc) const {
+    zm::MixParams m{};
+    m.dt = dt;
+    m.kappa_v = kv;
+    m.kappa_conv = kc;
+    m.T_ref = T_ref64;
+    m.S_ref = S_ref64;
+    m.dzn = dzn64;
+    m.dzi = dzi64;
+    return m;
+  }
+
+  void vertical_mix(const void* T, const void* S, void* To, void* So, double dt, double kv, double kc,
+                    const int* only, hipStream_t s) override {
+    zm::launch_vmix(g, mix(dt, kv, kc), (const real*)T, (const real*)S, (real*)To, (real*)So, only, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void column_features(const zm_state& in, const void* tau_x, const void* tau_y, const void* q_heat,
+                       const float* mean, const float* inv_std, void* X, long long rows, hipStream_t s) override {
+    zm::launch_col_features(g, mix(P.dt, 0.0, 0.0), (const real*)in.T, (const real*)in.S, (const real*)in.u,
+                            (const real*)in.v, q_heat ? (const real*)q_heat : P.q_heat, P.s_ref2,
+                            P.restore_coef_S, tau_x ? (const real*)tau_x : P.tau_x,
+                            tau_y ? (const real*)tau_y : P.tau_y, mean, inv_std, X, rows, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void mlp_apply(const zm_state& in, const float* Y, int ldy, const float* omean, const float* ostd, double dt,
+                 double kv, double kc, double tol_T, double tol_S, void* To, void* So, int* flags, int* count,
+                 hipStream_t s) override {
+    zm::launch_mlp_apply(g, mix(dt, kv, kc), (const real*)in.T, (const real*)in.S, Y, ldy, omean, ostd, tol_T, tol_S,
+                         (real*)To, (real*)So, flags, count, s);
+    ZM_HIP(hipGetLastError());
+  }
+
   double time_kernel(int which, int reps) override {
     hipEvent_t a, b;
     ZM_HIP(hipEventCreate(&a));
@@ -651,6 +708,95 @@ int zm_set_graph(zm_handle* h, int32_t enabled) {
 }
 
 int32_t zm_nu_substeps_per_launch(const zm_handle* h) { return h ? h->solver->nu_substeps_per_launch() : 0; }
+
+int zm_vertical_mix(zm_handle* h, const void* T, const void* S, void* T_out, void* S_out, double dt, double kappa_v,
+                    double kappa_conv, const int32_t* only_columns, void* stream) {
+  if (!h || !T || !S || !T_out || !S_out || !(dt >= 0.0) || !(kappa_v >= 0.0) || !(kappa_conv >= 0.0))
+    return fail(h, "invalid zm_vertical_mix arguments", ZM_EINVAL);
+  if (T == T_out || S == S_out) return fail(h, "zm_vertical_mix output must not alias its input", ZM_EINVAL);
+  if ((size_t)3 * h->solver->nz_ * 64 * sizeof(double) > 160 * 1024)
+    return fail(h, "zm_vertical_mix supports at most 106 levels", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->vertical_mix(T, S, T_out, S_out, dt, kappa_v, kappa_conv, only_columns, (hipStream_t)stream);
+  });
+}
+
+int zm_column_features(zm_handle* h, const zm_state* in, const void* tau_x, const void* tau_y, const void* q_heat,
+                       const float* mean, const float* inv_std, void* X, int64_t rows, void* stream) {
+  if (!h || !in || !mean || !inv_std || !X) return fail(h, "invalid zm_column_features arguments", ZM_EINVAL);
+  if (5 * h->solver->nz_ + 3 > 256) return fail(h, "column features need nz <= 50", ZM_EINVAL);
+  if (rows < h->solver->n2_) return fail(h, "feature rows must cover every column", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->column_features(*in, tau_x, tau_y, q_heat, mean, inv_std, X, rows, (hipStream_t)stream);
+  });
+}
+
+int zm_mlp_apply(zm_handle* h, const zm_state* in, const float* Y, int32_t ldy, const float* out_mean,
+                 const float* out_std, double dt, double kappa_v, double kappa_conv, double tol_T, double tol_S,
+                 void* T_out, void* S_out, int32_t* flags, int32_t* n_flagged, void* stream) {
+  if (!h || !in || !Y || !out_mean || !out_std || !T_out || !S_out || !flags || !n_flagged)
+    return fail(h, "invalid zm_mlp_apply arguments", ZM_EINVAL);
+  if (ldy < 2 * h->solver->nz_) return fail(h, "ldy must hold 2 nz outputs per column", ZM_EINVAL);
+  if (T_out == in->T || S_out == in->S) return fail(h, "zm_mlp_apply output must not alias its input", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->mlp_apply(*in, Y, ldy, out_mean, out_std, dt, kappa_v, kappa_conv, tol_T, tol_S, T_out, S_out,
+                         flags, n_flagged, (hipStream_t)stream);
+  });
+}
+
+int zm_gemm_bf16_nt(int64_t M, int64_t N, int64_t K, const void* A, const void* B, int32_t epilogue,
+                    const float* bias, const float* aux, float* C_f32, void* C_bf16, void* CT_bf16, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || M % 128 || N % 128 || K % 32 || !A || !B)
+    return fail(nullptr, "zm_gemm_bf16_nt: M, N must be positive multiples of 128 and K of 32", ZM_EINVAL);
+  const bool ok = (epilogue == zm::kEpiF32 && C_f32) || (epilogue == zm::kEpiBiasF32 && C_f32 && bias) ||
+                  (epilogue == zm::kEpiBiasGelu && bias && C_bf16) || (epilogue == zm::kEpiGeluGrad && aux && C_bf16);
+  if (!ok) return fail(nullptr, "zm_gemm_bf16_nt: missing operand for the epilogue", ZM_EINVAL);
+  return guarded(nullptr, [&] {
+    zm::launch_gemm_bf16_nt(M, N, K, A, B, epilogue, bias, aux, C_f32, C_bf16, CT_bf16, (hipStream_t)stream);
+    ZM_HIP(hipGetLastError());
+  });
+}
+
+int zm_mse_grad(int64_t rows, int32_t ld, int32_t n_out, const float* Y, const float* target, void* dY, void* dYT,
+                float* loss_sum, void* stream) {
+  if (rows <= 0 || ld <= 0 || n_out <= 0 || n_out > ld || !Y || !target || !dY || !dYT || !loss_sum)
+    return fail(nullptr, "invalid zm_mse_grad arguments", ZM_EINVAL);
+  return guarded(nullptr, [&] {
+    zm::launch_mse_grad(rows, ld, n_out, Y, target, dY, dYT, loss_sum, (hipStream_t)stream);
+    ZM_HIP(hipGetLastError());
+  });
+}
+
+int zm_rowsum_bf16(int64_t rows, int64_t cols, const void* in, float* out, void* stream) {
+  if (rows <= 0 || cols <= 0 || !in || !out) return fail(nullptr, "invalid zm_rowsum_bf16 arguments", ZM_EINVAL);
+  return guarded(nullptr, [&] {
+    zm::launch_rowsum_bf16(rows, cols, in, out, (hipStream_t)stream);
+    ZM_HIP(hipGetLastError());
+  });
+}
+
+int zm_transpose_bf16(int64_t rows, int64_t cols, const void* in, void* out, void* stream) {
+  if (rows <= 0 || cols <= 0 || !in || !out || in == out)
+    return fail(nullptr, "invalid zm_transpose_bf16 arguments", ZM_EINVAL);
+  return guarded(nullptr, [&] {
+    zm::launch_transpose_bf16(rows, cols, in, out, (hipStream_t)stream);
+    ZM_HIP(hipGetLastError());
+  });
+}
+
+int zm_adam(int32_t n_out, int32_t n_in, float* w, const float* gw, float* m, float* v, void* w_bf, void* wT_bf,
+            float* b, const float* gb, float* bm, float* bv, float lr, float beta1, float beta2, float eps, int32_t t,
+            void* stream) {
+  if (n_out <= 0 || n_in <= 0 || !w || !gw || !m || !v || !w_bf || !wT_bf || t < 1 ||
+      (b && (!gb || !bm || !bv)))
+    return fail(nullptr, "invalid zm_adam arguments", ZM_EINVAL);
+  const float c1 = 1.0f - std::pow(beta1, (float)t), c2 = 1.0f - std::pow(beta2, (float)t);
+  return guarded(nullptr, [&] {
+    zm::launch_adam(n_out, n_in, w, gw, m, v, w_bf, wT_bf, b, gb, bm, bv, lr, beta1, beta2, eps, c1, c2,
+                    (hipStream_t)stream);
+    ZM_HIP(hipGetLastError());
+  });
+}
 
 int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch) {
   if (!h || reps < 1 || !ms_per_launch) return fail(h, "invalid zm_time_kernel arguments", ZM_EINVAL);

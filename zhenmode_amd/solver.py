@@ -173,6 +173,22 @@ class Solver:
         _lib.check(self.lib.zm_time_kernel(self.handle, int(which), int(reps), ctypes.byref(ms)), self.handle)
         return ms.value
 
+    def vertical_mix(self, state, dt, kappa_v, kappa_conv, only=None, out=None):
+        """Implicit (backward-Euler) vertical mixing of T and S over dt, Thomas
+        solve per column; returns a state whose T, S are the mixed fields and
+        whose other fields are the input's (shared, not copied)."""
+        src = self.coerce(state)
+        s3, dt_, dev = self.shape3, self.dtype, self.device
+        To = empty3(*s3, dt_, dev) if out is None else out[0]
+        So = empty3(*s3, dt_, dev) if out is None else out[1]
+        if only is not None and (only.dtype != torch.int32 or only.numel() != self.shape2[0] * self.shape2[1]):
+            raise ValueError("only must be an int32 tensor with one entry per column")
+        rc = self.lib.zm_vertical_mix(self.handle, ptr(src.T), ptr(src.S), ptr(To), ptr(So), float(dt),
+                                      float(kappa_v), float(kappa_conv), ptr(only) if only is not None else None,
+                                      self._stream())
+        _lib.check(rc, self.handle)
+        return src._replace(T=To, S=So)
+
     def init_state(self, T_init=None, S_init=None):
         """Reference factory.py:766-788: zero velocity/eta/ice, land and ghost
         nodes set to the T_ref/S_ref sentinel."""
