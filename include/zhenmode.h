@@ -133,7 +133,8 @@ int zm_set_graph(zm_handle* h, int32_t enabled);
  * (zm_nu_substeps_per_launch substeps), 6 = plain copy of one 3-D field
  * (bandwidth calibration), 7 = linear-step finish, 8 = convection,
  * 9 = Fz column scan, 10 = unit D1 pair, 11 = momentum advection,
- * 12 = final masks. */
+ * 12 = final masks, 14 = x de-alias by the four-step DFT (3 uses the mixed-radix FFT when nx
+ * has no prime factor above 5). */
 int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch);
 
 /* Number of nu_h substeps one blocked launch performs (1 = unblocked). */
@@ -206,6 +207,36 @@ int zm_transpose_bf16(int64_t rows, int64_t cols, const void* in, void* out, voi
 int zm_adam(int32_t n_out, int32_t n_in, float* w, const float* gw, float* m, float* v, void* w_bf,
             void* wT_bf, float* b, const float* gb, float* bm, float* bv, float lr, float beta1, float beta2,
             float eps, int32_t t, void* stream);
+
+/* ---- tiles, autotuner and batched ensembles ------------------------------
+ * No reference counterpart: the reference steps one grid per jitted call. */
+
+/* Launch shapes of a handle: nu_block = nu_h substeps per strip launch
+ * (1..4 in float32, 1..4 in float64), seg = levels per thread of the
+ * column-marching kernels (1..nz).  Results do not depend on either. */
+int zm_set_tiles(zm_handle* h, int32_t nu_block, int32_t seg);
+int zm_get_tiles(const zm_handle* h, int32_t* nu_block, int32_t* seg);
+
+/* Time `reps` steps from a copy of *sample for every (nu_block, seg)
+ * candidate on this device, keep the fastest and report it.  Allocates two
+ * scratch states for the duration of the call; *sample is not modified. */
+int zm_autotune(zm_handle* h, const zm_state* sample, int32_t reps, int32_t* nu_block, int32_t* seg,
+                double* best_ms);
+
+/* An ensemble of float32 handles with equal barotropic substep counts and
+ * nx <= 1024 (sizes may differ).  One zm_ensemble_step advances every member
+ * one step as a single hipGraph launch: each member's 3-D stages, then the
+ * barotropic sub-cycle as one batched launch per substep over a device member
+ * table, then each member's final masks; results equal each member's own
+ * zm_step bit for bit.  Members keep their own workspaces, so a member handle
+ * must not be stepped on its own while an ensemble step is in flight. */
+typedef struct zm_ensemble zm_ensemble;
+int zm_ensemble_create(zm_handle* const* members, int32_t n_members, zm_ensemble** out);
+/* in[n_members], out[n_members]; out[i] may alias in[i] */
+int zm_ensemble_step(zm_ensemble* e, const zm_state* in, const zm_state* out, void* stream);
+int zm_ensemble_set_graph(zm_ensemble* e, int32_t enabled);
+int32_t zm_ensemble_size(const zm_ensemble* e);
+int zm_ensemble_destroy(zm_ensemble* e);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,85 @@
+"""Batched uneven ensembles and the tile autotuner on the GPU.
+
+The ensemble step runs the same kernels as each member's own step (the
+barotropic substep batched over a member table), so every member's state must
+equal its own step bit for bit; launch shapes never change results, so every
+autotuner candidate must give bit-identical steps.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from zhenmode_amd.cases import make_case  # noqa: E402
+from zhenmode_amd.solver import Ensemble, make_solver_global  # noqa: E402
+
+FIELDS = ("u", "v", "T", "S", "eta", "ice")
+
+
+def member(nx, ny, nz, seed_name="g4deg"):
+    case = make_case(seed_name, nx=nx, ny=ny, nz=nz)
+    step, init_state, _ = make_solver_global(case.grid, case.physics, case.dt, dtype="float32",
+                                             **case.solver_kwargs)
+    return step.solver, init_state(case.T_init, case.S_init)
+
+
+def test_ensemble_step_equals_member_steps():
+    print("case ensemble 3 members (90x40x12, 60x28x8, 120x52x12) float32", flush=True)
+    members = [member(90, 40, 12), member(60, 28, 8), member(120, 52, 12)]
+    sols = [m[0] for m in members]
+    states = [m[1] for m in members]
+    ens = Ensemble(sols)
+    a = states
+    b = states
+    for n in range(3):
+        a = ens.step(a)
+        b = [s.step(x) for s, x in zip(sols, b)]
+        torch.cuda.synchronize()
+        for i in range(len(sols)):
+            for f in FIELDS:
+                assert torch.equal(getattr(a[i], f), getattr(b[i], f)), (n, i, f)
+    # eager (no graph) path as well
+    ens.set_graph(False)
+    c = ens.step(states)
+    d = [s.step(x) for s, x in zip(sols, states)]
+    for i in range(len(sols)):
+        for f in FIELDS:
+            assert torch.equal(getattr(c[i], f), getattr(d[i], f)), (i, f)
+
+
+def test_ensemble_rejects_mixed_substep_counts():
+    print("case ensemble validation", flush=True)
+    s1, _ = member(90, 40, 12)
+    case = make_case("g4deg")
+    kw = dict(case.solver_kwargs)
+    kw["dt_bt"] = case.dt / 10
+    step, _, _ = make_solver_global(case.grid, case.physics, case.dt, dtype="float32", **kw)
+    with pytest.raises(ValueError):
+        Ensemble([s1, step.solver])
+    with pytest.raises(ValueError):
+        Ensemble([s1, s1])
+
+
+def test_autotuner_candidates_give_identical_steps():
+    print("case autotune 90x40x12 float32", flush=True)
+    sol, s0 = member(90, 40, 12)
+    ref = sol.step(s0)
+    for nu, seg in ((1, 12), (2, 5), (3, 4), (4, 7)):
+        sol.set_tiles(nu, seg)
+        assert sol.get_tiles() == (nu, seg)
+        out = sol.step(s0)
+        for f in FIELDS:
+            assert torch.equal(getattr(out, f), getattr(ref, f)), (nu, seg, f)
+    nu, seg, ms = sol.autotune(s0, reps=2)
+    assert 1 <= nu <= 4 and 1 <= seg <= 12 and ms > 0
+    assert sol.get_tiles() == (nu, seg)
+    out = sol.step(s0)
+    for f in FIELDS:
+        assert torch.equal(getattr(out, f), getattr(ref, f)), f
+    with pytest.raises(ValueError):
+        sol.set_tiles(0, 5)
+    with pytest.raises(ValueError):
+        sol.set_tiles(2, 13)

@@ -52,7 +52,9 @@ class ZmState(ctypes.Structure):
 EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", "zm_step_dyn",
            "zm_diagnostics", "zm_stage", "zm_set_graph", "zm_time_kernel", "zm_nu_substeps_per_launch",
            "zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
-           "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam")
+           "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
+           "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
+           "zm_ensemble_destroy")
 
 # epilogues of zm_gemm_bf16_nt
 EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
@@ -102,7 +104,9 @@ def load():
     lib.zm_transpose_bf16.argtypes = [i64, i64, vp, vp, vp]
     lib.zm_adam.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
     for name in ("zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
-                 "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam"):
+                 "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
+           "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
+           "zm_ensemble_destroy"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

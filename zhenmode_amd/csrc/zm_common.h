@@ -49,6 +49,7 @@ struct Geo {
   real idz_top, idz_bot;  // 1/(z1-z0), 1/(z_{nz-1}-z_{nz-2}) (negative)
   real ih0top2, ih0bot2;  // 1/h0^2 at top and bottom
   const int* nwet;        // wet levels per column [n2]
+  int seg;                // levels per thread in the column-marching kernels
 };
 
 // Scalars and 2-D fields of the physics.

@@ -57,6 +57,8 @@ struct SolverBase {
   virtual void mlp_apply(const zm_state& in, const float* Y, int ldy, const float* omean, const float* ostd,
                          double dt, double kv, double kc, double tol_T, double tol_S, void* To, void* So, int* flags,
                          int* count, hipStream_t s) = 0;
+  virtual void set_tiles(int nu_block, int seg) = 0;
+  virtual void get_tiles(int* nu_block, int* seg) const = 0;
   int nz_ = 0;
   long long n2_ = 0;
   bool use_graph = true;
@@ -69,6 +71,8 @@ struct Solver : SolverBase {
   int nx = 0, ny = 0, nz = 0;
   long long n2 = 0, n3 = 0;
   int n_sub = 0, n_nu = 0, N1 = 1, N2 = 1;
+  zm::FftPlan fft{};
+  bool use_fft = false;
   int* kept = nullptr;
   real dt_bt = 0, dtH = 0, drag = 0, decay_bot = 0, dts = 0;
   int nu_block = 1;
@@ -196,6 +200,7 @@ struct Solver : SolverBase {
     int* nw = dev_alloc<int>(n2);
     ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
     g.nwet = nw;
+    g.seg = 10;
     dzn64 = dev_alloc<double>(nz);
     ZM_HIP(hipMemcpy(dzn64, c.dz_node, nz * sizeof(double), hipMemcpyHostToDevice));
     dzi64 = dev_alloc<double>(nz > 1 ? nz - 1 : 1);
@@ -261,6 +266,18 @@ struct Solver : SolverBase {
     for (int d = 1; d * d <= nx; ++d)
       if (nx % d == 0) N1 = d;
     N2 = nx / N1;
+    // mixed-radix plan (radices 4, 2, 3, 5); the four-step DFT serves lengths
+    // with other prime factors
+    {
+      int rest = nx;
+      fft.nst = 0;
+      for (int r : {4, 2, 3, 5})
+        while (rest % r == 0 && fft.nst < 16) {
+          fft.rad[fft.nst++] = r;
+          rest /= r;
+        }
+      use_fft = rest == 1;
+    }
     {
       std::vector<int> km(c.dealias_mask, c.dealias_mask + nx);
       kept = dev_alloc<int>(nx);
@@ -287,6 +304,14 @@ struct Solver : SolverBase {
     means = dev_alloc<real>((size_t)4 * nz * (2 * (nb > 0 ? nb : 1)));
     ZM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
     ZM_HIP(hipDeviceSynchronize());
+  }
+
+  // x part of the de-alias of both advection fields (one complex row each)
+  void dealias(const real* a, const real* b, real* oa, real* ob, hipStream_t s) {
+    if (use_fft)
+      zm::launch_dealias_fft(nx, fft, kept, (int)(ny * nz), twr, twi, a, b, oa, ob, s);
+    else
+      zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, a, b, oa, ob, s);
   }
 
   // ---- stages -----------------------------------------------------------
@@ -327,7 +352,7 @@ struct Solver : SolverBase {
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
                       (real*)nullptr, s);
     zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
-    zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+    dealias(au, av, axu, axv, s);
     zm::launch_mom_residual(g, Pf, 0, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, up, vp, decay_bot, s);
     // tracer residual #2 at (u_pred, v_pred, T_pred, S_pred) -> T_new
     zm::launch_fz(g, up, vp, Fz2, s);
@@ -340,33 +365,86 @@ struct Solver : SolverBase {
     zm::launch_mom_residual(g, Pf, 1, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, uo, vo, decay_bot, s);
   }
 
-  void barotropic(const zm::Phys<real>& Pf, const real* eta_in, hipStream_t s) {
+  // barotropic entry: F_rho and ubt0 from the post-L2 state
+  void bt_prepare(const zm::Phys<real>& Pf, const real* eta_in, hipStream_t s) {
     zm::launch_pbc(g, Pf, T3, S3, pbc, s);
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, u3, v3, ubt0, vbt0, s);
     zm::launch_copy(eta_in, etaw, n2, s);
     zm::launch_copy((const real*)ubt0, ubt, n2, s);
     zm::launch_copy((const real*)vbt0, vbt, n2, s);
+  }
+
+  void barotropic(const zm::Phys<real>& Pf, const real* eta_in, hipStream_t s) {
+    bt_prepare(Pf, eta_in, s);
     for (int q = 0; q < n_sub; ++q) {
       zm::launch_bt_eta(g, Pf, etaw, ubt, vbt, dtH, s);
       zm::launch_bt_vel(g, Pf, etaw, ubt, vbt, Frx, Fry, fdt, fden, dt_bt, drag, s);
     }
   }
 
-  void one_step(const zm_state& in, const zm_state& out, const zm::Phys<real>& Pf, hipStream_t s) {
+  // the member record of a batched barotropic substep (frozen forcing)
+  zm::BtMember<real> bt_member(int row0) const {
+    zm::BtMember<real> m{};
+    m.g = g;
+    m.P = P;
+    m.eta = etaw;
+    m.ubt = ubt;
+    m.vbt = vbt;
+    m.Frx = Frx;
+    m.Fry = Fry;
+    m.fdt = fdt;
+    m.fden = fden;
+    m.dtH = dtH;
+    m.dt_bt = dt_bt;
+    m.drag = drag;
+    m.row0 = row0;
+    return m;
+  }
+
+  // everything before / after the barotropic substeps of one step
+  void pre_barotropic(const zm_state& in, const zm::Phys<real>& Pf, hipStream_t s) {
     const real* ui = (const real*)in.u;
     const real* vi = (const real*)in.v;
     const real* Ti = (const real*)in.T;
     const real* Si = (const real*)in.S;
-    const real* ei = (const real*)in.eta;
     linear(ui, vi, Ti, Si, u1, v1, T1, S1, s);
-    nonlinear(Pf, ei, u2, v2, T2, S2, s);
+    nonlinear(Pf, (const real*)in.eta, u2, v2, T2, S2, s);
     linear(u2, v2, T2, S2, u3, v3, T3, S3, s);
-    barotropic(Pf, ei, s);
+  }
+
+  void post_barotropic(const zm_state& in, const zm_state& out, const zm::Phys<real>& Pf, hipStream_t s) {
     zm::launch_cap_means(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, s);
-    zm::launch_final(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, ui, vi, Ti, Si, (real*)out.u,
-                     (real*)out.v, (real*)out.T, (real*)out.S, s);
+    zm::launch_final(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, (const real*)in.u, (const real*)in.v,
+                     (const real*)in.T, (const real*)in.S, (real*)out.u, (real*)out.v, (real*)out.T, (real*)out.S, s);
     zm::launch_copy((const real*)etaw, (real*)out.eta, n2, s);
     if (out.ice != in.ice) zm::launch_copy((const real*)in.ice, (real*)out.ice, n2, s);
+  }
+
+  void one_step(const zm_state& in, const zm_state& out, const zm::Phys<real>& Pf, hipStream_t s) {
+    pre_barotropic(in, Pf, s);
+    barotropic(Pf, (const real*)in.eta, s);
+    post_barotropic(in, out, Pf, s);
+  }
+
+  void set_tiles(int nu, int seg) override {
+    if (nu < 1 || nu > zm::nu_block_max<real>()) throw ConfigError{"nu_block out of range"};
+    if (seg < 1 || seg > nz) throw ConfigError{"column segment out of range"};
+    nu_block = nu;
+    g.seg = seg;
+    clear_graphs();
+  }
+
+  void get_tiles(int* nu, int* seg) const override {
+    *nu = nu_block;
+    *seg = g.seg;
+  }
+
+  void clear_graphs() {
+    for (auto& kv : graphs) {
+      hipGraphExecDestroy(kv.second.second);
+      hipGraphDestroy(kv.second.first);
+    }
+    graphs.clear();
   }
 
   zm::Phys<real> forced(const void* const* forcing) const {
@@ -470,7 +548,7 @@ struct Solver : SolverBase {
         zm::launch_bt_rho(g, P, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
                           (real*)nullptr, s);
         zm::launch_mom_adv(g, ui, vi, Fz1, au, av, s);
-        zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+        dealias(au, av, axu, axv, s);
         zm::launch_mom_residual(g, P, 0, axu, axv, ui, vi, pbc, ei, Frx, Fry, (real*)out.u, (real*)out.v, tmpA,
                                 tmpB, decay_bot, s);
         break;
@@ -537,7 +615,7 @@ struct Solver : SolverBase {
                                   s);
           break;
         case 3:
-          zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
+          dealias(au, av, axu, axv, s);
           break;
         case 5:
           zm::launch_nu_block(g, (r % 2) ? nBu : nAu, (r % 2) ? nBv : nAv, (r % 2) ? nAu : nBu,
@@ -567,6 +645,9 @@ struct Solver : SolverBase {
         case 4:
           zm::launch_bt_eta(g, P, etaw, ubt, vbt, dtH, s);
           zm::launch_bt_vel(g, P, etaw, ubt, vbt, Frx, Fry, fdt, fden, dt_bt, drag, s);
+          break;
+        case 14:
+          zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);
           break;
         default:
           throw ConfigError{"unknown kernel selector"};
@@ -612,12 +693,89 @@ void validate(const zm_config& c) {
 
 int fail(zm_handle* h, const std::string& m, int code);
 
+int zm_bt_row_max() { return 1024; }
+
 }  // namespace
 
 struct zm_handle {
   SolverBase* solver = nullptr;
   int dtype_bytes = 0;
   std::string err;
+};
+
+// Uneven float32 members advanced together: the 3-D stages of each member in
+// turn, the barotropic sub-cycle as one batched launch per substep over a
+// member table, the final masks of each member; the whole ensemble step is one
+// hipGraph launch.
+struct zm_ensemble {
+  std::vector<Solver<float>*> mem;
+  zm::BtMember<float>* table = nullptr;
+  int total_rows = 0, n_sub = 0;
+  hipStream_t own = nullptr;
+  bool use_graph = true;
+  std::map<std::vector<const void*>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  std::string err;
+
+  ~zm_ensemble() {
+    for (auto& kv : graphs) {
+      hipGraphExecDestroy(kv.second.second);
+      hipGraphDestroy(kv.second.first);
+    }
+    if (own) hipStreamDestroy(own);
+    if (table) hipFree(table);
+  }
+
+  void issue(const zm_state* in, const zm_state* out, hipStream_t s) {
+    const int n = (int)mem.size();
+    for (int i = 0; i < n; ++i) {
+      mem[i]->pre_barotropic(in[i], mem[i]->P, s);
+      mem[i]->bt_prepare(mem[i]->P, (const float*)in[i].eta, s);
+    }
+    for (int q = 0; q < n_sub; ++q) zm::launch_bt_multi(table, n, total_rows, s);
+    for (int i = 0; i < n; ++i) mem[i]->post_barotropic(in[i], out[i], mem[i]->P, s);
+  }
+
+  void step(const zm_state* in, const zm_state* out, hipStream_t s) {
+    const int n = (int)mem.size();
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        if (i != j && (out[i].u == in[j].u || out[i].u == out[j].u))
+          throw ConfigError{"ensemble members must not share state buffers"};
+    if (!use_graph) {
+      issue(in, out, s);
+      ZM_HIP(hipGetLastError());
+      return;
+    }
+    std::vector<const void*> key;
+    for (int i = 0; i < n; ++i) {
+      const void* p[] = {in[i].u, in[i].v, in[i].T, in[i].S, in[i].eta, in[i].ice,
+                         out[i].u, out[i].v, out[i].T, out[i].S, out[i].eta, out[i].ice};
+      key.insert(key.end(), p, p + 12);
+    }
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      if (graphs.size() >= 4) {
+        auto first = graphs.begin();
+        ZM_HIP(hipStreamSynchronize(s));
+        hipGraphExecDestroy(first->second.second);
+        hipGraphDestroy(first->second.first);
+        graphs.erase(first);
+      }
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      ZM_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
+      issue(in, out, own);
+      hipError_t e = hipStreamEndCapture(own, &graph);
+      if (e != hipSuccess) throw HipError{e, "hipStreamEndCapture"};
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        throw HipError{e, "hipGraphInstantiate"};
+      }
+      it = graphs.emplace(key, std::make_pair(graph, exec)).first;
+    }
+    ZM_HIP(hipGraphLaunch(it->second.second, s));
+  }
 };
 
 namespace {
@@ -801,6 +959,158 @@ int zm_adam(int32_t n_out, int32_t n_in, float* w, const float* gw, float* m, fl
 int zm_time_kernel(zm_handle* h, int32_t which, int32_t reps, double* ms_per_launch) {
   if (!h || reps < 1 || !ms_per_launch) return fail(h, "invalid zm_time_kernel arguments", ZM_EINVAL);
   return guarded(h, [&] { *ms_per_launch = h->solver->time_kernel(which, reps); });
+}
+
+int zm_set_tiles(zm_handle* h, int32_t nu_block, int32_t seg) {
+  if (!h) return fail(h, "null handle", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->set_tiles(nu_block, seg); });
+}
+
+int zm_get_tiles(const zm_handle* h, int32_t* nu_block, int32_t* seg) {
+  if (!h || !nu_block || !seg) return fail(nullptr, "invalid zm_get_tiles arguments", ZM_EINVAL);
+  int a = 0, b = 0;
+  h->solver->get_tiles(&a, &b);
+  *nu_block = a;
+  *seg = b;
+  return ZM_OK;
+}
+
+int zm_autotune(zm_handle* h, const zm_state* sample, int32_t reps, int32_t* nu_block, int32_t* seg,
+                double* best_ms) {
+  if (!h || !sample || reps < 1 || !nu_block || !seg || !best_ms)
+    return fail(h, "invalid zm_autotune arguments", ZM_EINVAL);
+  return guarded(h, [&] {
+    SolverBase* sv = h->solver;
+    const size_t w = (size_t)h->dtype_bytes;
+    const size_t b3 = (size_t)sv->n2_ * sv->nz_ * w, b2 = (size_t)sv->n2_ * w;
+    // two scratch states for ping-pong timing, freed before returning
+    std::vector<void*> bufs;
+    auto alloc = [&](size_t bytes) {
+      void* p = nullptr;
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) {
+        for (void* q : bufs) hipFree(q);
+        throw HipError{e, "hipMalloc (autotune scratch)"};
+      }
+      bufs.push_back(p);
+      return p;
+    };
+    zm_state st[2];
+    for (auto& x : st) {
+      x.u = alloc(b3);
+      x.v = alloc(b3);
+      x.T = alloc(b3);
+      x.S = alloc(b3);
+      x.eta = alloc(b2);
+      x.ice = alloc(b2);
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ZM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    ZM_HIP(hipEventCreate(&e0));
+    ZM_HIP(hipEventCreate(&e1));
+    int best_nu = 1, best_seg = 1;
+    double best = 1e30;
+    std::vector<int> segs;
+    for (int c : {4, 5, 7, 10, 13, 17, 25, 50})
+      if (c <= sv->nz_) segs.push_back(c);
+    if (segs.empty() || segs.back() != sv->nz_) segs.push_back(sv->nz_);
+    int nu_max = 1, cur_seg = 1;
+    sv->get_tiles(&nu_max, &cur_seg);
+    const bool graph = sv->use_graph;
+    sv->use_graph = true;
+    try {
+      for (int nu = 1; nu <= nu_max; ++nu) {
+        for (int sg : segs) {
+          sv->set_tiles(nu, sg);
+          const void* src[] = {sample->u, sample->v, sample->T, sample->S, sample->eta, sample->ice};
+          void* dst[] = {st[0].u, st[0].v, st[0].T, st[0].S, st[0].eta, st[0].ice};
+          for (int f = 0; f < 6; ++f)
+            ZM_HIP(hipMemcpyAsync(dst[f], src[f], f < 4 ? b3 : b2, hipMemcpyDeviceToDevice, s));
+          sv->step(st[0], st[1], 1, nullptr, s);
+          sv->step(st[1], st[0], 1, nullptr, s);
+          ZM_HIP(hipEventRecord(e0, s));
+          for (int r = 0; r < reps; ++r) sv->step(st[r % 2], st[(r + 1) % 2], 1, nullptr, s);
+          ZM_HIP(hipEventRecord(e1, s));
+          ZM_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          ZM_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (ms / reps < best) {
+            best = ms / reps;
+            best_nu = nu;
+            best_seg = sg;
+          }
+        }
+      }
+    } catch (...) {
+      sv->use_graph = graph;
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      hipStreamDestroy(s);
+      for (void* q : bufs) hipFree(q);
+      throw;
+    }
+    sv->use_graph = graph;
+    sv->set_tiles(best_nu, best_seg);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(s);
+    for (void* q : bufs) hipFree(q);
+    *nu_block = best_nu;
+    *seg = best_seg;
+    *best_ms = best;
+  });
+}
+
+int zm_ensemble_create(zm_handle* const* members, int32_t n, zm_ensemble** out) {
+  if (!members || n < 1 || !out) return fail(nullptr, "invalid zm_ensemble_create arguments", ZM_EINVAL);
+  *out = nullptr;
+  zm_ensemble* e = new zm_ensemble();
+  int rc = guarded(nullptr, [&] {
+    std::vector<zm::BtMember<float>> host;
+    for (int i = 0; i < n; ++i) {
+      if (!members[i]) throw ConfigError{"null member handle"};
+      auto* sv = dynamic_cast<Solver<float>*>(members[i]->solver);
+      if (!sv) throw ConfigError{"ensemble members must be float32 handles"};
+      for (auto* other : e->mem)
+        if (other == sv) throw ConfigError{"a handle may appear only once in an ensemble"};
+      if (i == 0) e->n_sub = sv->n_sub;
+      if (sv->n_sub != e->n_sub) throw ConfigError{"ensemble members must share the barotropic substep count"};
+      if (sv->nx > zm_bt_row_max()) throw ConfigError{"nx too large for the batched barotropic substep"};
+      host.push_back(sv->bt_member(e->total_rows));
+      e->total_rows += sv->ny;
+      e->mem.push_back(sv);
+    }
+    ZM_HIP(hipMalloc(&e->table, host.size() * sizeof(zm::BtMember<float>)));
+    ZM_HIP(hipMemcpy(e->table, host.data(), host.size() * sizeof(zm::BtMember<float>), hipMemcpyHostToDevice));
+    ZM_HIP(hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking));
+  });
+  if (rc != ZM_OK) {
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return ZM_OK;
+}
+
+int zm_ensemble_step(zm_ensemble* e, const zm_state* in, const zm_state* out, void* stream) {
+  if (!e || !in || !out) return fail(nullptr, "invalid zm_ensemble_step arguments", ZM_EINVAL);
+  int rc = guarded(nullptr, [&] { e->step(in, out, (hipStream_t)stream); });
+  if (rc != ZM_OK) e->err = g_thread_error;
+  return rc;
+}
+
+int zm_ensemble_set_graph(zm_ensemble* e, int32_t enabled) {
+  if (!e) return fail(nullptr, "null ensemble", ZM_EINVAL);
+  e->use_graph = enabled != 0;
+  return ZM_OK;
+}
+
+int32_t zm_ensemble_size(const zm_ensemble* e) { return e ? (int32_t)e->mem.size() : 0; }
+
+int zm_ensemble_destroy(zm_ensemble* e) {
+  delete e;
+  return ZM_OK;
 }
 
 }  // extern "C"

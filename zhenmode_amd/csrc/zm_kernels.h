@@ -6,6 +6,12 @@
 
 namespace zm {
 
+// Radix sequence of a mixed-radix FFT of length n (radices 2, 3, 4, 5).
+struct FftPlan {
+  int nst;
+  int rad[16];
+};
+
 template <typename real>
 void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,
                        real* lv, real nu_h, real dts, hipStream_t s);
@@ -44,6 +50,9 @@ template <typename real>
 void launch_dealias_x(int N, int N1, int N2, const int* kept, int nrows, const real* twr, const real* twi,
                       const real* ina, const real* inb, real* outa, real* outb, hipStream_t s);
 template <typename real>
+void launch_dealias_fft(int N, const FftPlan& plan, const int* kept, int nrows, const real* twr, const real* twi,
+                        const real* ina, const real* inb, real* outa, real* outb, hipStream_t s);
+template <typename real>
 void launch_mom_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* ax, const real* ay,
                          const real* u, const real* v, const real* pbc, const real* eta, const real* Frx,
                          const real* Fry, real* du1, real* dv1, real* uo, real* vo, real decay_bot,
@@ -55,6 +64,28 @@ template <typename real>
 void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, real* ubt, real* vbt,
                    const real* Frx, const real* Fry, const real* fdt, const real* fden, real dt_bt, real drag,
                    hipStream_t s);
+// A member of a batched barotropic substep: one grid's constants and state.
+// row0 = first workgroup of this member in the batched launch.
+template <typename real>
+struct BtMember {
+  Geo<real> g;
+  Phys<real> P;
+  real* eta;
+  real* ubt;
+  real* vbt;
+  const real* Frx;
+  const real* Fry;
+  const real* fdt;
+  const real* fden;
+  real dtH, dt_bt, drag;
+  int row0;
+};
+
+// One barotropic substep (eta then velocity kernel) for every member of a
+// device-resident table.
+template <typename real>
+void launch_bt_multi(const BtMember<real>* table, int nmem, int total_rows, hipStream_t s);
+
 template <typename real>
 void launch_cap_means(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
                       const real* S, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,

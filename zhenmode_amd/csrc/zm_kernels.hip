@@ -41,6 +41,20 @@ __device__ __forceinline__ Nb make_nb(const Geo<real>& g, int i, int j) {
   return n;
 }
 
+// Combination of the Laplacian's three terms and the nu_h update with
+// explicit fused multiply-adds: the single-substep and the register-strip
+// kernels then round identically whatever the compiler's contraction choices,
+// so the strip blocking never changes results.
+template <typename real>
+__device__ __forceinline__ real lap_sum(real dx, real dy, real dyc, real id2, real idy2, real hidy, real tr) {
+  return fma(dx, id2, fma(dy, idy2, -(dyc * hidy) * tr));
+}
+
+template <typename real>
+__device__ __forceinline__ real nu_update(real x, real L, real nu_h, real dts) {
+  return fma(nu_h * L, dts, x);
+}
+
 // _laplacian_h legacy branch (numerics/horizontal.py:181-215).
 template <typename real>
 __device__ __forceinline__ real lap_h(const real* __restrict__ f, long long plane, const Nb& n,
@@ -53,10 +67,7 @@ __device__ __forceinline__ real lap_h(const real* __restrict__ f, long long plan
   const real fw = (wc && k < n.nwW) ? (u - uw) : real(0);
   const real fn = (wc && k < n.nwN) ? (un - u) : real(0);
   const real fs = (wc && k < n.nwS) ? (u - us) : real(0);
-  const real d2x = (fe - fw) * g.idx2[n.j];
-  const real d2y = (fn - fs) * g.idy2;
-  const real dudy = (un - us) * g.hidy;
-  return d2x + d2y + (-dudy * g.tanR[n.j]);
+  return lap_sum(fe - fw, fn - fs, un - us, g.idx2[n.j], g.idy2, g.hidy, g.tanR[n.j]);
 }
 
 // _horizontal_diffusion_flux (numerics/horizontal.py:217-238); kap == nullptr
@@ -154,18 +165,19 @@ __global__ __launch_bounds__(256) void k_nu_substep(Geo<real> g, const real* __r
   ZM_CELL_PROLOGUE
   const real Lu = lap_h(u, plane, n, k, g);
   const real Lv = lap_h(v, plane, n, k, g);
-  uo[c] = u[c] + nu_h * Lu * dts;
-  vo[c] = v[c] + nu_h * Lv * dts;
+  uo[c] = nu_update(u[c], Lu, nu_h, dts);
+  vo[c] = nu_update(v[c], Lv, nu_h, dts);
   if (lu) {
     lu[c] = Lu;
     lv[c] = Lv;
   }
 }
 
-// Cross-lane moves (DPP wave shifts): lane l receives lane l+1 (from_right)
-// or lane l-1 (from_left); lanes shifted in from outside the wave read 0.
-__device__ __forceinline__ int dpp_from_right(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, false); }
-__device__ __forceinline__ int dpp_from_left(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
+// Cross-lane moves (DPP wave_shl:1 / wave_shr:1 with bound_ctrl): lane l
+// receives lane l+1 (from_right) or lane l-1 (from_left); lanes shifted in
+// from outside the wave read 0 (only halo lanes, never stored).
+__device__ __forceinline__ int dpp_from_right(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_from_left(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
 __device__ __forceinline__ float dpp_from_right(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
 __device__ __forceinline__ float dpp_from_left(float x) { return __int_as_float(dpp_from_left(__float_as_int(x))); }
 __device__ __forceinline__ double dpp_from_right(double x) {
@@ -211,13 +223,55 @@ __device__ __forceinline__ real lap_strip(real x, real xe, real xw, real xn, rea
                                           bool fn_on, bool fs_on, real id2, real idy2, real hidy, real tr) {
   const real fe = fe_on ? (xe - x) : real(0), fw = fw_on ? (x - xw) : real(0);
   const real fn = fn_on ? (xn - x) : real(0), fs = fs_on ? (x - xs) : real(0);
-  return (fe - fw) * id2 + (fn - fs) * idy2 + (-((xn - xs) * hidy) * tr);
+  return lap_sum(fe - fw, fn - fs, xn - xs, id2, idy2, hidy, tr);
 }
 
 template <typename real>
 __device__ __forceinline__ real lap_open(real x, real xe, real xw, real xn, real xs, real id2, real idy2,
                                          real hidy, real tr) {
-  return ((xe - x) - (x - xw)) * id2 + ((xn - x) - (x - xs)) * idy2 + (-((xn - xs) * hidy) * tr);
+  return lap_sum((xe - x) - (x - xw), (xn - x) - (x - xs), xn - xs, id2, idy2, hidy, tr);
+}
+
+// nsub substeps of the strip; WALL = the tile holds a closed-wall row (its
+// own mirror neighbour), otherwise the north/south selects are compiled out.
+template <typename real, int H, bool WALL>
+__device__ __forceinline__ void nu_strip_substeps(real (&U)[H], real (&V)[H], real lane_idx2, real lane_tan,
+                                                  unsigned int me, unsigned int mw, unsigned int mn, unsigned int ms,
+                                                  unsigned int wallN, unsigned int wallS, unsigned int open_rows,
+                                                  real idy2, real hidy, real nu_h, real dts, int nsub) {
+  for (int s = 0; s < nsub; ++s) {
+    // keep one copy of the row metrics instead of H hoisted scalars
+    asm volatile("" : "+v"(lane_idx2), "+v"(lane_tan), "+v"(me), "+v"(mw), "+v"(mn), "+v"(ms), "+s"(wallN),
+                 "+s"(wallS), "+s"(open_rows));
+    real pu = U[0], pv = V[0];  // old values of row r-1
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const real id2 = readlane(lane_idx2, r), tr = readlane(lane_tan, r);
+      const bool atN = WALL && ((wallN >> r) & 1u), atS = WALL && ((wallS >> r) & 1u);
+      const int rn = r + 1 < H ? r + 1 : r;
+      // ordering barrier: keep each row's neighbour moves next to its update
+      asm volatile("" : "+v"(U[r]), "+v"(V[r]));
+      const real x = U[r], y = V[r];
+      const real xn = atN ? x : U[rn], xs = atS ? x : pu;
+      const real yn = atN ? y : V[rn], ys = atS ? y : pv;
+      const real xe = dpp_from_right(x), xw = dpp_from_left(x);
+      const real ye = dpp_from_right(y), yw = dpp_from_left(y);
+      real Lx, Ly;
+      if ((open_rows >> r) & 1u) {
+        Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
+        Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+      } else {
+        const bool fe_on = (me >> r) & 1u, fw_on = (mw >> r) & 1u;
+        const bool fn_on = (mn >> r) & 1u, fs_on = (ms >> r) & 1u;
+        Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+        Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+      }
+      pu = x;
+      pv = y;
+      U[r] = nu_update(x, Lx, nu_h, dts);
+      V[r] = nu_update(y, Ly, nu_h, dts);
+    }
+  }
 }
 
 template <typename real, int H, int HALO>
@@ -284,39 +338,8 @@ __global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __res
     lane_idx2 = g.idx2[j];
     lane_tan = g.tanR[j];
   }
-  for (int s = 0; s < nsub; ++s) {
-    // keep one copy of the row metrics instead of H hoisted scalars
-    asm volatile("" : "+v"(lane_idx2), "+v"(lane_tan), "+v"(me), "+v"(mw), "+v"(mn), "+v"(ms), "+s"(wallN),
-                 "+s"(wallS), "+s"(open_rows));
-    real pu = U[0], pv = V[0];  // old values of row r-1
-#pragma unroll
-    for (int r = 0; r < H; ++r) {
-      const real id2 = readlane(lane_idx2, r), tr = readlane(lane_tan, r);
-      const bool atN = (wallN >> r) & 1u, atS = (wallS >> r) & 1u;
-      const int rn = r + 1 < H ? r + 1 : r;
-      // ordering barrier: keep each row's neighbour moves next to its update
-      asm volatile("" : "+v"(U[r]), "+v"(V[r]));
-      const real x = U[r], y = V[r];
-      const real xn = atN ? x : U[rn], xs = atS ? x : pu;
-      const real yn = atN ? y : V[rn], ys = atS ? y : pv;
-      const real xe = dpp_from_right(x), xw = dpp_from_left(x);
-      const real ye = dpp_from_right(y), yw = dpp_from_left(y);
-      real Lx, Ly;
-      if ((open_rows >> r) & 1u) {
-        Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
-        Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
-      } else {
-        const bool fe_on = (me >> r) & 1u, fw_on = (mw >> r) & 1u;
-        const bool fn_on = (mn >> r) & 1u, fs_on = (ms >> r) & 1u;
-        Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
-        Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
-      }
-      pu = x;
-      pv = y;
-      U[r] = x + nu_h * Lx * dts;
-      V[r] = y + nu_h * Ly * dts;
-    }
-  }
+  nu_strip_substeps<real, H, true>(U, V, lane_idx2, lane_tan, me, mw, mn, ms, wallN, wallS, open_rows, idy2, hidy,
+                                   nu_h, dts, nsub);
   const int io = tx * OX + (lane - HALO);
   real* __restrict__ uop = uo + plane;
   real* __restrict__ vop = vo + plane;
@@ -357,13 +380,12 @@ __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __rest
 
 // ---------------------------------------------------------------------------
 // Column-marching form.  A thread owns one (i, j) column over a segment of
-// kSeg levels; vertical neighbours come from a register window, the column's
+// g.seg levels; vertical neighbours come from a register window, the column's
 // 2-D data (wet counts, coefficients, metrics, forcing) is loaded once, and
 // each level only loads its horizontal neighbours.  The operators below take
 // explicit neighbour values and evaluate the same expressions as lap_h,
 // d1_flux and d2dz2.
 // ---------------------------------------------------------------------------
-constexpr int kSeg = 10;
 
 #define ZM_COLUMN_PROLOGUE                                                   \
   const int ntx_ = (g.nx + kTileI - 1) / kTileI;                             \
@@ -376,8 +398,8 @@ constexpr int kSeg = 10;
   const int i = tx_ * kTileI + (int)(threadIdx.x % kTileI);                  \
   const int j = ty_ * kTileJ + (int)(threadIdx.x / kTileI);                  \
   if (i >= g.nx || j >= g.ny) return;                                        \
-  const int k0 = seg_ * kSeg;                                                \
-  const int k1 = k0 + kSeg < g.nz ? k0 + kSeg : g.nz;                        \
+  const int k0 = seg_ * g.seg;                                               \
+  const int k1 = k0 + g.seg < g.nz ? k0 + g.seg : g.nz;                      \
   const Nb n = make_nb(g, i, j);                                             \
   const int c2 = n.c2;                                                       \
   const long long n2 = g.n2;                                                 \
@@ -425,7 +447,7 @@ __device__ __forceinline__ real ddz_win(real xm, real xc, real xp, int k, int kb
 template <typename real>
 static inline unsigned segs3(const Geo<real>& g) {
   return (unsigned)((long long)((g.nx + kTileI - 1) / kTileI) * ((g.ny + kTileJ - 1) / kTileJ) *
-                    ((g.nz + kSeg - 1) / kSeg));
+                    ((g.nz + g.seg - 1) / g.seg));
 }
 
 // Remainder of _linear_half_step after the nu_h sub-cycle (step.py:111-173):
@@ -977,6 +999,151 @@ __global__ __launch_bounds__(256) void k_dealias_x(int N, int N1, int N2, const 
 }
 
 // ---------------------------------------------------------------------------
+// Same projection with a mixed-radix (2, 3, 4, 5) Stockham FFT in LDS (the
+// autosorting formulation of Govindaraju et al., "High Performance Discrete
+// Fourier Transforms on Graphics Processors", SC 2008): stage s with radix R
+// reads R points at stride N/R, twiddles them by exp(-+2 pi i r k / (Ns R)),
+// does an R-point DFT and writes them at stride Ns, Ns *= R.  A workgroup
+// transforms kFftRowsPerBlock complex rows in turn with the twiddle table
+// held in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kFftRowsPerBlock = 4;
+
+// cos and sin of 2 pi m / R for the butterfly radices (mathematical constants)
+template <int R>
+struct RootsOfUnity;
+template <>
+struct RootsOfUnity<2> {
+  static constexpr double c[2] = {1.0, -1.0}, s[2] = {0.0, 0.0};
+};
+template <>
+struct RootsOfUnity<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+};
+template <>
+struct RootsOfUnity<4> {
+  static constexpr double c[4] = {1.0, 0.0, -1.0, 0.0}, s[4] = {0.0, 1.0, 0.0, -1.0};
+};
+template <>
+struct RootsOfUnity<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
+                                  0.30901699437494742410};
+  static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
+                                  -0.95105651629515357212};
+};
+
+// one radix-R butterfly of a Stockham stage: gather, twiddle, R-point DFT
+// with constant roots (exp(-2 pi i m / R), conjugated for the inverse), scatter
+template <typename real, int R>
+__device__ __forceinline__ void fft_butterfly(int bf, int N, int Ns, int tstep, bool inverse,
+                                              const real* __restrict__ src, real* __restrict__ dst,
+                                              const real* __restrict__ wr, const real* __restrict__ wi) {
+  const int T = N / R;
+  const int k = bf % Ns;
+  real vr[R], vi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const real xr = src[2 * (bf + r * T)], xi = src[2 * (bf + r * T) + 1];
+    if (r == 0) {
+      vr[0] = xr;
+      vi[0] = xi;
+    } else {
+      const int e = (r * k * tstep) % N;
+      const real c = wr[e], sn = inverse ? -wi[e] : wi[e];
+      vr[r] = xr * c - xi * sn;
+      vi[r] = xr * sn + xi * c;
+    }
+  }
+  const real sg = inverse ? real(1) : real(-1);
+  const int base = (bf / Ns) * Ns * R + k;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    real sr = vr[0], si = vi[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      const int m = (r * q) % R;
+      const real c = real(RootsOfUnity<R>::c[m]), sn = sg * real(RootsOfUnity<R>::s[m]);
+      sr += vr[r] * c - vi[r] * sn;
+      si += vr[r] * sn + vi[r] * c;
+    }
+    dst[2 * (base + q * Ns)] = sr;
+    dst[2 * (base + q * Ns) + 1] = si;
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void fft_stages(int N, const FftPlan& plan, bool inverse, real* __restrict__ a,
+                                           real* __restrict__ b, const real* __restrict__ wr,
+                                           const real* __restrict__ wi, real*& out) {
+  real* src = a;
+  real* dst = b;
+  int Ns = 1;
+  for (int st = 0; st < plan.nst; ++st) {
+    const int R = plan.rad[st];
+    const int T = N / R;
+    const int tstep = N / (Ns * R);
+    for (int bf = threadIdx.x; bf < T; bf += blockDim.x) {
+      switch (R) {
+        case 2: fft_butterfly<real, 2>(bf, N, Ns, tstep, inverse, src, dst, wr, wi); break;
+        case 3: fft_butterfly<real, 3>(bf, N, Ns, tstep, inverse, src, dst, wr, wi); break;
+        case 4: fft_butterfly<real, 4>(bf, N, Ns, tstep, inverse, src, dst, wr, wi); break;
+        default: fft_butterfly<real, 5>(bf, N, Ns, tstep, inverse, src, dst, wr, wi); break;
+      }
+    }
+    __syncthreads();
+    real* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  out = src;
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_dealias_fft(int N, FftPlan plan, const int* __restrict__ kept, int nrows,
+                                                     const real* __restrict__ twr, const real* __restrict__ twi,
+                                                     const real* __restrict__ ina, const real* __restrict__ inb,
+                                                     real* __restrict__ outa, real* __restrict__ outb) {
+  extern __shared__ unsigned char zm_lds[];
+  real* wr = reinterpret_cast<real*>(zm_lds);
+  real* wi = wr + N;
+  real* A = wi + N;      // interleaved complex row, 2N
+  real* B = A + 2 * N;   // ping-pong buffer, 2N
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    wr[t] = twr[t];
+    wi[t] = twi[t];
+  }
+  const real invN = real(1) / real(N);
+  for (int q = 0; q < kFftRowsPerBlock; ++q) {
+    const int row = blockIdx.x * kFftRowsPerBlock + q;
+    if (row >= nrows) break;  // uniform
+    const long long off = (long long)row * N;
+    __syncthreads();
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+      A[2 * t] = ina[off + t];
+      A[2 * t + 1] = inb[off + t];
+    }
+    __syncthreads();
+    real* X = nullptr;
+    fft_stages(N, plan, false, A, B, wr, wi, X);
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+      if (!kept[t]) {
+        X[2 * t] = 0;
+        X[2 * t + 1] = 0;
+      }
+    }
+    __syncthreads();
+    real* Y = nullptr;
+    fft_stages(N, plan, true, X, X == A ? B : A, wr, wi, Y);
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+      outa[off + t] = Y[2 * t] * invN;
+      outb[off + t] = Y[2 * t + 1] * invN;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Momentum residual (tendencies.py:65-98 and 273-345, legacy mode-split
 // branch) fused with the RK2 velocity update (step.py:263-319).
 //   mode 0: du1 -> buffers, u_pred = u + du1 dt
@@ -1126,12 +1293,12 @@ __device__ __forceinline__ real cap_weight(const Phys<real>& P, int j, int ny) {
 
 constexpr int kRowChunk = 4;  // cells per thread per row (nx <= 1024)
 
+// eta' = eta - dt H div(ubt) on row j with the polar-cap blend of band rows;
+// reads eta_row[i] (row j of eta), writes out[i].  out may alias eta_row.
 template <typename real>
-__global__ __launch_bounds__(256) void k_bt_eta(Geo<real> g, Phys<real> P, real* __restrict__ eta,
-                                                const real* __restrict__ ubt, const real* __restrict__ vbt,
-                                                real dtH) {
-  __shared__ real red[8];
-  const int j = blockIdx.x;
+__device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>& P, int j,
+                                           const real* __restrict__ eta_row, const real* __restrict__ ubt,
+                                           const real* __restrict__ vbt, real dtH, real* out, real* red) {
   const real* wet = P.wet2;
   real en[kRowChunk];
   real ssum = 0, wsum = 0;
@@ -1150,7 +1317,7 @@ __global__ __launch_bounds__(256) void k_bt_eta(Geo<real> g, Phys<real> P, real*
     if (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) fn = real(0.5) * (vbt[c2] + vbt[c2 + g.nx]) * g.cfn[j];
     if (j > 0 && wc && wet[c2 - g.nx] != real(0)) fs = real(0.5) * (vbt[c2 - g.nx] + vbt[c2]) * g.cfn[j - 1];
     const real divy = (fn - fs) * g.idyc[j];
-    const real e = wc ? eta[c2] - dtH * (divx + divy) : real(0);
+    const real e = wc ? eta_row[i] - dtH * (divx + divy) : real(0);
     en[q] = e;
     ssum += e;
     wsum += wet[c2];
@@ -1169,22 +1336,23 @@ __global__ __launch_bounds__(256) void k_bt_eta(Geo<real> g, Phys<real> P, real*
     const int c2 = j * g.nx + i;
     real e = en[q];
     if (wb >= real(0)) e = wet[c2] != real(0) ? wb * zm + (real(1) - wb) * e : real(0);
-    eta[c2] = e;
+    out[i] = e;
   }
 }
 
+// ubt, vbt update of row j from the new eta (rows j-1, j, j+1 given as row
+// pointers; en / es are only read when that row exists): gradient, forcing,
+// legacy implicit Coriolis, drag, mask, polar-cap blend.  Outputs may alias
+// the inputs.
 template <typename real>
-__global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const real* __restrict__ eta,
-                                                real* __restrict__ ubt, real* __restrict__ vbt,
-                                                const real* __restrict__ Frx, const real* __restrict__ Fry,
-                                                const real* __restrict__ fdt, const real* __restrict__ fden,
-                                                real dt_bt, real drag) {
-  __shared__ real red[16];
-  const int j = blockIdx.x;
+__device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>& P, int j, const real* ec_row,
+                                           const real* en_row, const real* es_row, const real* __restrict__ ubt,
+                                           const real* __restrict__ vbt, real* uo, real* vo,
+                                           const real* __restrict__ Frx, const real* __restrict__ Fry, real fd,
+                                           real den, real dt_bt, real drag, real* red) {
   const real* wet = P.wet2;
   real un[kRowChunk], vn[kRowChunk];
   real su = 0, sv = 0, sw = 0;
-  const real fd = fdt[j], den = fden[j];
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
     const int i = threadIdx.x + q * 256;
@@ -1192,12 +1360,12 @@ __global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const
     if (i >= g.nx) continue;
     const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
     const bool wc = wet[c2] != real(0);
-    const real ec = eta[c2];
-    const real dxp = (wc && wet[e2] != real(0)) ? eta[e2] - ec : real(0);
-    const real dxm = (wc && wet[w2] != real(0)) ? ec - eta[w2] : real(0);
+    const real ec = ec_row[i];
+    const real dxp = (wc && wet[e2] != real(0)) ? ec_row[wrap_e(i, g.nx)] - ec : real(0);
+    const real dxm = (wc && wet[w2] != real(0)) ? ec - ec_row[wrap_w(i, g.nx)] : real(0);
     const real gx = g.idx[j] * real(0.5) * (dxp + dxm);
-    const real dyp = (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) ? (eta[c2 + g.nx] - ec) * g.cfn[j] : real(0);
-    const real dym = (j > 0 && wc && wet[c2 - g.nx] != real(0)) ? (ec - eta[c2 - g.nx]) * g.cfs[j] : real(0);
+    const real dyp = (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) ? (en_row[i] - ec) * g.cfn[j] : real(0);
+    const real dym = (j > 0 && wc && wet[c2 - g.nx] != real(0)) ? (ec - es_row[i]) * g.cfs[j] : real(0);
     const real gy = g.idyc[j] * real(0.5) * (dyp + dym);
     const real Fx = Frx[c2] + P.tau_x[c2] * P.inv_rho0H;
     const real Fy = Fry[c2] + P.tau_y[c2] * P.inv_rho0H;
@@ -1234,9 +1402,62 @@ __global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const
       a = wc ? wb * zu + (real(1) - wb) * a : real(0);
       b = wc ? wb * zv + (real(1) - wb) * b : real(0);
     }
-    ubt[c2] = a;
-    vbt[c2] = b;
+    uo[c2] = a;
+    vo[c2] = b;
   }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_eta(Geo<real> g, Phys<real> P, real* __restrict__ eta,
+                                                const real* __restrict__ ubt, const real* __restrict__ vbt,
+                                                real dtH) {
+  __shared__ real red[8];
+  const int j = blockIdx.x;
+  bt_eta_row(g, P, j, eta + (long long)j * g.nx, ubt, vbt, dtH, eta + (long long)j * g.nx, red);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const real* __restrict__ eta,
+                                                real* __restrict__ ubt, real* __restrict__ vbt,
+                                                const real* __restrict__ Frx, const real* __restrict__ Fry,
+                                                const real* __restrict__ fdt, const real* __restrict__ fden,
+                                                real dt_bt, real drag) {
+  __shared__ real red[16];
+  const int j = blockIdx.x;
+  const real* ec = eta + (long long)j * g.nx;
+  bt_vel_row(g, P, j, ec, ec + g.nx, ec - g.nx, ubt, vbt, ubt, vbt, Frx, Fry, fdt[j], fden[j], dt_bt, drag, red);
+}
+
+// The substep's two kernels for many independent grids in one launch each:
+// workgroup b serves row b - row0[m] of member m (row0 ascending, a member
+// table in device memory), with exactly the arithmetic of k_bt_eta / k_bt_vel.
+template <typename real>
+__device__ __forceinline__ int bt_member_of(const BtMember<real>* __restrict__ mem, int nmem, int b) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {  // last member with row0 <= b
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].row0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_eta_multi(const BtMember<real>* __restrict__ mem, int nmem) {
+  __shared__ real red[8];
+  const BtMember<real>& m = mem[bt_member_of(mem, nmem, (int)blockIdx.x)];
+  const int j = (int)blockIdx.x - m.row0;
+  real* row = m.eta + (long long)j * m.g.nx;
+  bt_eta_row(m.g, m.P, j, row, m.ubt, m.vbt, m.dtH, row, red);
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bt_vel_multi(const BtMember<real>* __restrict__ mem, int nmem) {
+  __shared__ real red[16];
+  const BtMember<real>& m = mem[bt_member_of(mem, nmem, (int)blockIdx.x)];
+  const int j = (int)blockIdx.x - m.row0;
+  const real* ec = m.eta + (long long)j * m.g.nx;
+  bt_vel_row(m.g, m.P, j, ec, ec + m.g.nx, ec - m.g.nx, m.ubt, m.vbt, m.ubt, m.vbt, m.Frx, m.Fry, m.fdt[j],
+             m.fden[j], m.dt_bt, m.drag, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1437,6 +1658,15 @@ void launch_dealias_x(int N, int N1, int N2, const int* kept, int nrows, const r
 }
 
 template <typename real>
+void launch_dealias_fft(int N, const FftPlan& plan, const int* kept, int nrows, const real* twr, const real* twi,
+                        const real* ina, const real* inb, real* outa, real* outb, hipStream_t s) {
+  const size_t lds = (size_t)6 * N * sizeof(real);
+  const unsigned blocks = (unsigned)((nrows + kFftRowsPerBlock - 1) / kFftRowsPerBlock);
+  hipLaunchKernelGGL(k_dealias_fft<real>, dim3(blocks), dim3(256), lds, s, N, plan, kept, nrows, twr, twi, ina, inb,
+                     outa, outb);
+}
+
+template <typename real>
 void launch_mom_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* ax, const real* ay,
                          const real* u, const real* v, const real* pbc, const real* eta, const real* Frx,
                          const real* Fry, real* du1, real* dv1, real* uo, real* vo, real decay_bot,
@@ -1457,6 +1687,12 @@ void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, rea
                    hipStream_t s) {
   hipLaunchKernelGGL(k_bt_vel<real>, dim3(g.ny), dim3(256), 0, s, g, P, eta, ubt, vbt, Frx, Fry, fdt, fden,
                      dt_bt, drag);
+}
+
+template <typename real>
+void launch_bt_multi(const BtMember<real>* table, int nmem, int total_rows, hipStream_t s) {
+  hipLaunchKernelGGL(k_bt_eta_multi<real>, dim3(total_rows), dim3(256), 0, s, table, nmem);
+  hipLaunchKernelGGL(k_bt_vel_multi<real>, dim3(total_rows), dim3(256), 0, s, table, nmem);
 }
 
 template <typename real>
@@ -1515,6 +1751,8 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                      hipStream_t);                                                           \
   template void launch_dealias_x<real>(int, int, int, const int*, int, const real*, const real*, const real*, \
                                        const real*, real*, real*, hipStream_t);                              \
+  template void launch_dealias_fft<real>(int, const FftPlan&, const int*, int, const real*, const real*,     \
+                                         const real*, const real*, real*, real*, hipStream_t);                \
   template void launch_mom_residual<real>(const Geo<real>&, const Phys<real>&, int, const real*,            \
                                           const real*, const real*, const real*, const real*, const real*,  \
                                           const real*, const real*, real*, real*, real*, real*, real,       \
@@ -1524,6 +1762,7 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
   template void launch_bt_vel<real>(const Geo<real>&, const Phys<real>&, const real*, real*, real*,         \
                                     const real*, const real*, const real*, const real*, real, real,         \
                                     hipStream_t);                                                            \
+  template void launch_bt_multi<real>(const BtMember<real>*, int, int, hipStream_t);                        \
   template void launch_cap_means<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,       \
                                        const real*, const real*, const real*, const real*, const real*,     \
                                        const real*, real*, hipStream_t);                                     \

@@ -161,15 +161,20 @@ class ColumnMLP:
 
     # ---------------------------------------------------------- normalisation
     def fit_normalisation(self, X_raw, dX):
-        """Per-feature mean / inverse std of raw features (float32 rows) and
-        per-output mean / std of the raw tendencies dX (rows x 2 nz)."""
+        """Per-feature mean / inverse std of raw features (float32 rows);
+        per-output mean and per-tracer std of the raw tendencies dX (rows x 2 nz)."""
         mu = X_raw.mean(dim=0)
         sd = X_raw.std(dim=0)
         inv = torch.where(sd > 1e-12, 1.0 / sd.clamp_min(1e-30), torch.zeros_like(sd))
         self.in_mean.copy_(torch.where(sd > 1e-12, mu, torch.zeros_like(mu)))
         self.in_inv_std.copy_(inv)
+        # one scale per tracer (not per level): the loss then weighs absolute
+        # errors of each tracer equally at every level, where they matter
+        nz = self.nz
         om = dX.mean(dim=0)
-        osd = dX.std(dim=0).clamp_min(1e-12)
+        osd = torch.empty_like(om)
+        osd[:nz] = dX[:, :nz].std().clamp_min(1e-12)
+        osd[nz:] = dX[:, nz:].std().clamp_min(1e-12)
         self.out_mean.zero_()
         self.out_std.fill_(1.0)
         self.out_mean[:self.n_out] = om

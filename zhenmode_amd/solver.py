@@ -189,6 +189,24 @@ class Solver:
         _lib.check(rc, self.handle)
         return src._replace(T=To, S=So)
 
+    def set_tiles(self, nu_block, seg):
+        _lib.check(self.lib.zm_set_tiles(self.handle, int(nu_block), int(seg)), self.handle)
+
+    def get_tiles(self):
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(self.lib.zm_get_tiles(self.handle, ctypes.byref(a), ctypes.byref(b)), self.handle)
+        return a.value, b.value
+
+    def autotune(self, sample, reps=5):
+        """Pick (nu_block, seg) by timing steps from ``sample``; returns (nu_block, seg, ms)."""
+        src = self.coerce(sample)
+        a, b, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        torch.cuda.synchronize(self.device)
+        rc = self.lib.zm_autotune(self.handle, ctypes.byref(self.zm_state(src)), int(reps), ctypes.byref(a),
+                                  ctypes.byref(b), ctypes.byref(ms))
+        _lib.check(rc, self.handle)
+        return a.value, b.value, ms.value
+
     def init_state(self, T_init=None, S_init=None):
         """Reference factory.py:766-788: zero velocity/eta/ice, land and ghost
         nodes set to the T_ref/S_ref sentinel."""
@@ -248,6 +266,41 @@ def make_solver_global(grid, physics, dt, forcing=None, *, return_params=False, 
     if return_params:
         return step, init_state, diagnostics, params, terms_fn
     return step, init_state, diagnostics
+
+
+class Ensemble:
+    """Float32 solvers of different sizes stepped together (``zm_ensemble_*``)."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        self.lib = _lib.load()
+        arr = (ctypes.c_void_p * len(self.solvers))(*(s.handle.value for s in self.solvers))
+        h = ctypes.c_void_p()
+        rc = self.lib.zm_ensemble_create(arr, len(self.solvers), ctypes.byref(h))
+        _lib.check(rc, None)
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self.lib.zm_ensemble_destroy(h)
+            self.handle = None
+
+    def set_graph(self, enabled):
+        _lib.check(self.lib.zm_ensemble_set_graph(self.handle, int(bool(enabled))), None)
+
+    def step(self, states, out=None):
+        if len(states) != len(self.solvers):
+            raise ValueError("one state per member is required")
+        src = [s.coerce(st) for s, st in zip(self.solvers, states)]
+        dst = [s.new_state() for s in self.solvers] if out is None else list(out)
+        n = len(self.solvers)
+        a_in = (_lib.ZmState * n)(*(Solver.zm_state(x) for x in src))
+        a_out = (_lib.ZmState * n)(*(Solver.zm_state(x) for x in dst))
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.solvers[0].device).cuda_stream)
+        rc = self.lib.zm_ensemble_step(self.handle, a_in, a_out, stream)
+        _lib.check(rc, None)
+        return dst
 
 
 def to_numpy_state(state):
