@@ -75,6 +75,122 @@ def cpu_baseline():
                        f"scaled to 720x320x{nz} by the algorithmic-byte ratio {b2 / b1:.1f}x")}
 
 
+def learned_leg(case, plain_ms, steps, warmup):
+    """The learned+mixed step: the plain FP32 step without explicit tracer
+    vertical mixing, followed by the BF16 MFMA column surrogate (FP64
+    column-integral correction, FP64 Thomas fallback for rejected columns).
+    The surrogate is trained here on columns of the same generator (outside
+    the timed region)."""
+    import torch
+
+    from zhenmode_amd.learned import ColumnMLP, MixingStep
+    from zhenmode_amd.surrogate import generate_columns, normalise, train
+
+    t0 = time.perf_counter()
+    X, Y = generate_columns(case, 600_000, spinup=24, every=24, log=lambda m: None)
+    mlp = ColumnMLP(case.grid.nz, device=X.device)
+    Xn, Yn = normalise(mlp, X, Y)
+    del X
+    losses = train(mlp, Xn, Yn, epochs=6, log=lambda m: None)
+    del Xn, Yn
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+
+    def timed(stepper):
+        s = stepper.init_state(case.T_init, case.S_init)
+        for _ in range(warmup):
+            s = stepper(s)
+        torch.cuda.synchronize()
+        if stepper.mixer is not None:
+            stepper.mixer.count.zero_()
+        t = time.perf_counter()
+        for _ in range(steps):
+            s = stepper(s)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t) / steps, s
+
+    kw = case.solver_kwargs
+    learned = MixingStep(case.grid, case.physics, case.dt, mode="learned", mlp=mlp, tol_T=1e-3, tol_S=2e-4,
+                         dtype="float32", **kw)
+    ms_l, st = timed(learned)
+    # the surrogate's forward pass over every column (4 BF16 MFMA GEMMs)
+    X = learned.mixer.features(st)
+    mlp.forward(X)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        mlp.forward(X)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / 10
+    sizes = mlp.sizes
+    flops = 2.0 * X.shape[0] * sum(k * n for k, n in zip(sizes[:-1], sizes[1:]))
+    wet_cols = int((case.grid.wet_mask > 0.5).sum())
+    fallback = learned.fallback_count() / max(1, steps * wet_cols)
+    thomas = MixingStep(case.grid, case.physics, case.dt, mode="thomas", dtype="float32", **kw)
+    ms_t, _ = timed(thomas)
+    return {"sypd": case.dt / (365.0 * ms_l * 1e-3), "ms_per_step": ms_l, "vs_plain_fp32": plain_ms / ms_l,
+            "fallback_fraction": fallback, "tol_T_degC": 1e-3, "tol_S_psu": 2e-4,
+            "thomas_mixing_ms_per_step": ms_t, "train_columns": 600_000, "train_epochs": 6,
+            "mlp_forward": {"rows": int(X.shape[0]), "ms": fwd_ms, "tflops": flops / (fwd_ms * 1e-3) / 1e12,
+                            "bound": "mfma", "peak_tflops_bf16_dense": 2500.0,
+                            "frac": flops / (fwd_ms * 1e-3) / 1e12 / 2500.0},
+            "final_loss": losses[-1], "train_s": train_s,
+            "note": "frozen forcing; the learned step replaces the explicit tracer vertical diffusion and "
+                    "convection of the plain step by the implicit mixing operator's surrogate"}
+
+
+def ensemble_leg(steps, warmup):
+    """24 uneven members (8 x 180x80x30, 8 x 360x160x50, 8 x 480x214x40) in one
+    hipGraph launch per ensemble step, tiles autotuned per member size."""
+    import torch
+
+    from zhenmode_amd.cases import ensemble_members
+    from zhenmode_amd.solver import Ensemble, make_solver_global
+
+    members = ensemble_members()
+    sols, states = [], []
+    for m in members:
+        step, init_state, _ = make_solver_global(m.grid, m.physics, m.dt, dtype="float32", **m.solver_kwargs)
+        sols.append(step.solver)
+        states.append(init_state(m.T_init, m.S_init))
+    tuned = {}
+    for sol, st in zip(sols, states):
+        key = sol.shape3
+        if key not in tuned:
+            tuned[key] = sol.autotune(st, reps=3)
+        else:
+            sol.set_tiles(*tuned[key][:2])
+    ens = Ensemble(sols)
+    a = states
+    for _ in range(warmup):
+        a = ens.step(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = ens.step(a)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    b = states
+    for _ in range(warmup):
+        b = [s.step(x) for s, x in zip(sols, b)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b = [s.step(x) for s, x in zip(sols, b)]
+    torch.cuda.synchronize()
+    ms_sep = 1e3 * (time.perf_counter() - t0) / steps
+    dt = members[0].dt
+    cells = sum(int(np.prod(s.shape3)) for s in sols)
+    finite = all(bool(torch.isfinite(x.T).all()) for x in a)
+    return {"members": len(sols), "cells": cells, "ms_per_ensemble_step": ms,
+            "member_sim_days_per_s": len(sols) * dt / 86400.0 / (ms * 1e-3),
+            "aggregate_sypd": len(sols) * dt / (365.0 * ms * 1e-3),
+            "ms_per_step_members_separately": ms_sep, "batching_speedup": ms_sep / ms,
+            "tiles": {f"{k[0]}x{k[1]}x{k[2]}": {"nu_block": v[0], "seg": v[1], "ms": v[2]} for k, v in tuned.items()},
+            "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +199,8 @@ def main():
     ap.add_argument("--case", default="g05deg")
     ap.add_argument("--dtype", default="float32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
@@ -190,6 +308,10 @@ def main():
                           "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    if not args.no_learned:
+        out["learned_mixed"] = learned_leg(case, ms_per_step, min(args.steps, 20), 3)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(min(args.steps, 10), 2)
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     if rank == 0:

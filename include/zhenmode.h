@@ -153,7 +153,8 @@ int32_t zm_nu_substeps_per_launch(const zm_handle* h);
 /* Thomas solve per column (one column per lane, float64 elimination).
  * T, S, T_out, S_out: 3-D device fields in the handle's dtype; outputs must not
  * alias inputs.  only_columns: optional int32 [ny*nx] device mask; when given,
- * only columns with a nonzero entry are solved (others untouched). */
+ * only columns with a nonzero entry are solved (others untouched).  The first
+ * call on a handle allocates its float64 elimination scratch (3 nz ny nx). */
 int zm_vertical_mix(zm_handle* h, const void* T, const void* S, void* T_out, void* S_out, double dt,
                     double kappa_v, double kappa_conv, const int32_t* only_columns, void* stream);
 
@@ -173,7 +174,7 @@ int zm_column_features(zm_handle* h, const zm_state* in, const void* tau_x, cons
  * residual max|X_new - dt D(X_new) - X| exceeds tol_T / tol_S get flags[c] = 1
  * and are counted into *n_flagged (device int32, accumulated); the caller
  * passes the same flags to zm_vertical_mix to replace them by the Thomas
- * solution. */
+ * solution.  nz <= 50. */
 int zm_mlp_apply(zm_handle* h, const zm_state* in, const float* Y, int32_t ldy, const float* out_mean,
                  const float* out_std, double dt, double kappa_v, double kappa_conv, double tol_T,
                  double tol_S, void* T_out, void* S_out, int32_t* flags, int32_t* n_flagged, void* stream);

@@ -19,9 +19,10 @@ struct MixParams {
   const double* dzi;  // dz_iface [nz-1], float64
 };
 
+// scratch: 3 nz n2 float64 values
 template <typename real>
 void launch_vmix(const Geo<real>& g, const MixParams& m, const real* T, const real* S, real* To, real* So,
-                 const int* only, hipStream_t s);
+                 const int* only, double* scratch, hipStream_t s);
 
 // Feature rows X[c][0..255] (bf16, row c = j*nx + i) of every column:
 // T, S, u, v, N^2 at nz levels, q_heat, SSS-restoring salt flux, |tau|, zero pad;

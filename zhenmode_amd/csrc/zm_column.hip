@@ -30,25 +30,25 @@ __device__ __forceinline__ double mix_kappa(double Tc, double Sc, double Tn, dou
 
 // ---------------------------------------------------------------------------
 // Thomas solve of (I - dt D_kappa) X_new = X for X in (T, S), one column per
-// lane, float64 elimination.  The forward sweep keeps c' and d' in LDS
-// ([k][lane], conflict-free), the back substitution reads them in reverse.
-// Dry levels are held.  `only` (optional) restricts the solve to flagged
-// columns.
+// lane, float64 elimination.  The forward sweep writes c' and d'_T, d'_S to a
+// float64 scratch of 3 nz n2 values laid out [k][column] (coalesced across
+// lanes), the back substitution reads them in reverse; one reciprocal of the
+// pivot per level.  Dry levels are held.  `only` (optional) restricts the
+// solve to flagged columns.
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ __launch_bounds__(64) void k_vmix(Geo<real> g, MixParams m, const real* __restrict__ T,
-                                             const real* __restrict__ S, real* __restrict__ To,
-                                             real* __restrict__ So, const int* __restrict__ only) {
-  extern __shared__ double zm_vmix_lds[];
-  const int lane = threadIdx.x;
-  const long long c2 = xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
+__global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const real* __restrict__ T,
+                                              const real* __restrict__ S, real* __restrict__ To,
+                                              real* __restrict__ So, const int* __restrict__ only,
+                                              double* __restrict__ scratch) {
+  const long long c2 = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (c2 >= g.n2) return;
   if (only && only[c2] == 0) return;
   const int nz = g.nz, nw = g.nwet[c2];
-  const long long n2 = g.n2;
-  double* cpl = zm_vmix_lds + lane;
-  double* dTl = cpl + (size_t)nz * 64;
-  double* dSl = dTl + (size_t)nz * 64;
+  const long long n2 = g.n2, n3 = n2 * nz;
+  double* cpl = scratch + c2;
+  double* dTl = cpl + n3;
+  double* dSl = dTl + n3;
   for (int k = nw; k < nz; ++k) {
     To[k * n2 + c2] = T[k * n2 + c2];
     So[k * n2 + c2] = S[k * n2 + c2];
@@ -66,13 +66,13 @@ __global__ __launch_bounds__(64) void k_vmix(Geo<real> g, MixParams m, const rea
     const double a = k > 0 ? -m.dt * km / (m.dzn[k] * m.dzi[k - 1]) : 0.0;
     const double c = k + 1 < nw ? -m.dt * kp / (m.dzn[k] * m.dzi[k]) : 0.0;
     const double b = 1.0 - a - c;
-    const double piv = b - a * cprev;
-    cprev = c / piv;
-    dT = (Tc - a * dT) / piv;
-    dS = (Sc - a * dS) / piv;
-    cpl[k * 64] = cprev;
-    dTl[k * 64] = dT;
-    dSl[k * 64] = dS;
+    const double inv = 1.0 / (b - a * cprev);
+    cprev = c * inv;
+    dT = (Tc - a * dT) * inv;
+    dS = (Sc - a * dS) * inv;
+    cpl[k * n2] = cprev;
+    dTl[k * n2] = dT;
+    dSl[k * n2] = dS;
     Tc = Tn;
     Sc = Sn;
     km = kp;
@@ -81,8 +81,9 @@ __global__ __launch_bounds__(64) void k_vmix(Geo<real> g, MixParams m, const rea
   To[(nw - 1) * n2 + c2] = (real)xT;
   So[(nw - 1) * n2 + c2] = (real)xS;
   for (int k = nw - 2; k >= 0; --k) {
-    xT = dTl[k * 64] - cpl[k * 64] * xT;
-    xS = dSl[k * 64] - cpl[k * 64] * xS;
+    const double cp = cpl[k * n2];
+    xT = dTl[k * n2] - cp * xT;
+    xS = dSl[k * n2] - cp * xS;
     To[k * n2 + c2] = (real)xT;
     So[k * n2 + c2] = (real)xS;
   }
@@ -90,72 +91,67 @@ __global__ __launch_bounds__(64) void k_vmix(Geo<real> g, MixParams m, const rea
 
 // ---------------------------------------------------------------------------
 // Feature rows of the surrogate: a workgroup builds the rows of 64 columns in
-// LDS (one column per lane) and writes them as one contiguous 32 KB block.
+// LDS (4 waves: T; S; u, v; N^2, surface fluxes and padding) and writes them
+// as one contiguous block of 64 x 512 bytes.
 // ---------------------------------------------------------------------------
 constexpr int kFeat = 256;
-constexpr int kFeatLdsWords = kFeat / 2 + 1;  // odd word stride: conflict-free row writes
+constexpr int kFeatRow = kFeat + 2;  // bf16 per LDS row: 129 words, conflict-free row writes
 
 template <typename real>
-__global__ __launch_bounds__(64) void k_col_features(Geo<real> g, MixParams m, const real* __restrict__ T,
-                                                     const real* __restrict__ S, const real* __restrict__ u,
-                                                     const real* __restrict__ v, const real* __restrict__ q_heat,
-                                                     const real* __restrict__ s_ref2, real restore_coef_S,
-                                                     const real* __restrict__ tau_x, const real* __restrict__ tau_y,
-                                                     const float* __restrict__ mean,
-                                                     const float* __restrict__ inv_std, unsigned* __restrict__ X,
-                                                     long long rows) {
-  __shared__ unsigned rowbuf[64 * kFeatLdsWords];
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_col_features(Geo<real> g, MixParams m, const real* __restrict__ T,
+                                                      const real* __restrict__ S, const real* __restrict__ u,
+                                                      const real* __restrict__ v, const real* __restrict__ q_heat,
+                                                      const real* __restrict__ s_ref2, real restore_coef_S,
+                                                      const real* __restrict__ tau_x,
+                                                      const real* __restrict__ tau_y, const float* __restrict__ mean,
+                                                      const float* __restrict__ inv_std, unsigned* __restrict__ X,
+                                                      long long rows) {
+  __shared__ unsigned short rowbuf[64 * kFeatRow];
+  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
   const long long r0 = (long long)blockIdx.x * 64;
   const long long c2 = r0 + lane;
   const int nz = g.nz;
   const long long n2 = g.n2;
-  auto put = [&](int f, float x0, float x1) {
-    const float y0 = (x0 - mean[f]) * inv_std[f];
-    const float y1 = (x1 - mean[f + 1]) * inv_std[f + 1];
-    const __bf16 b0 = (__bf16)y0, b1 = (__bf16)y1;
-    const unsigned w = (unsigned)__builtin_bit_cast(unsigned short, b0) |
-                       ((unsigned)__builtin_bit_cast(unsigned short, b1) << 16);
-    rowbuf[lane * kFeatLdsWords + f / 2] = w;
-  };
-  float vals[2];
-  int nv = 0, fbase = 0;
-  auto push = [&](float x) {
-    vals[nv++] = x;
-    if (nv == 2) {
-      put(fbase, vals[0], vals[1]);
-      fbase += 2;
-      nv = 0;
-    }
+  unsigned short* row = rowbuf + lane * kFeatRow;
+  auto put = [&](int f, float x) {
+    const __bf16 b = (__bf16)((x - mean[f]) * inv_std[f]);
+    row[f] = __builtin_bit_cast(unsigned short, b);
   };
   if (c2 < g.n2) {
     const int nw = g.nwet[c2];
-    for (int k = 0; k < nz; ++k) push(k < nw ? (float)T[k * n2 + c2] : 0.f);
-    for (int k = 0; k < nz; ++k) push(k < nw ? (float)S[k * n2 + c2] : 0.f);
-    for (int k = 0; k < nz; ++k) push(k < nw ? (float)u[k * n2 + c2] : 0.f);
-    for (int k = 0; k < nz; ++k) push(k < nw ? (float)v[k * n2 + c2] : 0.f);
-    for (int k = 0; k < nz; ++k) {
-      float n2k = 0.f;
-      if (k + 1 < nw) {
-        const double rc = rho_anom((double)T[k * n2 + c2], (double)S[k * n2 + c2], m);
-        const double rn = rho_anom((double)T[(k + 1) * n2 + c2], (double)S[(k + 1) * n2 + c2], m);
-        n2k = (float)(kG / kRho0 * (rn - rc) / m.dzi[k]);
+    if (part == 0) {
+      for (int k = 0; k < nz; ++k) put(k, k < nw ? (float)T[k * n2 + c2] : 0.f);
+    } else if (part == 1) {
+      for (int k = 0; k < nz; ++k) put(nz + k, k < nw ? (float)S[k * n2 + c2] : 0.f);
+    } else if (part == 2) {
+      for (int k = 0; k < nz; ++k) put(2 * nz + k, k < nw ? (float)u[k * n2 + c2] : 0.f);
+      for (int k = 0; k < nz; ++k) put(3 * nz + k, k < nw ? (float)v[k * n2 + c2] : 0.f);
+    } else {
+      double rc = nw > 0 ? rho_anom((double)T[c2], (double)S[c2], m) : 0.0;
+      for (int k = 0; k < nz; ++k) {
+        float n2k = 0.f;
+        if (k + 1 < nw) {
+          const double rn = rho_anom((double)T[(k + 1) * n2 + c2], (double)S[(k + 1) * n2 + c2], m);
+          n2k = (float)(kG / kRho0 * (rn - rc) / m.dzi[k]);
+          rc = rn;
+        }
+        put(4 * nz + k, n2k);
       }
-      push(n2k);
+      const bool wet = nw > 0;
+      put(5 * nz, wet ? (float)q_heat[c2] : 0.f);
+      put(5 * nz + 1, wet ? (float)(restore_coef_S * (s_ref2[c2] - S[c2])) : 0.f);
+      put(5 * nz + 2, wet ? sqrtf((float)tau_x[c2] * (float)tau_x[c2] + (float)tau_y[c2] * (float)tau_y[c2]) : 0.f);
+      for (int f = 5 * nz + 3; f < kFeat; ++f) put(f, 0.f);
     }
-    const bool wet = nw > 0;
-    push(wet ? (float)q_heat[c2] : 0.f);
-    push(wet ? (float)(restore_coef_S * (s_ref2[c2] - S[c2])) : 0.f);
-    push(wet ? sqrtf((float)tau_x[c2] * (float)tau_x[c2] + (float)tau_y[c2] * (float)tau_y[c2]) : 0.f);
-    while (fbase + nv < kFeat) push(0.f);
-  } else {
-    for (int w = 0; w < kFeat / 2; ++w) rowbuf[lane * kFeatLdsWords + w] = 0u;
+  } else if (part == 0) {
+    for (int f = 0; f < kFeat; ++f) row[f] = 0;
   }
   __syncthreads();
   const long long nrows = rows - r0 < 64 ? rows - r0 : 64;
-  for (int q = lane; q < nrows * (kFeat / 2); q += 64) {
+  const unsigned* words = reinterpret_cast<const unsigned*>(rowbuf);
+  for (int q = threadIdx.x; q < nrows * (kFeat / 2); q += 256) {
     const int rr = q / (kFeat / 2), w = q - rr * (kFeat / 2);
-    X[(r0 + rr) * (kFeat / 2) + w] = rowbuf[rr * kFeatLdsWords + w];
+    X[(r0 + rr) * (kFeat / 2) + w] = words[rr * (kFeatRow / 2) + w];
   }
 }
 
@@ -167,27 +163,50 @@ __global__ __launch_bounds__(64) void k_col_features(Geo<real> g, MixParams m, c
 // system, r = X_new - dt D(X_new) - X, bounds |X_new - X_thomas| (the matrix is
 // an M-matrix with unit row sums, so ||A^-1||_inf <= 1): a column with
 // max|r_T| > tol_T or max|r_S| > tol_S is flagged for the Thomas fallback.
+// A workgroup stages the Y rows and T, S columns of 64 columns in LDS with
+// coalesced loads (4 waves); one wave then walks the columns.
 // ---------------------------------------------------------------------------
+constexpr int kApplyMaxNz = 50;
+
 template <typename real>
-__global__ __launch_bounds__(64) void k_mlp_apply(Geo<real> g, MixParams m, const real* __restrict__ T,
-                                                  const real* __restrict__ S, const float* __restrict__ Y, int ldy,
-                                                  const float* __restrict__ omean, const float* __restrict__ ostd,
-                                                  double tol_T, double tol_S, real* __restrict__ To,
-                                                  real* __restrict__ So, int* __restrict__ flag,
-                                                  int* __restrict__ count) {
-  const long long c2 = xcd_tile(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  if (c2 >= g.n2) return;
-  const int nz = g.nz, nw = g.nwet[c2];
+__global__ __launch_bounds__(256) void k_mlp_apply(Geo<real> g, MixParams m, const real* __restrict__ T,
+                                                   const real* __restrict__ S, const float* __restrict__ Y, int ldy,
+                                                   const float* __restrict__ omean, const float* __restrict__ ostd,
+                                                   double tol_T, double tol_S, real* __restrict__ To,
+                                                   real* __restrict__ So, int* __restrict__ flag,
+                                                   int* __restrict__ count) {
+  constexpr int YW = 2 * kApplyMaxNz + 1;
+  __shared__ float ys[64 * YW];
+  __shared__ real ts[kApplyMaxNz * 64], ss[kApplyMaxNz * 64];
+  const int nz = g.nz;
   const long long n2 = g.n2;
+  const long long r0 = (long long)blockIdx.x * 64;
+  const int ncol = (int)(n2 - r0 < 64 ? n2 - r0 : 64);
+  for (int q = threadIdx.x; q < ncol * 2 * nz; q += 256) {
+    const int rr = q / (2 * nz), k = q - rr * 2 * nz;
+    ys[rr * YW + k] = Y[(r0 + rr) * ldy + k];
+  }
+  for (int q = threadIdx.x; q < nz * 64; q += 256) {
+    const int k = q >> 6, l = q & 63;
+    if (l < ncol) {
+      ts[k * 64 + l] = T[k * n2 + r0 + l];
+      ss[k * 64 + l] = S[k * n2 + r0 + l];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64 || (int)threadIdx.x >= ncol) return;
+  const int l = threadIdx.x;
+  const long long c2 = r0 + l;
+  const int nw = g.nwet[c2];
   for (int k = nw; k < nz; ++k) {
-    To[k * n2 + c2] = T[k * n2 + c2];
-    So[k * n2 + c2] = S[k * n2 + c2];
+    To[k * n2 + c2] = ts[k * 64 + l];
+    So[k * n2 + c2] = ss[k * 64 + l];
   }
   if (nw == 0) {
     flag[c2] = 0;
     return;
   }
-  const float* y = Y + c2 * (long long)ldy;
+  const float* y = ys + l * YW;
   auto incT = [&](int k) { return m.dt * ((double)y[k] * (double)ostd[k] + (double)omean[k]); };
   auto incS = [&](int k) { return m.dt * ((double)y[nz + k] * (double)ostd[nz + k] + (double)omean[nz + k]); };
   double sT = 0.0, sS = 0.0, H = 0.0;
@@ -197,15 +216,14 @@ __global__ __launch_bounds__(64) void k_mlp_apply(Geo<real> g, MixParams m, cons
     H += m.dzn[k];
   }
   const double shT = sT / H, shS = sS / H;
-  // march with a window: x_{k-1}, x_k, x_{k+1} of the new state and the input
-  double T0 = T[c2], S0 = S[c2];
+  double T0 = ts[l], S0 = ss[l];
   double xT = T0 + (incT(0) - shT), xS = S0 + (incS(0) - shS);
   double xTm = 0.0, xSm = 0.0, km = 0.0, rT = 0.0, rS = 0.0;
   for (int k = 0; k < nw; ++k) {
     double T1 = 0.0, S1 = 0.0, xTp = 0.0, xSp = 0.0, kp = 0.0;
     if (k + 1 < nw) {
-      T1 = T[(k + 1) * n2 + c2];
-      S1 = S[(k + 1) * n2 + c2];
+      T1 = ts[(k + 1) * 64 + l];
+      S1 = ss[(k + 1) * 64 + l];
       xTp = T1 + (incT(k + 1) - shT);
       xSp = S1 + (incS(k + 1) - shS);
       kp = mix_kappa(T0, S0, T1, S1, m);
@@ -417,24 +435,11 @@ __global__ __launch_bounds__(256) void k_adam(int n_out, int n_in, float* __rest
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// 160 KB of LDS per workgroup on gfx950
-inline size_t lds_limit_vmix() { return 160 * 1024; }
-
 template <typename real>
 void launch_vmix(const Geo<real>& g, const MixParams& m, const real* T, const real* S, real* To, real* So,
-                 const int* only, hipStream_t s) {
-  const unsigned blocks = (unsigned)((g.n2 + 63) / 64);
-  const size_t lds = (size_t)3 * g.nz * 64 * sizeof(double);
-  if (lds > 65536) {
-    static bool raised[2] = {false, false};
-    bool& done = raised[sizeof(real) == 8];
-    if (!done) {
-      hipFuncSetAttribute(reinterpret_cast<const void*>(k_vmix<real>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds_limit_vmix());
-      done = true;
-    }
-  }
-  hipLaunchKernelGGL(k_vmix<real>, dim3(blocks), dim3(64), lds, s, g, m, T, S, To, So, only);
+                 const int* only, double* scratch, hipStream_t s) {
+  const unsigned blocks = (unsigned)((g.n2 + 255) / 256);
+  hipLaunchKernelGGL(k_vmix<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, To, So, only, scratch);
 }
 
 template <typename real>
@@ -443,7 +448,7 @@ void launch_col_features(const Geo<real>& g, const MixParams& m, const real* T, 
                          const real* tau_x, const real* tau_y, const float* mean, const float* inv_std, void* X,
                          long long rows, hipStream_t s) {
   const unsigned blocks = (unsigned)((rows + 63) / 64);
-  hipLaunchKernelGGL(k_col_features<real>, dim3(blocks), dim3(64), 0, s, g, m, T, S, u, v, q_heat, s_ref2,
+  hipLaunchKernelGGL(k_col_features<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, u, v, q_heat, s_ref2,
                      restore_coef_S, tau_x, tau_y, mean, inv_std, (unsigned*)X, rows);
 }
 
@@ -452,7 +457,7 @@ void launch_mlp_apply(const Geo<real>& g, const MixParams& m, const real* T, con
                       const float* out_mean, const float* out_std, double tol_T, double tol_S, real* To, real* So,
                       int* flag, int* count, hipStream_t s) {
   const unsigned blocks = (unsigned)((g.n2 + 63) / 64);
-  hipLaunchKernelGGL(k_mlp_apply<real>, dim3(blocks), dim3(64), 0, s, g, m, T, S, Y, ldy, out_mean, out_std, tol_T,
+  hipLaunchKernelGGL(k_mlp_apply<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, Y, ldy, out_mean, out_std, tol_T,
                      tol_S, To, So, flag, count);
 }
 
@@ -490,7 +495,7 @@ void launch_adam(int n_out, int n_in, float* w, const float* gw, float* m, float
 
 #define ZM_COL_INSTANTIATE(real)                                                                              \
   template void launch_vmix<real>(const Geo<real>&, const MixParams&, const real*, const real*, real*, real*, \
-                                  const int*, hipStream_t);                                                   \
+                                  const int*, double*, hipStream_t);                                          \
   template void launch_col_features<real>(const Geo<real>&, const MixParams&, const real*, const real*,       \
                                           const real*, const real*, const real*, const real*, real,           \
                                           const real*, const real*, const float*, const float*, void*,        \

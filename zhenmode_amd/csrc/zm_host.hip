@@ -79,6 +79,7 @@ struct Solver : SolverBase {
   std::vector<void*> allocs;
   real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fden = nullptr;
   double *dzn64 = nullptr, *dzi64 = nullptr;
+  double* vmix_scratch = nullptr;  // Thomas c', d' (3 n3), allocated on first use
   double T_ref64 = 0, S_ref64 = 0;
   // workspace
   real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
@@ -575,7 +576,8 @@ struct Solver : SolverBase {
 
   void vertical_mix(const void* T, const void* S, void* To, void* So, double dt, double kv, double kc,
                     const int* only, hipStream_t s) override {
-    zm::launch_vmix(g, mix(dt, kv, kc), (const real*)T, (const real*)S, (real*)To, (real*)So, only, s);
+    if (!vmix_scratch) vmix_scratch = dev_alloc<double>((size_t)3 * n3);
+    zm::launch_vmix(g, mix(dt, kv, kc), (const real*)T, (const real*)S, (real*)To, (real*)So, only, vmix_scratch, s);
     ZM_HIP(hipGetLastError());
   }
 
@@ -872,8 +874,6 @@ int zm_vertical_mix(zm_handle* h, const void* T, const void* S, void* T_out, voi
   if (!h || !T || !S || !T_out || !S_out || !(dt >= 0.0) || !(kappa_v >= 0.0) || !(kappa_conv >= 0.0))
     return fail(h, "invalid zm_vertical_mix arguments", ZM_EINVAL);
   if (T == T_out || S == S_out) return fail(h, "zm_vertical_mix output must not alias its input", ZM_EINVAL);
-  if ((size_t)3 * h->solver->nz_ * 64 * sizeof(double) > 160 * 1024)
-    return fail(h, "zm_vertical_mix supports at most 106 levels", ZM_EINVAL);
   return guarded(h, [&] {
     h->solver->vertical_mix(T, S, T_out, S_out, dt, kappa_v, kappa_conv, only_columns, (hipStream_t)stream);
   });
@@ -895,6 +895,7 @@ int zm_mlp_apply(zm_handle* h, const zm_state* in, const float* Y, int32_t ldy, 
   if (!h || !in || !Y || !out_mean || !out_std || !T_out || !S_out || !flags || !n_flagged)
     return fail(h, "invalid zm_mlp_apply arguments", ZM_EINVAL);
   if (ldy < 2 * h->solver->nz_) return fail(h, "ldy must hold 2 nz outputs per column", ZM_EINVAL);
+  if (h->solver->nz_ > 50) return fail(h, "the surrogate apply step takes at most 50 levels", ZM_EINVAL);
   if (T_out == in->T || S_out == in->S) return fail(h, "zm_mlp_apply output must not alias its input", ZM_EINVAL);
   return guarded(h, [&] {
     h->solver->mlp_apply(*in, Y, ldy, out_mean, out_std, dt, kappa_v, kappa_conv, tol_T, tol_S, T_out, S_out,
