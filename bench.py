@@ -223,6 +223,8 @@ def main():
     tdtype = solver.dtype
     a = init_state(case.T_init, case.S_init)
     b = solver.new_state()
+    # launch shapes for this device and grid (results do not depend on them)
+    tiles = solver.autotune(a, reps=5)
     # on-device seasonal wind: tau_x(t) = (0.1 + 0.05 sin(2 pi t / 365 d)) cos(3 lat)
     from zhenmode_amd.solver import empty2
     base = torch.as_tensor(np.cos(3.0 * np.radians(g.lat)), dtype=tdtype, device=dev)[:, None].expand(g.ny, g.nx).contiguous()
@@ -293,7 +295,8 @@ def main():
                                f"(0.5 deg), dt={case.dt:g}s, n_sub={params.n_subcyc}, n_nu={params.n_nu}, "
                                f"seasonal wind via step_dyn, {args.dtype}",
                    "sim_days_per_s": case.dt / 86400.0 / (ms_per_step * 1e-3),
-                   "event_ms_per_step": ev_ms, "finite": finite},
+                   "event_ms_per_step": ev_ms, "finite": finite,
+                   "tiles": {"nu_block": tiles[0], "seg": tiles[1]}},
         "roofline": {"bound": "hbm",
                      "kernel": (f"k_nu_strip ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
                                 else "k_nu_substep (one nu_h Laplacian substep of u and v)"),

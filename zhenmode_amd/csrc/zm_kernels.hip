@@ -1585,14 +1585,25 @@ int nu_block_max() {
   return NuTile<real>::HALO;
 }
 
+template <typename real, int H, int HALO>
+static void launch_nu_strip(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h,
+                            real dts, int nsub, hipStream_t s) {
+  constexpr int OX = 64 - 2 * HALO, OY = H - 2 * HALO;
+  const long long tiles = (long long)((g.nx + OX - 1) / OX) * ((g.ny + OY - 1) / OY) * g.nz;
+  hipLaunchKernelGGL((k_nu_strip<real, H, HALO>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, g, u, v, uo,
+                     vo, nu_h, dts, nsub);
+}
+
+// nsub substeps per launch on the narrowest halo that covers them: a 3-wide
+// halo computes 12 % more useful nodes per tile than a 4-wide one.
 template <typename real>
 void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
                      int nsub, hipStream_t s) {
   using T = NuTile<real>;
-  constexpr int OX = 64 - 2 * T::HALO, OY = T::H - 2 * T::HALO;
-  const long long tiles = (long long)((g.nx + OX - 1) / OX) * ((g.ny + OY - 1) / OY) * g.nz;
-  hipLaunchKernelGGL((k_nu_strip<real, T::H, T::HALO>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, g, u,
-                     v, uo, vo, nu_h, dts, nsub);
+  if (nsub <= 3)
+    launch_nu_strip<real, T::H, 3>(g, u, v, uo, vo, nu_h, dts, nsub, s);
+  else
+    launch_nu_strip<real, T::H, T::HALO>(g, u, v, uo, vo, nu_h, dts, nsub, s);
 }
 
 template <typename real>
