@@ -77,6 +77,11 @@ typedef struct zm_config {
   const double* coastal_bulk_lambda;
   const double* kappa_h_2d;     /* kappa_h + coastal kappa_h */
   const double* kappa_v_2d;     /* kappa_v + coastal kappa_v */
+  /* production tracer/mixing options of the reference factory (factory.py:49-51, 649-653) */
+  int32_t conservative_kv;      /* interface-flux vertical tracer diffusion (physics/vertical.py:45-53) */
+  int32_t localize_conv;        /* convective flux needs its own interface unstable (tendencies.py:149-165) */
+  int32_t fct_adv;              /* minmod-limited tracer face values (transport.py:199-245, 266-275) */
+  int32_t monotone_adv;         /* donor-cell tracer face values; fct_adv takes precedence */
 } zm_config;
 
 /* Prognostic state (reference JaxStateG(u, v, T, S, eta, ice),

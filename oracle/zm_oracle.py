@@ -104,8 +104,12 @@ def make_params(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
                 coastal_kappa_v_mask=None, coastal_kappa_v=0.0,
                 sponge_days=0.0, sponge_cells=0, T_init=None, S_init=None,
                 polar_cap_rows=2, polar_cap_taper=3, dt_bt=150.0, nu_nsub=None,
-                ice_freeze_temp_c=-1.8, ice_salt_flux=0.0):
-    """Frozen parameter set of the legacy mode-split step (float64)."""
+                ice_freeze_temp_c=-1.8, ice_salt_flux=0.0, conservative_kv=False, localize_conv=False,
+                fct_adv=False, monotone_adv=False):
+    """Frozen parameter set of the legacy mode-split step (float64).
+
+    The last four flags are the production tracer/mixing options of the
+    reference factory (factory.py:49-51, 649-653)."""
     nx, ny, nz = int(grid.nx), int(grid.ny), int(grid.nz)
     p = SimpleNamespace(nx=nx, ny=ny, nz=nz, dt=float(dt))
     dx_2d = np.asarray(grid.dx_2d, dtype=np.float64)
@@ -233,6 +237,10 @@ def make_params(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
     p.adv_nsub = max(1, int(np.ceil(dt * 4.0e-3 / (0.5 * min_dz))))
     p.ice_freeze_temp_c = float(ice_freeze_temp_c)
     p.ice_salt_flux = float(ice_salt_flux)
+    p.conservative_kv = bool(conservative_kv)
+    p.localize_conv = bool(localize_conv)
+    p.fct_adv = bool(fct_adv)
+    p.monotone_adv = bool(monotone_adv)
     return p
 
 
@@ -475,29 +483,49 @@ def density_anomaly(T, S, p):
     return RHO_0 * (-ALPHA_T * (T - p.T_ref) + BETA_S * (S - p.S_ref))
 
 
-def convective_mask(T, S, p):
-    """Column gate: any wet/wet interface with denser water above (physics/vertical.py:63-82)."""
+def unstable_interfaces(T, S, p):
+    """Per-interface gate: wet/wet with denser water above (physics/vertical.py:63-82)."""
     wm = p.wet_mask_z
     wet_iface = (wm[..., :-1] > 0.5) & (wm[..., 1:] > 0.5)
     rho = density_anomaly(T, S, p) * wm
-    unstable = (rho[..., :-1] > rho[..., 1:]) & wet_iface
-    return np.any(unstable, axis=-1, keepdims=True)
+    return (rho[..., :-1] > rho[..., 1:]) & wet_iface
 
 
-def conv_flux_tendency(tracer, gate, kappa, p):
-    """Interface-flux convective mixing (physics/vertical.py:13-37)."""
+def convective_mask(T, S, p):
+    """Column gate: any unstable interface (physics/vertical.py:63-82)."""
+    return np.any(unstable_interfaces(T, S, p), axis=-1, keepdims=True)
+
+
+def conv_flux_tendency(tracer, gate, kappa, p, iface_gate=None):
+    """Interface-flux convective mixing (physics/vertical.py:13-37); with
+    iface_gate (localize_conv) a flux also needs its own interface unstable."""
     if kappa <= 0.0:
         return np.zeros_like(tracer)
     flux = diffusive_interface_flux(tracer, kappa, p)
     wm = p.wet_mask_z
     allow = gate & (wm[..., :-1] > 0.5) & (wm[..., 1:] > 0.5)
+    if iface_gate is not None:
+        allow = allow & iface_gate
     flux = np.where(allow, flux, 0.0)
     return iface_flux_divergence(flux, p) * wm
 
 
+def d2_dz2_flux(tracer, kappa, p):
+    """Interface-flux vertical diffusion (numerics/vertical.py:317-332)."""
+    flux = diffusive_interface_flux(tracer, kappa, p)
+    wm = p.wet_mask_z
+    flux = np.where((wm[..., :-1] > 0.5) & (wm[..., 1:] > 0.5), flux, 0.0)
+    return iface_flux_divergence(flux, p) * wm
+
+
 def vertical_diffusion(tracer, p):
-    """Node-form kappa_v d2/dz2 (physics/vertical.py:45-53, conservative_kv off)."""
+    """kappa_v vertical diffusion of a tracer (physics/vertical.py:45-53): node
+    form by default, interface-flux form with conservative_kv."""
     kappa = p.kappa_v + p.coastal_kappa_v_2d[:, :, None]
+    if getattr(p, "conservative_kv", False):
+        if np.all(kappa == 0.0):
+            return np.zeros_like(tracer)
+        return d2_dz2_flux(tracer, kappa, p)
     return kappa * d2_dz2(tracer, p)
 
 
@@ -543,12 +571,39 @@ def advection_flux_form(u, v, w, p):
     return dealias(adv_u, p) * p.wet_mask_z, dealias(adv_v, p) * p.wet_mask_z
 
 
+def limited_slope(T, wm, axis):
+    """Minmod slope from wet neighbours only (transport.py:199-213); y uses
+    edge replication at the walls, x is periodic."""
+    if axis == 1:
+        prev, nxt = south_edge(T), north_edge(T)
+        pw, nw = south_edge(wm), north_edge(wm)
+    else:
+        prev, nxt = west(T), east(T)
+        pw, nw = west(wm), east(wm)
+    left, right = T - prev, nxt - T
+    signs = np.sign(left) + np.sign(right)
+    return 0.5 * signs * np.minimum(np.abs(left), np.abs(right)) * wm * pw * nw
+
+
 def horizontal_tracer_fluxes(T, u, v, p):
-    """Centered face-gated tracer fluxes (transport.py:224-290, default branch)."""
+    """Face-gated tracer fluxes (transport.py:224-290): centred face values by
+    default, the minmod-limited upwind reconstruction with fct_adv, donor cell
+    with monotone_adv (fct_adv takes precedence)."""
     wm = p.wet_mask_z
-    fx = 0.5 * (u + east(u)) * (0.5 * (T + east(T))) * (wm * east(wm))
+    ux = 0.5 * (u + east(u))
     vy = 0.5 * (v + north_edge(v))
-    ty = 0.5 * (T + north_edge(T))
+    if getattr(p, "fct_adv", False):
+        sx = limited_slope(T, wm, 0)
+        tx = np.where(ux >= 0.0, T + 0.5 * sx, east(T) - 0.5 * east(sx))
+        sy = limited_slope(T, wm, 1)
+        ty = np.where(vy >= 0.0, T + 0.5 * sy, north_edge(T) - 0.5 * north_edge(sy))
+    elif getattr(p, "monotone_adv", False):
+        tx = np.where(ux >= 0.0, T, east(T))
+        ty = np.where(vy >= 0.0, T, north_edge(T))
+    else:
+        tx = 0.5 * (T + east(T))
+        ty = 0.5 * (T + north_edge(T))
+    fx = ux * tx * (wm * east(wm))
     gate_y = wm * north_edge(wm)
     fy = vy * ty * gate_y * _row(cos_face_north(p), 3)
     fy[:, -1] = 0.0
@@ -630,7 +685,9 @@ def tracer_tendency(u, v, T, S, p):
     diff_h_T, diff_h_S = tracer_diffusion(T, p), tracer_diffusion(S, p)
     diff_v_T, diff_v_S = vertical_diffusion(T, p), vertical_diffusion(S, p)
 
-    gate = convective_mask(T, S, p)
+    unstable = unstable_interfaces(T, S, p)
+    gate = np.any(unstable, axis=-1, keepdims=True)
+    iface_gate = unstable if getattr(p, "localize_conv", False) else None
     n_c = int(p.conv_nsub)
     if n_c > 1:
         kappa_c = p.kappa_conv / n_c
@@ -638,14 +695,14 @@ def tracer_tendency(u, v, T, S, p):
         tT, tS = T, S
         accT = accS = 0.0
         for _ in range(n_c):
-            ttT = conv_flux_tendency(tT, gate, kappa_c, p)
-            ttS = conv_flux_tendency(tS, gate, kappa_c, p)
+            ttT = conv_flux_tendency(tT, gate, kappa_c, p, iface_gate)
+            ttS = conv_flux_tendency(tS, gate, kappa_c, p, iface_gate)
             accT, accS = accT + ttT, accS + ttS
             tT, tS = tT + ttT * h_c, tS + ttS * h_c
         conv_T, conv_S = accT / n_c, accS / n_c
     else:
-        conv_T = conv_flux_tendency(T, gate, p.kappa_conv, p)
-        conv_S = conv_flux_tendency(S, gate, p.kappa_conv, p)
+        conv_T = conv_flux_tendency(T, gate, p.kappa_conv, p, iface_gate)
+        conv_S = conv_flux_tendency(S, gate, p.kappa_conv, p, iface_gate)
 
     heat_factor = surface_heat_weights(p) / (RHO_0 * C_P)
     sst = T[:, :, 0:1]

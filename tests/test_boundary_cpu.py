@@ -72,7 +72,7 @@ def test_host_freeze_matches_oracle_parameters():
     (dict(mode_split=False), "mode_split"),
     (dict(mode_split=True, dtype="float16"), "dtype"),
     (dict(mode_split=True, column_geometry="nodal_dual_v1"), "column_geometry"),
-    (dict(mode_split=True, fct_adv=True), "fct_adv"),
+    (dict(mode_split=True, project_adv_vel=True), "project_adv_vel"),
     (dict(mode_split=True, sponge_days=10.0, sponge_cells=3), "sponge"),
     (dict(mode_split=True, polar_cap_rows=30, polar_cap_taper=3), "overlap"),
     (dict(mode_split=True, dt_bt=-1.0), "dt_bt"),
@@ -101,3 +101,13 @@ def test_case_generator_is_seeded_and_near_thirty_percent_land():
     assert a.grid.z[0] == 0.0 and a.grid.z[-1] == -4000.0 and a.grid.z[1] == pytest.approx(-10.0)
     g1deg = make_case("g1deg", nx=90, ny=40)
     assert g1deg.grid.z[-1] == -5500.0 and g1deg.grid.nz == 50
+
+
+def test_production_flags_are_frozen_into_the_config():
+    from zhenmode_amd.params import make_config
+    case = make_case("g4deg")
+    p = freeze(case.grid, case.physics, case.dt, conservative_kv=True, localize_conv=True, fct_adv=True,
+               monotone_adv=False, **{k: v for k, v in case.solver_kwargs.items() if k != "mode_split"},
+               mode_split=True)
+    cfg, keep = make_config(p)
+    assert (cfg.conservative_kv, cfg.localize_conv, cfg.fct_adv, cfg.monotone_adv) == (1, 1, 1, 0)

@@ -191,3 +191,34 @@ def test_init_state_masks_land_and_ghost_like_the_reference():
     ref = O.init_state(p, case.T_init, case.S_init)
     for f in FIELDS:
         np.testing.assert_array_equal(getattr(s, f), getattr(ref, f))
+
+
+FLAG_SETS = [dict(conservative_kv=True), dict(localize_conv=True), dict(fct_adv=True), dict(monotone_adv=True),
+             dict(conservative_kv=True, localize_conv=True, fct_adv=True)]
+
+
+@pytest.mark.parametrize("flags", FLAG_SETS, ids=lambda f: "+".join(sorted(f)))
+@pytest.mark.parametrize("dtype,tol", [("float64", TOL64), ("float32", dict(u=1e-4, v=1e-4, T=2e-6, S=2e-6, eta=1e-4))])
+def test_production_flags_match_oracle(flags, dtype, tol):
+    print(f"case land flags {sorted(flags)} {dtype}", flush=True)
+    grid, physics, dt, kwargs, T, S = land_case()
+    kw = dict(kwargs, **flags)
+    step, _, _, params, _ = make_solver_global(grid, physics, dt, return_params=True, dtype=dtype,
+                                               mode_split=True, **kw)
+    sol = step.solver
+    p = oracle_params(grid, physics, dt, kw)
+    s0 = perturbed_state(p, T, S)
+    assert np.any(O.unstable_interfaces(s0.T, s0.S, p))
+    stages = {}
+    ref = O.step(s0, p, stages=stages)
+    check(sol.stage(_lib.STAGE_LINEAR, JaxStateG(*s0)), stages["L1"], tol, "linear")
+    check(sol.stage(_lib.STAGE_NONLINEAR, JaxStateG(*stages["L1"])), stages["N"], tol, "nonlinear")
+    check(sol.step(JaxStateG(*s0)), ref, tol, "step")
+    dT, dS = O.tracer_residual(s0.u, s0.v, s0.T, s0.S, p)
+    gt = to_numpy_state(sol.stage(_lib.STAGE_TRACER_RES, JaxStateG(*s0)))
+    wet = p.wet_mask_z > 0.5
+    rtol = 1e-10 if dtype == "float64" else 2e-3
+    for got, r, name in ((gt.T, dT, "dT"), (gt.S, dS, "dS")):
+        e = rel(got[wet], r[wet])
+        print(name, f"{e:.1e}", flush=True)
+        assert e <= rtol, (name, e)

@@ -42,6 +42,8 @@ class ZmConfig(ctypes.Structure):
         ("tau_x", _D), ("tau_y", _D), ("q_heat", _D), ("t_atm", _D),
         ("s_ref_surf", _D), ("coastal_restore_coef", _D), ("coastal_restore_T", _D),
         ("coastal_bulk_lambda", _D), ("kappa_h_2d", _D), ("kappa_v_2d", _D),
+        ("conservative_kv", ctypes.c_int32), ("localize_conv", ctypes.c_int32),
+        ("fct_adv", ctypes.c_int32), ("monotone_adv", ctypes.c_int32),
     ]
 
 

@@ -64,6 +64,9 @@ struct Phys {
   real lambda_bulk, restore_coef_S, ice_freeze, ice_salt_flux;
   real heat_factor;            // 1 / (dz_surface rho0 cp)
   int conv_nsub;
+  int cons_kv;                 // conservative_kv: interface-flux vertical tracer diffusion
+  int local_conv;              // localize_conv: convective flux needs its own interface unstable
+  int adv_scheme;              // horizontal tracer face values: 0 centred, 1 donor cell, 2 minmod (fct_adv)
   int ncap, nb;                // polar cap rows and band width (ncap + taper)
   const real* capw;            // pole-inward blend weights [nb]
   const real* wet2;            // wet_mask [n2]

@@ -232,6 +232,9 @@ struct Solver : SolverBase {
     P.ice_salt_flux = (real)c.ice_salt_flux;
     P.heat_factor = (real)((1.0 / c.dz_surface) / (1025.0 * 3992.0));
     P.conv_nsub = c.conv_nsub;
+    P.cons_kv = c.conservative_kv != 0;
+    P.local_conv = c.localize_conv != 0;
+    P.adv_scheme = c.fct_adv ? 2 : (c.monotone_adv ? 1 : 0);
     P.ncap = ncap;
     P.nb = nb;
     std::vector<double> capw(nb > 0 ? nb : 1, 0.0);
@@ -677,6 +680,9 @@ void validate(const zm_config& c) {
   if (!(c.dt > 0) || !(c.dt_bt > 0)) bad("dt and dt_bt must be positive");
   if (c.n_subcyc < 1 || c.n_nu < 1 || c.conv_nsub < 1) bad("substep counts must be >= 1");
   if (c.adv_nsub != 1) bad("adv_nsub > 1 (vertical advection sub-cycling) is not supported");
+  if ((long long)4 * c.nz * 64 * c.dtype_bytes > 160 * 1024)
+    bad("too many levels for the convective column kernel (nz <= 156 in float32, 78 in float64)");
+  if (c.localize_conv && c.nz > 129) bad("localize_conv supports at most 129 levels");
   if (c.polar_cap_rows < 0 || c.polar_cap_taper < 0) bad("polar cap rows must be >= 0");
   if (c.polar_cap_rows > 0 && 2 * (c.polar_cap_rows + c.polar_cap_taper) > c.ny)
     bad("polar cap bands must not overlap");

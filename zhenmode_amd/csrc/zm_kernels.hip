@@ -444,6 +444,28 @@ __device__ __forceinline__ real ddz_win(real xm, real xc, real xp, int k, int kb
   return ((k < kb ? xp : xc) - xm) * g.iddz[k - 1];
 }
 
+// kappa_v vertical diffusion of a tracer at level k (physics/vertical.py:45-53):
+// node form kappa d2/dz2 by default; with conservative_kv the interface-flux
+// form of _d2_dz2_flux (numerics/vertical.py:317-332), F[k+1/2] =
+// -kappa (X[k+1] - X[k]) / dz_iface[k] on wet interfaces, (F[k-1/2] - F[k+1/2]) / dz_node.
+template <typename real>
+__device__ __forceinline__ real vdiff_win(real xm, real xc, real xp, int k, int nw, int kb, real kv, bool cons,
+                                          const Geo<real>& g) {
+  if (!cons) return kv * d2z_win(xm, xc, xp, k, kb, g);
+  if (k >= nw) return real(0);
+  const real fup = k > 0 ? -kv * (xc - xm) * g.idzi[k - 1] : real(0);
+  const real fdn = k + 1 < nw ? -kv * (xp - xc) * g.idzi[k] : real(0);
+  return (fup - fdn) * g.idzn[k];
+}
+
+// minmod slope from wet neighbours (transport.py:199-213)
+template <typename real>
+__device__ __forceinline__ real minmod_slope(real xm, real x, real xp, bool wm, bool w, bool wp) {
+  const real l = x - xm, r = xp - x;
+  const real sg = real((l > real(0)) - (l < real(0)) + (r > real(0)) - (r < real(0)));
+  return (wm && w && wp) ? real(0.5) * sg * fmin(fabs(l), fabs(r)) : real(0);
+}
+
 template <typename real>
 static inline unsigned segs3(const Geo<real>& g) {
   return (unsigned)((long long)((g.nx + kTileI - 1) / kTileI) * ((g.ny + kTileJ - 1) / kTileJ) *
@@ -503,8 +525,8 @@ __global__ __launch_bounds__(256) void k_linear_finish(
       S = S - P.kappa_bi * d1_vals(DS0[o], DS0[pl + n.e2], DS0[pl + n.w2], DS0[pl + n.n2e], DS0[pl + n.s2e], ge, gw,
                                    gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
     }
-    T = T + kv * d2z_win(Tm, Tc, Tp, k, kb, g) * dth;
-    S = S + kv * d2z_win(Sm, Sc, Sp, k, kb, g) * dth;
+    T = T + vdiff_win(Tm, Tc, Tp, k, n.nwC, kb, kv, P.cons_kv != 0, g) * dth;
+    S = S + vdiff_win(Sm, Sc, Sp, k, n.nwC, kb, kv, P.cons_kv != 0, g) * dth;
     if (!wc) {
       u = uc;
       v = vc;
@@ -568,66 +590,71 @@ template <typename real>
 __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const real* __restrict__ T,
                                              const real* __restrict__ S, real* __restrict__ cT,
                                              real* __restrict__ cS, int* __restrict__ gate) {
+  // LDS columns [k][lane]: the state being mixed and the sum of the substep terms
   extern __shared__ unsigned char zm_lds[];
   real* tT = reinterpret_cast<real*>(zm_lds);
   real* tS = tT + (size_t)g.nz * 64;
+  real* aT = tS + (size_t)g.nz * 64;
+  real* aS = aT + (size_t)g.nz * 64;
   const int lane = threadIdx.x;
   const int c2 = blockIdx.x * 64 + lane;
   if (c2 >= g.n2) return;
   const int nw = g.nwet[c2];
   const int n2 = g.n2;
+  // stage the wet part of the column (independent loads, issued together)
+  for (int k = 0; k < nw; ++k) {
+    tT[k * 64 + lane] = T[(long long)k * n2 + c2];
+    tS[k * 64 + lane] = S[(long long)k * n2 + c2];
+  }
   bool unstable = false;
+  unsigned long long ub0 = 0, ub1 = 0;  // unstable interfaces k+1/2 (k < 128), frozen over the sub-cycle
   if (nw >= 2) {
-    real rprev = rho_anom(T[c2], S[c2], P);
+    real rprev = rho_anom(tT[lane], tS[lane], P);
     for (int k = 1; k < nw; ++k) {
-      const real r = rho_anom(T[(long long)k * n2 + c2], S[(long long)k * n2 + c2], P);
-      unstable |= (rprev > r);
+      const real r = rho_anom(tT[k * 64 + lane], tS[k * 64 + lane], P);
+      const bool un = rprev > r;
+      unstable |= un;
+      if (un) {
+        if (k - 1 < 64) ub0 |= 1ull << (k - 1);
+        else ub1 |= 1ull << (k - 65);
+      }
       rprev = r;
     }
   }
-  gate[c2] = unstable ? 1 : 0;
   if (!unstable || P.kappa_conv <= real(0)) {
     gate[c2] = 0;
     return;
   }
-  for (int k = 0; k < g.nz; ++k) {
-    tT[k * 64 + lane] = T[(long long)k * n2 + c2];
-    tS[k * 64 + lane] = S[(long long)k * n2 + c2];
-  }
+  gate[c2] = 1;
   const int nc = P.conv_nsub;
   const real kap = nc > 1 ? P.kappa_conv / real(nc) : P.kappa_conv;
   const real h = P.dt / real(nc);
   for (int s = 0; s < nc; ++s) {
     real fTprev = 0, fSprev = 0;
     for (int k = 0; k < g.nz; ++k) {
-      real fT = 0, fS = 0;
-      const real ot = tT[k * 64 + lane], os = tS[k * 64 + lane];
-      if (k + 1 < nw) {
-        fT = (-kap * (tT[(k + 1) * 64 + lane] - ot)) * g.idzi[k];
-        fS = (-kap * (tS[(k + 1) * 64 + lane] - os)) * g.idzi[k];
+      real fT = 0, fS = 0, et = 0, es = 0;
+      if (k < nw) {
+        const real ot = tT[k * 64 + lane], os = tS[k * 64 + lane];
+        const bool open = !P.local_conv || ((k < 64 ? (ub0 >> k) : (ub1 >> (k - 64))) & 1ull);
+        if (k + 1 < nw && open) {
+          fT = (-kap * (tT[(k + 1) * 64 + lane] - ot)) * g.idzi[k];
+          fS = (-kap * (tS[(k + 1) * 64 + lane] - os)) * g.idzi[k];
+        }
+        et = (fTprev - fT) * g.idzn[k];
+        es = (fSprev - fS) * g.idzn[k];
+        tT[k * 64 + lane] = ot + et * h;
+        tS[k * 64 + lane] = os + es * h;
       }
-      const real et = k < nw ? (fTprev - fT) * g.idzn[k] : real(0);
-      const real es = k < nw ? (fSprev - fS) * g.idzn[k] : real(0);
-      tT[k * 64 + lane] = ot + et * h;
-      tS[k * 64 + lane] = os + es * h;
-      const long long o = (long long)k * n2 + c2;
-      if (s == 0) {
-        cT[o] = et;
-        cS[o] = es;
-      } else {
-        cT[o] = cT[o] + et;
-        cS[o] = cS[o] + es;
-      }
+      aT[k * 64 + lane] = s == 0 ? et : aT[k * 64 + lane] + et;
+      aS[k * 64 + lane] = s == 0 ? es : aS[k * 64 + lane] + es;
       fTprev = fT;
       fSprev = fS;
     }
   }
-  if (nc > 1) {
-    for (int k = 0; k < g.nz; ++k) {
-      const long long o = (long long)k * n2 + c2;
-      cT[o] = cT[o] / real(nc);
-      cS[o] = cS[o] / real(nc);
-    }
+  for (int k = 0; k < g.nz; ++k) {
+    const long long o = (long long)k * n2 + c2;
+    cT[o] = nc > 1 ? aT[k * 64 + lane] / real(nc) : aT[k * 64 + lane];
+    cS[o] = nc > 1 ? aS[k * 64 + lane] / real(nc) : aS[k * 64 + lane];
   }
 }
 
@@ -670,7 +697,7 @@ __device__ __forceinline__ real adv_scalar(const real* __restrict__ X, const rea
   return -(hdiv + divz);
 }
 
-template <typename real>
+template <typename real, int ADV>
 __global__ __launch_bounds__(256) void k_tracer_residual(
     Geo<real> g, Phys<real> P, int mode, const real* __restrict__ u, const real* __restrict__ v,
     const real* __restrict__ T, const real* __restrict__ S, const real* __restrict__ Fz,
@@ -686,6 +713,18 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
   const real kv = P.kv2[c2];
   const bool gated = gate[c2] != 0;
   const real h = real(0.5);
+  // second neighbours for the minmod reconstruction (x periodic, y edge-clamped)
+  int ee2 = 0, ww2 = 0, nn2 = 0, ss2 = 0, nwEE = 0, nwWW = 0, nwNN = 0, nwSS = 0;
+  if (ADV == 2) {
+    ee2 = j * g.nx + wrap_e(wrap_e(i, g.nx), g.nx);
+    ww2 = j * g.nx + wrap_w(wrap_w(i, g.nx), g.nx);
+    nn2 = (j + 2 < g.ny ? j + 2 : g.ny - 1) * g.nx + i;
+    ss2 = (j >= 2 ? j - 2 : 0) * g.nx + i;
+    nwEE = g.nwet[ee2];
+    nwWW = g.nwet[ww2];
+    nwNN = g.nwet[nn2];
+    nwSS = g.nwet[ss2];
+  }
   long long o = (long long)k0 * n2 + c2;
   real Tm = k0 > 0 ? T[o - n2] : real(0), Tc = T[o];
   real Sm = k0 > 0 ? S[o - n2] : real(0), Sc = S[o];
@@ -702,20 +741,51 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
     const real SE = S[pl + n.e2], SW = S[pl + n.w2], SN = S[pl + n.n2e], SS = S[pl + n.s2e];
     const real dhT = d1_vals(Tc, TE, TW, TN, TS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
     const real dhS = d1_vals(Sc, SE, SW, SN, SS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
-    const real dvT = kv * d2z_win(Tm, Tc, Tp, k, kb, g);
-    const real dvS = kv * d2z_win(Sm, Sc, Sp, k, kb, g);
+    const real dvT = vdiff_win(Tm, Tc, Tp, k, n.nwC, kb, kv, P.cons_kv != 0, g);
+    const real dvS = vdiff_win(Sm, Sc, Sp, k, n.nwC, kb, kv, P.cons_kv != 0, g);
     real dTdt = 0, dSdt = 0;
     if (wc) {
       // centred face-gated horizontal fluxes (transport.py:224-290)
       const real uc = u[o], vc = v[o];
-      const real ufe = ge ? h * (uc + u[pl + n.e2]) : real(0);
-      const real ufw = gw ? h * (u[pl + n.w2] + uc) : real(0);
-      const real vfn = gn ? h * (vc + v[pl + n.n2e]) : real(0);
-      const real vfs = gs ? h * (v[pl + n.s2e] + vc) : real(0);
-      const real hT = (ufe * (h * (Tc + TE)) - ufw * (h * (TW + Tc))) * idx +
-                      (vfn * (h * (Tc + TN)) * cfN - vfs * (h * (TS + Tc)) * cfS) * idyc;
-      const real hS = (ufe * (h * (Sc + SE)) - ufw * (h * (SW + Sc))) * idx +
-                      (vfn * (h * (Sc + SN)) * cfN - vfs * (h * (SS + Sc)) * cfS) * idyc;
+      const real uxe = h * (uc + u[pl + n.e2]), uxw = h * (u[pl + n.w2] + uc);
+      const real vyn = h * (vc + v[pl + n.n2e]), vys = h * (v[pl + n.s2e] + vc);
+      const real ufe = ge ? uxe : real(0);
+      const real ufw = gw ? uxw : real(0);
+      const real vfn = gn ? vyn : real(0);
+      const real vfs = gs ? vys : real(0);
+      real tE, tW, tN, tS, sE_, sW_, sN_, sS_;
+      if (ADV == 0) {  // centred face values
+        tE = h * (Tc + TE), tW = h * (TW + Tc), tN = h * (Tc + TN), tS = h * (TS + Tc);
+        sE_ = h * (Sc + SE), sW_ = h * (SW + Sc), sN_ = h * (Sc + SN), sS_ = h * (SS + Sc);
+      } else if (ADV == 1) {  // donor cell (monotone_adv)
+        tE = uxe >= real(0) ? Tc : TE, tW = uxw >= real(0) ? TW : Tc;
+        tN = vyn >= real(0) ? Tc : TN, tS = vys >= real(0) ? TS : Tc;
+        sE_ = uxe >= real(0) ? Sc : SE, sW_ = uxw >= real(0) ? SW : Sc;
+        sN_ = vyn >= real(0) ? Sc : SN, sS_ = vys >= real(0) ? SS : Sc;
+      } else {  // minmod-limited upwind reconstruction (fct_adv)
+        const bool wC = true, wE = k < n.nwE, wW = k < n.nwW, wN = k < n.nwN, wS = k < n.nwS;
+        const bool wEE = k < nwEE, wWW = k < nwWW, wNN = k < nwNN, wSS = k < nwSS;
+        const real TEE = T[pl + ee2], TWW = T[pl + ww2], TNN = T[pl + nn2], TSS = T[pl + ss2];
+        const real SEE = S[pl + ee2], SWW = S[pl + ww2], SNN = S[pl + nn2], SSS_ = S[pl + ss2];
+        const real tsx = minmod_slope(TW, Tc, TE, wW, wC, wE), tsE = minmod_slope(Tc, TE, TEE, wC, wE, wEE);
+        const real tsW = minmod_slope(TWW, TW, Tc, wWW, wW, wC);
+        const real tsy = minmod_slope(TS, Tc, TN, wS, wC, wN), tsN = minmod_slope(Tc, TN, TNN, wC, wN, wNN);
+        const real tsS = minmod_slope(TSS, TS, Tc, wSS, wS, wC);
+        tE = uxe >= real(0) ? Tc + h * tsx : TE - h * tsE;
+        tW = uxw >= real(0) ? TW + h * tsW : Tc - h * tsx;
+        tN = vyn >= real(0) ? Tc + h * tsy : TN - h * tsN;
+        tS = vys >= real(0) ? TS + h * tsS : Tc - h * tsy;
+        const real ssx = minmod_slope(SW, Sc, SE, wW, wC, wE), ssE = minmod_slope(Sc, SE, SEE, wC, wE, wEE);
+        const real ssW = minmod_slope(SWW, SW, Sc, wWW, wW, wC);
+        const real ssy = minmod_slope(SS, Sc, SN, wS, wC, wN), ssN = minmod_slope(Sc, SN, SNN, wC, wN, wNN);
+        const real ssS = minmod_slope(SSS_, SS, Sc, wSS, wS, wC);
+        sE_ = uxe >= real(0) ? Sc + h * ssx : SE - h * ssE;
+        sW_ = uxw >= real(0) ? SW + h * ssW : Sc - h * ssx;
+        sN_ = vyn >= real(0) ? Sc + h * ssy : SN - h * ssN;
+        sS_ = vys >= real(0) ? SS + h * ssS : Sc - h * ssy;
+      }
+      const real hT = (ufe * tE - ufw * tW) * idx + (vfn * tN * cfN - vfs * tS * cfS) * idyc;
+      const real hS = (ufe * sE_ - ufw * sW_) * idx + (vfn * sN_ * cfN - vfs * sS_ * cfS) * idyc;
       // donor-cell vertical flux on Fz (transport.py:444-475)
       const real upT = k == 0 ? fz0 * Tc : fz0 * (fz0 > real(0) ? Tm : Tc);
       const real upS = k == 0 ? fz0 * Sc : fz0 * (fz0 > real(0) ? Sm : Sc);
@@ -1628,7 +1698,15 @@ void launch_fz(const Geo<real>& g, const real* u, const real* v, real* Fz, hipSt
 template <typename real>
 void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
                  int* gate, hipStream_t s) {
-  const size_t lds = (size_t)2 * g.nz * 64 * sizeof(real);
+  const size_t lds = (size_t)4 * g.nz * 64 * sizeof(real);
+  if (lds > 65536) {
+    static bool raised = false;  // once per instantiation (one per real type)
+    if (!raised) {
+      hipFuncSetAttribute(reinterpret_cast<const void*>(k_conv<real>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          160 * 1024);
+      raised = true;
+    }
+  }
   hipLaunchKernelGGL(k_conv<real>, dim3(blocks_for(g.n2, 64)), dim3(64), lds, s, g, P, T, S, cT, cS, gate);
 }
 
@@ -1637,7 +1715,14 @@ void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, c
                             const real* T, const real* S, const real* Fz, const real* cT, const real* cS,
                             const int* gate, real* dT1, real* dS1, const real* Tb, const real* Sb, real* To,
                             real* So, hipStream_t s) {
-  hipLaunchKernelGGL(k_tracer_residual<real>, dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
+  if (P.adv_scheme == 2)
+    hipLaunchKernelGGL((k_tracer_residual<real, 2>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
+                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
+  else if (P.adv_scheme == 1)
+    hipLaunchKernelGGL((k_tracer_residual<real, 1>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
+                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
+  else
+    hipLaunchKernelGGL((k_tracer_residual<real, 0>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
                      T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
 }
 

@@ -31,8 +31,9 @@ LEGACY_ONLY = {
     "material_point_gradient_scheme": "minmod_v1",
     "momentum_tangent_filter_scheme": "binomial_v1",
 }
-OFF_ONLY = ("dynamic_ice", "freeze_adv_vel", "conservative_kv", "project_adv_vel",
-            "localize_conv", "monotone_adv", "fct_adv", "match_barotropic_transport")
+OFF_ONLY = ("dynamic_ice", "freeze_adv_vel", "project_adv_vel", "match_barotropic_transport")
+# production tracer/mixing options (reference factory.py:49-51, 649-653)
+FLAGS = ("conservative_kv", "localize_conv", "fct_adv", "monotone_adv")
 
 
 def cap_weights(ncap, ntaper):
@@ -68,7 +69,7 @@ def freeze(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
     """Validate the arguments and return the frozen parameter namespace."""
     finite_number("dt", dt, positive=True)
     finite_number("dt_bt", dt_bt, positive=True)
-    unknown = set(schemes) - set(LEGACY_ONLY) - set(OFF_ONLY)
+    unknown = set(schemes) - set(LEGACY_ONLY) - set(OFF_ONLY) - set(FLAGS)
     if unknown:
         raise TypeError(f"unexpected keyword arguments: {sorted(unknown)}")
     for name, default in LEGACY_ONLY.items():
@@ -202,6 +203,8 @@ def freeze(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
         nu_nsub = max(1, int(nu_nsub))
     p.nu_nsub = nu_nsub
     p.n_nu = max(1, int(p.n_subcyc if nu_nsub is None else nu_nsub))
+    for name in FLAGS:
+        setattr(p, name, bool(schemes.get(name, False)))
     min_dz = float(np.min(p.dz_3d))
     p.conv_nsub = max(1, int(np.ceil(p.kappa_conv * dt / (0.4 * min_dz ** 2))))
     p.adv_nsub = max(1, int(np.ceil(dt * 4.0e-3 / (0.5 * min_dz))))
@@ -261,4 +264,8 @@ def make_config(p):
     c.coastal_bulk_lambda = dptr(to_internal2(p.coastal_bulk_lambda_2d))
     c.kappa_h_2d = dptr(to_internal2(p.kappa_h + p.coastal_kappa_h_2d))
     c.kappa_v_2d = dptr(to_internal2(p.kappa_v + p.coastal_kappa_v_2d))
+    c.conservative_kv = int(p.conservative_kv)
+    c.localize_conv = int(p.localize_conv)
+    c.fct_adv = int(p.fct_adv)
+    c.monotone_adv = int(p.monotone_adv)
     return c, keep
