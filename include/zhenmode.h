@@ -214,6 +214,41 @@ int zm_adam(int32_t n_out, int32_t n_in, float* w, const float* gw, float* m, fl
             void* wT_bf, float* b, const float* gb, float* bm, float* bv, float lr, float beta1, float beta2,
             float eps, int32_t t, void* stream);
 
+/* ---- run-time kernels: forcing, monitor, budget --------------------------- */
+
+/* Seasonal forcing blend of interp_monthly_field_jit
+ * (inputs/forcing/seasonal.py:86-103): stack holds 12 months of nfields 2-D
+ * fields, [12][nfields][ny][nx] in the handle's dtype; out [nfields][ny][nx]
+ * = (1 - wa - wb) month(mi) + wa month(mi+1) + wb month(mi-1) for the 30-day
+ * calendar position of `day` (wa, wb from blend_days, 0 <= blend_days <= 30). */
+int zm_seasonal_forcing(zm_handle* h, const void* stack, int32_t nfields, double day, double blend_days,
+                        void* out, void* stream);
+
+/* Device-resident monitor status: classify_state (runtime/monitor.py:20-30)
+ * of the last classified state, running peaks, and the first-rejection
+ * record of the run loop (runtime/run_loop.py:53-96): once a state fails,
+ * later steps leave the status untouched.  failure: 0 accepted, 1 non-finite,
+ * 3 velocity above the limit, 4 surface height above the limit. */
+typedef struct zm_monitor_status {
+  double max_u, max_velocity, max_eta;
+  double peak_u, peak_velocity, peak_eta;
+  int32_t finite, failure;
+  int64_t first_failure_step; /* -1 while every state was accepted */
+  int64_t last_step;
+} zm_monitor_status;
+
+/* status: device pointer to one zm_monitor_status. */
+int zm_monitor_reset(zm_handle* h, zm_monitor_status* status, void* stream);
+/* Classify state number `step` (>= 0, increasing) on the device; no host sync. */
+int zm_monitor(zm_handle* h, const zm_state* state, int64_t step, double velocity_limit, double eta_limit,
+               int32_t inclusive, zm_monitor_status* status, void* stream);
+
+/* Float64 compensated budget sums (diagnostics/snapshot.py:67-128) into the
+ * device array sums[9]: wet volume (area x dz_node), sum T vol, sum S vol, wet
+ * surface area, sum T_top area, sum S_top area, total area, sum eta area over
+ * wet columns, 0. */
+int zm_budget(zm_handle* h, const zm_state* state, double* sums, void* stream);
+
 /* ---- tiles, autotuner and batched ensembles ------------------------------
  * No reference counterpart: the reference steps one grid per jitted call. */
 

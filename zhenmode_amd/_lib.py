@@ -47,6 +47,13 @@ class ZmConfig(ctypes.Structure):
     ]
 
 
+class ZmMonitorStatus(ctypes.Structure):
+    _fields_ = [("max_u", ctypes.c_double), ("max_velocity", ctypes.c_double), ("max_eta", ctypes.c_double),
+                ("peak_u", ctypes.c_double), ("peak_velocity", ctypes.c_double), ("peak_eta", ctypes.c_double),
+                ("finite", ctypes.c_int32), ("failure", ctypes.c_int32),
+                ("first_failure_step", ctypes.c_int64), ("last_step", ctypes.c_int64)]
+
+
 class ZmState(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in ("u", "v", "T", "S", "eta", "ice")]
 
@@ -56,7 +63,7 @@ EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", 
            "zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
            "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
            "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
-           "zm_ensemble_destroy")
+           "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget")
 
 # epilogues of zm_gemm_bf16_nt
 EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
@@ -105,10 +112,16 @@ def load():
     lib.zm_rowsum_bf16.argtypes = [i64, i64, vp, vp, vp]
     lib.zm_transpose_bf16.argtypes = [i64, i64, vp, vp, vp]
     lib.zm_adam.argtypes = [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
+    lib.zm_seasonal_forcing.argtypes = [vp, vp, i32, f64, f64, vp, vp]
+    lib.zm_monitor_reset.argtypes = [vp, vp, vp]
+    lib.zm_monitor.argtypes = [vp, ctypes.POINTER(ZmState), i64, f64, f64, i32, vp, vp]
+    lib.zm_budget.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp]
+    for name in ("zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget"):
+        getattr(lib, name).restype = ctypes.c_int
     for name in ("zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
                  "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
            "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
-           "zm_ensemble_destroy"):
+           "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

@@ -6,6 +6,7 @@
 // from a hipGraph keyed by the state/forcing pointers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +20,7 @@
 #include "zm_common.h"
 #include "zm_column.h"
 #include "zm_kernels.h"
+#include "zm_runtime.h"
 
 namespace {
 
@@ -57,6 +59,11 @@ struct SolverBase {
   virtual void mlp_apply(const zm_state& in, const float* Y, int ldy, const float* omean, const float* ostd,
                          double dt, double kv, double kc, double tol_T, double tol_S, void* To, void* So, int* flags,
                          int* count, hipStream_t s) = 0;
+  virtual void seasonal(const void* stack, int nfields, double day, double blend_days, void* out,
+                        hipStream_t s) = 0;
+  virtual void monitor(const zm_state& st, long long step, double vlim, double elim, int inclusive, void* status,
+                       hipStream_t s) = 0;
+  virtual void budget(const zm_state& st, double* sums, hipStream_t s) = 0;
   virtual void set_tiles(int nu_block, int seg) = 0;
   virtual void get_tiles(int* nu_block, int* seg) const = 0;
   int nz_ = 0;
@@ -80,6 +87,8 @@ struct Solver : SolverBase {
   real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fden = nullptr;
   double *dzn64 = nullptr, *dzi64 = nullptr;
   double* vmix_scratch = nullptr;  // Thomas c', d' (3 n3), allocated on first use
+  double* area64 = nullptr;        // cell area per latitude row (dx dy), float64
+  double* red_partials = nullptr;  // monitor / budget block partials
   double T_ref64 = 0, S_ref64 = 0;
   // workspace
   real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
@@ -206,6 +215,13 @@ struct Solver : SolverBase {
     ZM_HIP(hipMemcpy(dzn64, c.dz_node, nz * sizeof(double), hipMemcpyHostToDevice));
     dzi64 = dev_alloc<double>(nz > 1 ? nz - 1 : 1);
     if (nz > 1) ZM_HIP(hipMemcpy(dzi64, c.dz_iface, (nz - 1) * sizeof(double), hipMemcpyHostToDevice));
+    {
+      std::vector<double> ar(ny);
+      for (int j = 0; j < ny; ++j) ar[j] = c.dx_row[j] * c.dy;
+      area64 = dev_alloc<double>(ny);
+      ZM_HIP(hipMemcpy(area64, ar.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+      red_partials = dev_alloc<double>((size_t)9 * zm::kReduceBlocks);
+    }
     T_ref64 = c.T_ref;
     S_ref64 = c.S_ref;
     nz_ = nz;
@@ -428,6 +444,38 @@ struct Solver : SolverBase {
     pre_barotropic(in, Pf, s);
     barotropic(Pf, (const real*)in.eta, s);
     post_barotropic(in, out, Pf, s);
+  }
+
+  void seasonal(const void* stack, int nfields, double day, double blend_days, void* out,
+                hipStream_t s) override {
+    // the 30-day calendar of interp_monthly_field_jit (inputs/forcing/seasonal.py:86-103)
+    const double month_len = 30.0;
+    double mpos = std::fmod(day, month_len);
+    if (mpos < 0) mpos += month_len;
+    long long mi = (long long)std::floor(day / month_len) % 12;
+    if (mi < 0) mi += 12;
+    const double half = blend_days / 2.0;
+    double wa = 0.0, wb = 0.0;
+    if (blend_days > 0.0) {
+      wa = std::min(1.0, std::max(0.0, (mpos - (month_len - half)) / blend_days));
+      wb = std::min(1.0, std::max(0.0, (half - mpos) / blend_days));
+    }
+    zm::launch_seasonal_blend((const real*)stack, nfields, n2, (int)mi, wa, wb, (real*)out, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void monitor(const zm_state& st, long long step, double vlim, double elim, int inclusive, void* status,
+               hipStream_t s) override {
+    zm::launch_monitor(g, (const real*)st.u, (const real*)st.v, (const real*)st.T, (const real*)st.S,
+                       (const real*)st.eta, (const real*)st.ice, step, vlim, elim, inclusive, red_partials,
+                       (zm::MonitorStatus*)status, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void budget(const zm_state& st, double* sums, hipStream_t s) override {
+    zm::launch_budget(g, area64, dzn64, (const real*)st.T, (const real*)st.S, (const real*)st.eta, red_partials, sums,
+                      s);
+    ZM_HIP(hipGetLastError());
   }
 
   void set_tiles(int nu, int seg) override {
@@ -1067,6 +1115,35 @@ int zm_autotune(zm_handle* h, const zm_state* sample, int32_t reps, int32_t* nu_
     *seg = best_seg;
     *best_ms = best;
   });
+}
+
+int zm_seasonal_forcing(zm_handle* h, const void* stack, int32_t nfields, double day, double blend_days, void* out,
+                        void* stream) {
+  if (!h || !stack || !out || nfields < 1 || !std::isfinite(day) || !std::isfinite(blend_days) ||
+      blend_days > 30.0)
+    return fail(h, "invalid zm_seasonal_forcing arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->seasonal(stack, nfields, day, blend_days, out, (hipStream_t)stream); });
+}
+
+int zm_monitor_reset(zm_handle* h, zm_monitor_status* status, void* stream) {
+  if (!h || !status) return fail(h, "invalid zm_monitor_reset arguments", ZM_EINVAL);
+  return guarded(h, [&] {
+    zm::launch_monitor_reset((zm::MonitorStatus*)status, (hipStream_t)stream);
+    ZM_HIP(hipGetLastError());
+  });
+}
+
+int zm_monitor(zm_handle* h, const zm_state* st, int64_t step, double velocity_limit, double eta_limit,
+               int32_t inclusive, zm_monitor_status* status, void* stream) {
+  if (!h || !st || !status || step < 0) return fail(h, "invalid zm_monitor arguments", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->monitor(*st, step, velocity_limit, eta_limit, inclusive, status, (hipStream_t)stream);
+  });
+}
+
+int zm_budget(zm_handle* h, const zm_state* st, double* sums, void* stream) {
+  if (!h || !st || !sums) return fail(h, "invalid zm_budget arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->budget(*st, sums, (hipStream_t)stream); });
 }
 
 int zm_ensemble_create(zm_handle* const* members, int32_t n, zm_ensemble** out) {
