@@ -201,6 +201,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--tiles", default="", help="nu_block,seg: fixed launch shapes instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
@@ -224,7 +225,12 @@ def main():
     a = init_state(case.T_init, case.S_init)
     b = solver.new_state()
     # launch shapes for this device and grid (results do not depend on them)
-    tiles = solver.autotune(a, reps=5)
+    if args.tiles:
+        nu_b, seg = (int(x) for x in args.tiles.split(","))
+        solver.set_tiles(nu_b, seg)
+        tiles = (nu_b, seg, None)
+    else:
+        tiles = solver.autotune(a, reps=5)
     # on-device seasonal wind: tau_x(t) = (0.1 + 0.05 sin(2 pi t / 365 d)) cos(3 lat)
     from zhenmode_amd.solver import empty2
     base = torch.as_tensor(np.cos(3.0 * np.radians(g.lat)), dtype=tdtype, device=dev)[:, None].expand(g.ny, g.nx).contiguous()

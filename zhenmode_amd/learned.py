@@ -221,15 +221,16 @@ class LearnedMixer:
         _lib.check(rc, s.handle)
         return X
 
-    def apply(self, state, dt, tau_x=None, tau_y=None, q_heat=None):
-        """Mixed state; self.count accumulates the columns sent to the fallback."""
+    def apply(self, state, dt, tau_x=None, tau_y=None, q_heat=None, out=None):
+        """Mixed state (T, S into ``out`` when given); self.count accumulates
+        the columns sent to the fallback."""
         s = self.solver
         src = s.coerce(state)
         X = self.features(src, tau_x, tau_y, q_heat)
         Y, _ = self.mlp.forward(X)
         from .solver import empty3
-        To = empty3(*s.shape3, s.dtype, s.device)
-        So = empty3(*s.shape3, s.dtype, s.device)
+        To = empty3(*s.shape3, s.dtype, s.device) if out is None else out[0]
+        So = empty3(*s.shape3, s.dtype, s.device) if out is None else out[1]
         rc = s.lib.zm_mlp_apply(s.handle, ctypes.byref(s.zm_state(src)), _p(Y), N_OUT_PAD, _p(self.mlp.out_mean),
                                 _p(self.mlp.out_std), float(dt), self.kappa_v, self.kappa_conv, self.tol_T,
                                 self.tol_S, _p(To), _p(So), _p(self.flags), _p(self.count), s._stream())
@@ -284,20 +285,37 @@ class MixingStep:
         self.mode = mode
         self.mixer = (LearnedMixer(self.solver, mlp, self.kappa_v, self.kappa_conv, tol_T, tol_S)
                       if mode == "learned" else None)
+        self._bufs = None
+        self._k = 0
 
-    def mix(self, state, tau_x=None, tau_y=None, q_heat=None):
+    def mix(self, state, tau_x=None, tau_y=None, q_heat=None, out=None):
         if self.mode == "thomas":
-            return self.solver.vertical_mix(state, self.dt, self.kappa_v, self.kappa_conv)
-        return self.mixer.apply(state, self.dt, tau_x, tau_y, q_heat)
+            return self.solver.vertical_mix(state, self.dt, self.kappa_v, self.kappa_conv, out=out)
+        return self.mixer.apply(state, self.dt, tau_x, tau_y, q_heat, out=out)
+
+    def _buffers(self):
+        """Two-deep ping-pong of step outputs and mixed T, S: the returned
+        state stays valid until the second next call (the step's hipGraph is
+        then replayed from a fixed set of pointers)."""
+        if self._bufs is None:
+            from .solver import empty3
+            s = self.solver
+            self._bufs = [(s.new_state(), (empty3(*s.shape3, s.dtype, s.device), empty3(*s.shape3, s.dtype, s.device)))
+                          for _ in range(2)]
+        k = self._k
+        self._k ^= 1
+        return self._bufs[k]
 
     def __call__(self, state):
-        return self.mix(self.solver.step(state))
+        stepped, ts = self._buffers()
+        return self.mix(self.solver.step(state, out=stepped), out=ts)
 
     def step_dyn(self, state, tau_x, tau_y, q_heat):
-        s1 = self.solver.step_dyn(state, tau_x, tau_y, q_heat)
+        stepped, ts = self._buffers()
+        s1 = self.solver.step_dyn(state, tau_x, tau_y, q_heat, out=stepped)
         s = self.solver
         tx, ty, qh = (_as2(a, s) for a in (tau_x, tau_y, q_heat))
-        return self.mix(s1, tx, ty, qh)
+        return self.mix(s1, tx, ty, qh, out=ts)
 
     def fallback_count(self):
         return int(self.mixer.count.item()) if self.mixer is not None else 0
