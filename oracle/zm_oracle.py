@@ -105,7 +105,9 @@ def make_params(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
                 sponge_days=0.0, sponge_cells=0, T_init=None, S_init=None,
                 polar_cap_rows=2, polar_cap_taper=3, dt_bt=150.0, nu_nsub=None,
                 ice_freeze_temp_c=-1.8, ice_salt_flux=0.0, conservative_kv=False, localize_conv=False,
-                fct_adv=False, monotone_adv=False):
+                fct_adv=False, monotone_adv=False, project_adv_vel=False, projection_niter=None,
+                projection_rtol=None, projection_preconditioner="none", projection_max_refinements=2,
+                projection_eps=np.finfo(np.float64).eps):
     """Frozen parameter set of the legacy mode-split step (float64).
 
     The last four flags are the production tracer/mixing options of the
@@ -241,6 +243,13 @@ def make_params(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
     p.localize_conv = bool(localize_conv)
     p.fct_adv = bool(fct_adv)
     p.monotone_adv = bool(monotone_adv)
+    # column-divergence projection of the stage-2 tracer velocity (factory.py:368-386)
+    p.project_adv_vel = bool(project_adv_vel)
+    p.projection_niter = 150 if projection_niter is None else int(projection_niter)
+    p.projection_rtol = max(float(projection_rtol or 1e-12), 32.0 * float(projection_eps))
+    p.projection_eps = float(projection_eps)
+    p.projection_preconditioner = projection_preconditioner
+    p.projection_max_refinements = int(projection_max_refinements)
     return p
 
 
@@ -829,7 +838,11 @@ def explicit_full_step(state, p, dt):
     dv1 = dv1 + p.r_bot * v * bm
     u_pred = u + du1 * dt
     v_pred = v + dv1 * dt
-    dT2, dS2 = tracer_residual(u_pred, v_pred, T_pred, S_pred, p)
+    if getattr(p, "project_adv_vel", False):
+        u_adv, v_adv = project_column_divergence(u_pred, v_pred, p, dt)
+    else:
+        u_adv, v_adv = u_pred, v_pred
+    dT2, dS2 = tracer_residual(u_adv, v_adv, T_pred, S_pred, p)
     T_new = T + 0.5 * (dT1 + dT2) * dt
     S_new = S + 0.5 * (dS1 + dS2) * dt
     du2, dv2 = momentum_residual(u, v, T_new, S_new, eta, p)
@@ -839,6 +852,106 @@ def explicit_full_step(state, p, dt):
     v_new = v + 0.5 * (dv1 + dv2) * dt
     decay = np.exp(-p.r_bot * dt * bm)
     return State(u_new * decay, v_new * decay, T_new, S_new, eta, state.ice)
+
+
+# --------------------------------------------------------------------------
+# column-divergence projection (reference dynamics/projection.py:64-127)
+# --------------------------------------------------------------------------
+def column_divergence(u, v, p):
+    """sum_k div_h dz_node, masked (transport.py:45-57, legacy branch)."""
+    return np.sum(divergence_h(u, v, p) * p.dz_node, axis=-1) * p.wet_mask
+
+
+def projection_matvec(psi, p):
+    """-column_divergence(grad_c3d(psi)) * area (projection.py:85-87)."""
+    gx, gy = gradient_conservative_3d(psi[:, :, None], p)
+    return -column_divergence(gx, gy, p) * (p.dx_2d * p.dy * p.wet_mask)
+
+
+def projection_diagonal(p):
+    """Exact diagonal of the projection operator, legacy branch (projection.py:10-50)."""
+    wet = p.wet_mask_z
+    pos_x = wet * east(wet)
+    neg_x = wet * west(wet)
+    diag_x = 0.25 * p.inv_dx[..., :1] ** 2 * ((pos_x - neg_x) ** 2 + pos_x ** 2 + neg_x ** 2)
+    if p.nx <= 2:
+        diag_x = np.zeros_like(wet)
+    c = _row(p.cos_lat, 3)
+    cn, cs = np.roll(c, -1, axis=1), np.roll(c, 1, axis=1)
+    pos_y = wet * north_wrap(wet)
+    pos_y[:, -1, :] = 0.0
+    neg_y = wet * south_wrap(wet)
+    neg_y[:, 0, :] = 0.0
+    pos_y = pos_y * 0.5 * (c + cn)
+    neg_y = neg_y * 0.5 * (c + cs)
+    diag_y = 0.25 * p.inv_dy ** 2 * ((pos_y - neg_y) ** 2 / c ** 2 + pos_y ** 2 / (c * cn) + neg_y ** 2 / (c * cs))
+    area = p.dx_2d * p.dy * p.wet_mask
+    return area * np.sum((diag_x + diag_y) * p.dz_node, axis=-1)
+
+
+def pcg(matvec, b, tol, atol, maxiter, precond=None):
+    """Preconditioned conjugate gradients from x0 = 0 with the stopping rule of
+    jax.scipy.sparse.linalg.cg: iterate while |r|^2 > max(tol^2 |b|^2, atol^2)
+    and k < maxiter (the |r|^2 test uses r, not the preconditioned residual)."""
+    M = precond if precond is not None else (lambda r: r)
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = M(r)
+    p_ = z.copy()
+    gamma = float(np.vdot(r, z))
+    atol2 = max(tol * tol * float(np.vdot(b, b)), atol * atol)
+    k = 0
+    while float(np.vdot(r, r)) > atol2 and k < maxiter:
+        Ap = matvec(p_)
+        alpha = gamma / float(np.vdot(p_, Ap))
+        x = x + alpha * p_
+        r = r - alpha * Ap
+        z = M(r)
+        gamma_new = float(np.vdot(r, z))
+        p_ = z + (gamma_new / gamma) * p_
+        gamma = gamma_new
+        k += 1
+    return x, k
+
+
+def project_column_divergence(u, v, p, dt, return_iterations=False):
+    """(u, v) minus dt g grad(psi) removing the column divergence, with the
+    bounded transport-checked refinements (projection.py:64-127)."""
+    g = G_EARTH
+    area = p.dx_2d * p.dy * p.wet_mask
+    rhs = -(column_divergence(u, v, p) / (dt * g)) * area
+    tol = p.projection_rtol
+    precond = None
+    if p.projection_preconditioner == "jacobi":
+        d = projection_diagonal(p)
+        inv = 1.0 / np.where(d > 0.0, d, 1.0)
+        precond = lambda r: r * inv  # noqa: E731
+    matvec = lambda x: projection_matvec(x, p)  # noqa: E731
+    psi, its = pcg(matvec, rhs, tol, 0.0, p.projection_niter, precond)
+    iterations = [its]
+
+    def correct(vel, pot):
+        gx, gy = gradient_conservative_3d(pot[:, :, None], p)
+        return vel[0] - (dt * g) * gx, vel[1] - (dt * g) * gy
+
+    corrected = correct((u, v), psi)
+
+    def transport_norm2(vel):
+        top = vertical_transport_iface(vel[0], vel[1], p)[..., 0]
+        return float(np.sum(top ** 2 * area))
+
+    t_tol = max(1e-10, 32.0 * p.projection_eps)
+    threshold = t_tol ** 2 * transport_norm2((u, v))
+    floor = tol * float(np.linalg.norm(rhs))
+    for _ in range(p.projection_max_refinements):
+        if transport_norm2(corrected) > threshold:
+            res = -(column_divergence(*corrected, p) / (dt * g)) * area
+            delta, its = pcg(matvec, res, tol, floor, p.projection_niter, precond)
+            iterations.append(its)
+            corrected = correct(corrected, delta)
+    if return_iterations:
+        return corrected, iterations
+    return corrected
 
 
 def free_surface_substep(eta, ubt, vbt, Fx_rho, Fy_rho, p):
