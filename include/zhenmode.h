@@ -82,6 +82,14 @@ typedef struct zm_config {
   int32_t localize_conv;        /* convective flux needs its own interface unstable (tendencies.py:149-165) */
   int32_t fct_adv;              /* minmod-limited tracer face values (transport.py:199-245, 266-275) */
   int32_t monotone_adv;         /* donor-cell tracer face values; fct_adv takes precedence */
+  /* column-divergence projection of the advecting velocity of the second
+   * tracer residual (factory.py:49-53, 368-386; dynamics/projection.py:64-127) */
+  int32_t project_adv_vel;
+  int32_t projection_niter;            /* CG iteration cap per solve (>= 1) */
+  int32_t projection_preconditioner;   /* 0 none, 1 Jacobi */
+  int32_t projection_max_refinements;  /* 0..2 transport-checked re-solves */
+  double projection_rtol;              /* CG relative tolerance, dtype floor applied */
+  double projection_transport_rtol;    /* max(1e-10, 32 eps(dtype)) */
 } zm_config;
 
 /* Prognostic state (reference JaxStateG(u, v, T, S, eta, ice),
@@ -248,6 +256,31 @@ int zm_monitor(zm_handle* h, const zm_state* state, int64_t step, double velocit
  * surface area, sum T_top area, sum S_top area, total area, sum eta area over
  * wet columns, 0. */
 int zm_budget(zm_handle* h, const zm_state* state, double* sums, void* stream);
+
+/* ---- column-divergence projection ----------------------------------------
+ * Replaces _project_column_divergence(u, v, p, dt) (dynamics/projection.py:64-127):
+ * (u_out, v_out) = (u, v) - dt g grad_c3d(psi) with psi the preconditioned CG
+ * solution of the area-weighted column-divergence Poisson problem, then up to
+ * projection_max_refinements re-solves while the surface transport norm^2
+ * exceeds transport_rtol^2 times that of (u, v).  Float64 solver arithmetic
+ * in both dtypes; u, v, u_out, v_out are 3-D fields in the handle's dtype
+ * (in place allowed).  iterations: optional device int32[3] receiving the CG
+ * iterations of the first solve and of each refinement (0 = not run). */
+int zm_project_velocity(zm_handle* h, const void* u, const void* v, void* u_out, void* v_out, double dt,
+                        int32_t* iterations, void* stream);
+/* The projection operator A(psi) = -column_divergence(grad_c3d(psi)) * area
+ * (projection.py:85-87) as nine taps per column, computed on the host from
+ * the grid fields of cfg (nwet, wet_mask, dx_row, dy, cos_lat, dz_node):
+ *   A(psi)[c] = sum_t taps[t][c] psi[c + offset_t], t = centre, east, west,
+ *   east+2, west+2, north, south, north+2, south+2 (x periodic; taps reaching
+ *   past the first or last row are zero), c = j*nx + i.
+ * diagonal receives _column_projection_diagonal (projection.py:10-50), the
+ * Jacobi scaling.  taps [9*ny*nx] and diagonal [ny*nx] are host arrays; either
+ * may be NULL.  No device work. */
+int zm_projection_operator(const zm_config* cfg, double* taps, double* diagonal);
+/* CG iterations of the last projection issued on the handle (by a step with
+ * project_adv_vel or by zm_project_velocity) into device int32[3]. */
+int zm_projection_iterations(zm_handle* h, int32_t* iterations, void* stream);
 
 /* ---- tiles, autotuner and batched ensembles ------------------------------
  * No reference counterpart: the reference steps one grid per jitted call. */

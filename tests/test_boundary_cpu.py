@@ -72,7 +72,12 @@ def test_host_freeze_matches_oracle_parameters():
     (dict(mode_split=False), "mode_split"),
     (dict(mode_split=True, dtype="float16"), "dtype"),
     (dict(mode_split=True, column_geometry="nodal_dual_v1"), "column_geometry"),
-    (dict(mode_split=True, project_adv_vel=True), "project_adv_vel"),
+    (dict(mode_split=True, freeze_adv_vel=True), "freeze_adv_vel"),
+    (dict(mode_split=True, projection_niter=0), "projection_niter"),
+    (dict(mode_split=True, projection_niter=True), "projection_niter"),
+    (dict(mode_split=True, projection_rtol=-1.0), "projection_rtol"),
+    (dict(mode_split=True, projection_preconditioner="ilu"), "none or jacobi"),
+    (dict(mode_split=True, projection_max_refinements=3), "0 to 2"),
     (dict(mode_split=True, sponge_days=10.0, sponge_cells=3), "sponge"),
     (dict(mode_split=True, polar_cap_rows=30, polar_cap_taper=3), "overlap"),
     (dict(mode_split=True, dt_bt=-1.0), "dt_bt"),
@@ -111,3 +116,25 @@ def test_production_flags_are_frozen_into_the_config():
                mode_split=True)
     cfg, keep = make_config(p)
     assert (cfg.conservative_kv, cfg.localize_conv, cfg.fct_adv, cfg.monotone_adv) == (1, 1, 1, 0)
+
+
+def test_projection_options_resolve_like_the_reference(monkeypatch):
+    """factory.py:368-386 and projection.py:55-62: iteration cap from the
+    argument, else OCEAN_PAV_NITER, else 150; tolerances floored at 32 eps."""
+    case = make_case("g4deg")
+    p = freeze(case.grid, case.physics, case.dt, mode_split=True, dtype="float32", project_adv_vel=True,
+               projection_preconditioner="jacobi")
+    eps32 = float(np.finfo(np.float32).eps)
+    assert p.project_adv_vel and p.projection_niter == 150
+    assert p.projection_rtol == 32.0 * eps32 and p.projection_transport_rtol == 32.0 * eps32
+    cfg, _ = make_config(p)
+    assert (cfg.project_adv_vel, cfg.projection_niter, cfg.projection_preconditioner) == (1, 150, 1)
+    assert cfg.projection_max_refinements == 2 and cfg.projection_rtol == p.projection_rtol
+    q = freeze(case.grid, case.physics, case.dt, mode_split=True, projection_rtol=1e-9)
+    assert q.projection_rtol == 1e-9 and q.projection_transport_rtol == 1e-10 and not q.project_adv_vel
+    monkeypatch.setenv("OCEAN_PAV_NITER", "40")
+    assert freeze(case.grid, case.physics, case.dt, mode_split=True).projection_niter == 40
+    assert freeze(case.grid, case.physics, case.dt, mode_split=True, projection_niter=7).projection_niter == 7
+    monkeypatch.setenv("OCEAN_PAV_NITER", "many")
+    with pytest.raises(ValueError, match="environment"):
+        freeze(case.grid, case.physics, case.dt, mode_split=True)

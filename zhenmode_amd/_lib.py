@@ -44,6 +44,9 @@ class ZmConfig(ctypes.Structure):
         ("coastal_bulk_lambda", _D), ("kappa_h_2d", _D), ("kappa_v_2d", _D),
         ("conservative_kv", ctypes.c_int32), ("localize_conv", ctypes.c_int32),
         ("fct_adv", ctypes.c_int32), ("monotone_adv", ctypes.c_int32),
+        ("project_adv_vel", ctypes.c_int32), ("projection_niter", ctypes.c_int32),
+        ("projection_preconditioner", ctypes.c_int32), ("projection_max_refinements", ctypes.c_int32),
+        ("projection_rtol", ctypes.c_double), ("projection_transport_rtol", ctypes.c_double),
     ]
 
 
@@ -63,7 +66,8 @@ EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", 
            "zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
            "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
            "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
-           "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget")
+           "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget",
+           "zm_project_velocity", "zm_projection_iterations", "zm_projection_operator")
 
 # epilogues of zm_gemm_bf16_nt
 EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
@@ -116,7 +120,11 @@ def load():
     lib.zm_monitor_reset.argtypes = [vp, vp, vp]
     lib.zm_monitor.argtypes = [vp, ctypes.POINTER(ZmState), i64, f64, f64, i32, vp, vp]
     lib.zm_budget.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp]
-    for name in ("zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget"):
+    lib.zm_project_velocity.argtypes = [vp, vp, vp, vp, vp, f64, vp, vp]
+    lib.zm_projection_iterations.argtypes = [vp, vp, vp]
+    lib.zm_projection_operator.argtypes = [ctypes.POINTER(ZmConfig), vp, vp]
+    for name in ("zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget", "zm_project_velocity",
+                 "zm_projection_iterations", "zm_projection_operator"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
                  "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",

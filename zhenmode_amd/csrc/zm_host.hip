@@ -20,6 +20,7 @@
 #include "zm_common.h"
 #include "zm_column.h"
 #include "zm_kernels.h"
+#include "zm_project.h"
 #include "zm_runtime.h"
 
 namespace {
@@ -64,6 +65,9 @@ struct SolverBase {
   virtual void monitor(const zm_state& st, long long step, double vlim, double elim, int inclusive, void* status,
                        hipStream_t s) = 0;
   virtual void budget(const zm_state& st, double* sums, hipStream_t s) = 0;
+  virtual void project_velocity(const void* u, const void* v, void* uo, void* vo, double dt, int* its,
+                                hipStream_t s) = 0;
+  virtual void projection_iterations(int* its, hipStream_t s) = 0;
   virtual void set_tiles(int nu_block, int seg) = 0;
   virtual void get_tiles(int* nu_block, int* seg) const = 0;
   int nz_ = 0;
@@ -90,6 +94,13 @@ struct Solver : SolverBase {
   double* area64 = nullptr;        // cell area per latitude row (dx dy), float64
   double* red_partials = nullptr;  // monitor / budget block partials
   double T_ref64 = 0, S_ref64 = 0;
+  // column-divergence projection (float64 solver state)
+  bool proj_on = false;
+  int proj_niter = 1, proj_maxref = 0;
+  double dt64 = 0;
+  zm::ProjGeo pg{};
+  zm::ProjWork pw{};
+  real *ua = nullptr, *va = nullptr;  // projected advecting velocity
   // workspace
   real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
   real *Fz1, *Fz2, *cT, *cS;
@@ -224,6 +235,8 @@ struct Solver : SolverBase {
     }
     T_ref64 = c.T_ref;
     S_ref64 = c.S_ref;
+    dt64 = c.dt;
+    setup_projection(c);
     nz_ = nz;
     n2_ = n2;
 
@@ -326,6 +339,85 @@ struct Solver : SolverBase {
     ZM_HIP(hipDeviceSynchronize());
   }
 
+  // operator taps, Jacobi scaling and solver buffers of the projection
+  void setup_projection(const zm_config& c) {
+    proj_on = c.project_adv_vel != 0;
+    proj_niter = c.projection_niter;
+    proj_maxref = c.projection_max_refinements;
+    std::vector<double> coef((size_t)zm::kProjTaps * n2), diag(n2), inv(n2), wet(c.wet_mask, c.wet_mask + n2);
+    zm::proj_stencil_host(nx, ny, nz, c.nwet, c.wet_mask, c.dx_row, c.dy, c.cos_lat, c.dz_node, coef.data(),
+                          diag.data());
+    for (long long q = 0; q < n2; ++q)
+      inv[q] = c.projection_preconditioner == 1 ? 1.0 / (diag[q] > 0.0 ? diag[q] : 1.0) : 1.0;
+    std::vector<double> idx(ny), idyc(ny), cfn(ny), cfs(ny);
+    for (int j = 0; j < ny; ++j) {
+      idx[j] = 1.0 / c.dx_row[j];
+      idyc[j] = (1.0 / c.dy) / c.cos_lat[j];
+      cfn[j] = 0.5 * (c.cos_lat[j] + c.cos_lat[(j + 1) % ny]);
+      cfs[j] = 0.5 * (c.cos_lat[j] + c.cos_lat[(j + ny - 1) % ny]);
+    }
+    auto up64 = [&](const std::vector<double>& h) {
+      double* d = dev_alloc<double>(h.size());
+      ZM_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+      return d;
+    };
+    pg.nx = nx;
+    pg.ny = ny;
+    pg.n2 = (int)n2;
+    pg.nblocks = (int)((n2 + zm::kProjBlock - 1) / zm::kProjBlock);
+    pg.coef = up64(coef);
+    pg.inv = up64(inv);
+    pg.wet = up64(wet);
+    pg.idx = up64(idx);
+    pg.idyc = up64(idyc);
+    pg.cfn = up64(cfn);
+    pg.cfs = up64(cfs);
+    pg.area = area64;
+    pg.dzn = dzn64;
+    pw.x = dev_alloc<double>(n2);
+    pw.r = dev_alloc<double>(n2);
+    pw.p0 = dev_alloc<double>(n2);
+    pw.p1 = dev_alloc<double>(n2);
+    pw.Ap = dev_alloc<double>(n2);
+    pw.partials = dev_alloc<double>((size_t)4 * pg.nblocks);
+    pw.sc = dev_alloc<zm::ProjScalars>(1);
+    zm::ProjScalars sc{};
+    sc.tol2 = c.projection_rtol * c.projection_rtol;
+    sc.ttol2 = c.projection_transport_rtol * c.projection_transport_rtol;
+    sc.maxiter = proj_niter;
+    sc.done = 1;
+    ZM_HIP(hipMemcpy(pw.sc, &sc, sizeof(sc), hipMemcpyHostToDevice));
+    if (proj_on) {
+      ua = dev_alloc<real>(n3);
+      va = dev_alloc<real>(n3);
+    }
+  }
+
+  // _project_column_divergence(u, v, p, dt) (dynamics/projection.py:64-127)
+  void project(const real* u, const real* v, real* uo, real* vo, double dt, hipStream_t s) {
+    const double dtg = dt * zm::kG;
+    zm::launch_proj_rhs(g, pg, pw, u, v, dtg, 0, s);
+    zm::launch_proj_cg(pg, pw, proj_niter, 0, s);
+    zm::launch_proj_correct(g, pg, pw, dtg, u, v, uo, vo, s);
+    for (int q = 0; q < proj_maxref; ++q) {
+      zm::launch_proj_rhs(g, pg, pw, (const real*)uo, (const real*)vo, dtg, 1, s);
+      zm::launch_proj_cg(pg, pw, proj_niter, q + 1, s);
+      zm::launch_proj_correct(g, pg, pw, dtg, (const real*)uo, (const real*)vo, uo, vo, s);
+    }
+  }
+
+  void project_velocity(const void* u, const void* v, void* uo, void* vo, double dt, int* its,
+                        hipStream_t s) override {
+    if (proj_niter < 1) throw ConfigError{"projection_niter must be a positive integer"};
+    project((const real*)u, (const real*)v, (real*)uo, (real*)vo, dt, s);
+    if (its) projection_iterations(its, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void projection_iterations(int* its, hipStream_t s) override {
+    ZM_HIP(hipMemcpyAsync(its, pw.sc->its, 3 * sizeof(int), hipMemcpyDeviceToDevice, s));
+  }
+
   // x part of the de-alias of both advection fields (one complex row each)
   void dealias(const real* a, const real* b, real* oa, real* ob, hipStream_t s) {
     if (use_fft)
@@ -374,10 +466,18 @@ struct Solver : SolverBase {
     zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
     dealias(au, av, axu, axv, s);
     zm::launch_mom_residual(g, Pf, 0, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, up, vp, decay_bot, s);
-    // tracer residual #2 at (u_pred, v_pred, T_pred, S_pred) -> T_new
-    zm::launch_fz(g, up, vp, Fz2, s);
+    // tracer residual #2 at (u_adv, v_adv, T_pred, S_pred) -> T_new, with
+    // (u_adv, v_adv) the predictor velocity or its column-divergence projection
+    const real* uad = up;
+    const real* vad = vp;
+    if (proj_on) {
+      project(up, vp, ua, va, dt64, s);
+      uad = ua;
+      vad = va;
+    }
+    zm::launch_fz(g, uad, vad, Fz2, s);
     zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, s);
-    zm::launch_tracer_residual(g, Pf, 1, up, vp, Tp, Sp, Fz2, cT, cS, gate, dT1, dS1, T1, S1, To, So, s);
+    zm::launch_tracer_residual(g, Pf, 1, uad, vad, Tp, Sp, Fz2, cT, cS, gate, dT1, dS1, T1, S1, To, So, s);
     // momentum residual #2 at (u1, v1, T_new, S_new); advection of (u1, v1) is reused
     zm::launch_pbc(g, Pf, To, So, pbc, s);
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
@@ -731,6 +831,16 @@ void validate(const zm_config& c) {
   if ((long long)4 * c.nz * 64 * c.dtype_bytes > 160 * 1024)
     bad("too many levels for the convective column kernel (nz <= 156 in float32, 78 in float64)");
   if (c.localize_conv && c.nz > 129) bad("localize_conv supports at most 129 levels");
+  if (c.project_adv_vel) {
+    if (c.projection_niter < 1) bad("projection_niter must be a positive integer");
+    if (c.projection_max_refinements < 0 || c.projection_max_refinements > 2)
+      bad("projection_max_refinements must be an integer from 0 to 2");
+    if (c.projection_preconditioner != 0 && c.projection_preconditioner != 1)
+      bad("projection_preconditioner must be none or jacobi");
+    if (!(c.projection_rtol > 0) || !std::isfinite(c.projection_rtol)) bad("projection_rtol must be finite and positive");
+    if (!(c.projection_transport_rtol > 0) || !std::isfinite(c.projection_transport_rtol))
+      bad("projection_transport_rtol must be finite and positive");
+  }
   if (c.polar_cap_rows < 0 || c.polar_cap_taper < 0) bad("polar cap rows must be >= 0");
   if (c.polar_cap_rows > 0 && 2 * (c.polar_cap_rows + c.polar_cap_taper) > c.ny)
     bad("polar cap bands must not overlap");
@@ -1144,6 +1254,34 @@ int zm_monitor(zm_handle* h, const zm_state* st, int64_t step, double velocity_l
 int zm_budget(zm_handle* h, const zm_state* st, double* sums, void* stream) {
   if (!h || !st || !sums) return fail(h, "invalid zm_budget arguments", ZM_EINVAL);
   return guarded(h, [&] { h->solver->budget(*st, sums, (hipStream_t)stream); });
+}
+
+int zm_project_velocity(zm_handle* h, const void* u, const void* v, void* u_out, void* v_out, double dt,
+                        int32_t* iterations, void* stream) {
+  if (!h || !u || !v || !u_out || !v_out || !(dt > 0.0) || !std::isfinite(dt))
+    return fail(h, "invalid zm_project_velocity arguments", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->project_velocity(u, v, u_out, v_out, dt, iterations, (hipStream_t)stream);
+  });
+}
+
+int zm_projection_operator(const zm_config* cfg, double* taps, double* diagonal) {
+  if (!cfg || !cfg->nwet || !cfg->wet_mask || !cfg->dx_row || !cfg->cos_lat || !cfg->dz_node || cfg->nx < 2 ||
+      cfg->ny < 2 || cfg->nz < 1 || !(cfg->dy > 0))
+    return fail(nullptr, "invalid zm_projection_operator arguments", ZM_EINVAL);
+  return guarded(nullptr, [&] {
+    const long long n2 = (long long)cfg->nx * cfg->ny;
+    std::vector<double> t((size_t)zm::kProjTaps * n2), d(n2);
+    zm::proj_stencil_host(cfg->nx, cfg->ny, cfg->nz, cfg->nwet, cfg->wet_mask, cfg->dx_row, cfg->dy, cfg->cos_lat,
+                          cfg->dz_node, t.data(), d.data());
+    if (taps) std::copy(t.begin(), t.end(), taps);
+    if (diagonal) std::copy(d.begin(), d.end(), diagonal);
+  });
+}
+
+int zm_projection_iterations(zm_handle* h, int32_t* iterations, void* stream) {
+  if (!h || !iterations) return fail(h, "invalid zm_projection_iterations arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->projection_iterations(iterations, (hipStream_t)stream); });
 }
 
 int zm_ensemble_create(zm_handle* const* members, int32_t n, zm_ensemble** out) {

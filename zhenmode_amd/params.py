@@ -11,6 +11,7 @@ that names them.  The result is handed to ``zm_create`` unchanged.
 from __future__ import annotations
 
 import ctypes
+import os
 from types import SimpleNamespace
 
 import numpy as np
@@ -31,9 +32,41 @@ LEGACY_ONLY = {
     "material_point_gradient_scheme": "minmod_v1",
     "momentum_tangent_filter_scheme": "binomial_v1",
 }
-OFF_ONLY = ("dynamic_ice", "freeze_adv_vel", "project_adv_vel", "match_barotropic_transport")
-# production tracer/mixing options (reference factory.py:49-51, 649-653)
-FLAGS = ("conservative_kv", "localize_conv", "fct_adv", "monotone_adv")
+OFF_ONLY = ("dynamic_ice", "freeze_adv_vel", "match_barotropic_transport")
+# production tracer/mixing/projection options (reference factory.py:49-53, 649-653)
+FLAGS = ("conservative_kv", "localize_conv", "fct_adv", "monotone_adv", "project_adv_vel")
+PRECONDITIONERS = {"none": 0, "jacobi": 1}
+
+
+def projection_options(dtype, projection_niter=None, projection_rtol=None, projection_preconditioner="none",
+                       projection_max_refinements=2):
+    """Validate and resolve the projection options like the reference factory
+    (``factory.py:368-386``) and ``projection.py:55-62``: the iteration cap
+    falls back to ``OCEAN_PAV_NITER`` and then 150, the CG tolerance and the
+    transport tolerance get a floor of 32 eps of the compute dtype."""
+    if projection_niter is None:
+        env = os.environ.get("OCEAN_PAV_NITER")
+        try:
+            projection_niter = 150 if env is None else int(env)
+        except ValueError as error:
+            raise ValueError("projection_niter environment value must be an integer") from error
+    if (isinstance(projection_niter, (bool, np.bool_)) or not isinstance(projection_niter, (int, np.integer))
+            or projection_niter <= 0):
+        raise ValueError("projection_niter must be a positive integer")
+    if projection_rtol is not None and (not np.isfinite(projection_rtol) or projection_rtol <= 0.0):
+        raise ValueError("projection_rtol must be finite and positive")
+    if projection_preconditioner not in PRECONDITIONERS:
+        raise ValueError("projection_preconditioner must be none or jacobi")
+    if (isinstance(projection_max_refinements, (bool, np.bool_))
+            or not isinstance(projection_max_refinements, (int, np.integer))
+            or not 0 <= projection_max_refinements <= 2):
+        raise ValueError("projection_max_refinements must be an integer from 0 to 2")
+    eps = float(np.finfo(np.float32 if dtype == "float32" else np.float64).eps)
+    return dict(projection_niter=int(projection_niter),
+                projection_rtol=max(float(projection_rtol or 1e-12), 32.0 * eps),
+                projection_preconditioner=projection_preconditioner,
+                projection_max_refinements=int(projection_max_refinements),
+                projection_transport_rtol=max(1e-10, 32.0 * eps))
 
 
 def cap_weights(ncap, ntaper):
@@ -85,6 +118,8 @@ def freeze(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
         raise ValueError("the HIP step implements mode_split=True only (the unsplit path is out of scope)")
     if dtype not in ("float32", "float64"):
         raise ValueError("dtype must be float32 or float64")
+    projection = projection_options(dtype, projection_niter, projection_rtol, projection_preconditioner,
+                                    projection_max_refinements)
     validate_grid(grid)
     if physics.thermodynamics != "linear":
         raise ValueError("only the linear equation of state is supported by the HIP step")
@@ -205,6 +240,8 @@ def freeze(grid, physics, dt, forcing=None, *, T_atm=None, lambda_bulk=0.0,
     p.n_nu = max(1, int(p.n_subcyc if nu_nsub is None else nu_nsub))
     for name in FLAGS:
         setattr(p, name, bool(schemes.get(name, False)))
+    for name, value in projection.items():
+        setattr(p, name, value)
     min_dz = float(np.min(p.dz_3d))
     p.conv_nsub = max(1, int(np.ceil(p.kappa_conv * dt / (0.4 * min_dz ** 2))))
     p.adv_nsub = max(1, int(np.ceil(dt * 4.0e-3 / (0.5 * min_dz))))
@@ -268,4 +305,10 @@ def make_config(p):
     c.localize_conv = int(p.localize_conv)
     c.fct_adv = int(p.fct_adv)
     c.monotone_adv = int(p.monotone_adv)
+    c.project_adv_vel = int(p.project_adv_vel)
+    c.projection_niter = p.projection_niter
+    c.projection_preconditioner = PRECONDITIONERS[p.projection_preconditioner]
+    c.projection_max_refinements = p.projection_max_refinements
+    c.projection_rtol = p.projection_rtol
+    c.projection_transport_rtol = p.projection_transport_rtol
     return c, keep

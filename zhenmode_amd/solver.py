@@ -189,6 +189,27 @@ class Solver:
         _lib.check(rc, self.handle)
         return src._replace(T=To, S=So)
 
+    def project_velocity(self, u, v, dt=None, out=None, iterations=False):
+        """_project_column_divergence(u, v, p, dt) (dynamics/projection.py:64-127):
+        (u, v) minus dt g grad(psi) with the column divergence removed, using
+        the handle's projection settings.  With ``iterations`` also returns
+        the device int32[3] CG iteration counts (first solve, refinements)."""
+        s3, dt_, dev = self.shape3, self.dtype, self.device
+        uu, vv = as_field(u, s3, dt_, dev), as_field(v, s3, dt_, dev)
+        uo = empty3(*s3, dt_, dev) if out is None else out[0]
+        vo = empty3(*s3, dt_, dev) if out is None else out[1]
+        its = torch.zeros(3, dtype=torch.int32, device=dev)
+        rc = self.lib.zm_project_velocity(self.handle, ptr(uu), ptr(vv), ptr(uo), ptr(vo),
+                                          float(self.params.dt if dt is None else dt), ptr(its), self._stream())
+        _lib.check(rc, self.handle)
+        return (uo, vo, its) if iterations else (uo, vo)
+
+    def projection_iterations(self):
+        """CG iterations of the last projection on this handle (synchronises)."""
+        its = torch.zeros(3, dtype=torch.int32, device=self.device)
+        _lib.check(self.lib.zm_projection_iterations(self.handle, ptr(its), self._stream()), self.handle)
+        return [int(x) for x in its.cpu()]
+
     def set_tiles(self, nu_block, seg):
         _lib.check(self.lib.zm_set_tiles(self.handle, int(nu_block), int(seg)), self.handle)
 
