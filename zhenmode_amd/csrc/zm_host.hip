@@ -364,7 +364,7 @@ struct Solver : SolverBase {
     pg.nx = nx;
     pg.ny = ny;
     pg.n2 = (int)n2;
-    pg.nblocks = (int)((n2 + zm::kProjBlock - 1) / zm::kProjBlock);
+    pg.nblocks = (int)std::min<long long>((n2 + zm::kProjBlock - 1) / zm::kProjBlock, zm::kProjMaxBlocks);
     pg.coef = up64(coef);
     pg.inv = up64(inv);
     pg.wet = up64(wet);
@@ -379,7 +379,9 @@ struct Solver : SolverBase {
     pw.p0 = dev_alloc<double>(n2);
     pw.p1 = dev_alloc<double>(n2);
     pw.Ap = dev_alloc<double>(n2);
-    pw.partials = dev_alloc<double>((size_t)4 * pg.nblocks);
+    pw.part_rhs = dev_alloc<double>((size_t)4 * pg.nblocks);
+    pw.part_a = dev_alloc<double>((size_t)pg.nblocks);
+    pw.part_b = dev_alloc<double>((size_t)2 * pg.nblocks);
     pw.sc = dev_alloc<zm::ProjScalars>(1);
     zm::ProjScalars sc{};
     sc.tol2 = c.projection_rtol * c.projection_rtol;
@@ -397,11 +399,11 @@ struct Solver : SolverBase {
   void project(const real* u, const real* v, real* uo, real* vo, double dt, hipStream_t s) {
     const double dtg = dt * zm::kG;
     zm::launch_proj_rhs(g, pg, pw, u, v, dtg, 0, s);
-    zm::launch_proj_cg(pg, pw, proj_niter, 0, s);
+    zm::launch_proj_cg(pg, pw, proj_niter, 0, 0, s);
     zm::launch_proj_correct(g, pg, pw, dtg, u, v, uo, vo, s);
     for (int q = 0; q < proj_maxref; ++q) {
       zm::launch_proj_rhs(g, pg, pw, (const real*)uo, (const real*)vo, dtg, 1, s);
-      zm::launch_proj_cg(pg, pw, proj_niter, q + 1, s);
+      zm::launch_proj_cg(pg, pw, proj_niter, 1, q + 1, s);
       zm::launch_proj_correct(g, pg, pw, dtg, (const real*)uo, (const real*)vo, uo, vo, s);
     }
   }

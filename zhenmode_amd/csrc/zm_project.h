@@ -15,12 +15,13 @@ namespace zm {
 // centre, east, west, east+2, west+2, north, south, north+2, south+2.
 constexpr int kProjTaps = 9;
 constexpr int kProjBlock = 256;
+constexpr int kProjMaxBlocks = 1024;  // columns beyond 256 K are grid-strided
 
 // Solver state kept in device memory between the launches of one projection.
+// Scalars that a launch derives from the previous launch's block partials are
+// written by block 0 only and read by later launches only.
 struct ProjScalars {
-  double gamma;      // r . M r of the current residual
-  double alpha, beta;
-  double rr;         // r . r
+  double gam[2];     // r . M r, double-buffered by iteration parity
   double atol2;      // a solve stops once r . r <= atol2
   double floor2;     // (tol |b_0|)^2, the absolute floor of the refinement solves
   double threshold;  // refinements run while the surface transport norm^2 exceeds it
@@ -29,7 +30,6 @@ struct ProjScalars {
   int done;          // the running solve has stopped
   int active;        // the running solve's correction is applied
   int its[4];        // iterations of the first solve and of each refinement
-  unsigned cnt[3];   // arrival counters of the three reducing kernels
 };
 
 struct ProjGeo {
@@ -47,7 +47,9 @@ struct ProjGeo {
 
 struct ProjWork {
   double *x, *r, *p0, *p1, *Ap;
-  double* partials;  // [nblocks][4]
+  double* part_rhs;  // [nblocks][4]: b.b, b.M b, transport norm^2 of the right-hand side launch
+  double* part_a;    // [nblocks]: p.Ap
+  double* part_b;    // [nblocks][2]: r.M r, r.r
   ProjScalars* sc;
 };
 
@@ -60,8 +62,8 @@ void launch_proj_rhs(const Geo<real>& g, const ProjGeo& pg, const ProjWork& w, c
                      double dtg, int mode, hipStream_t s);
 
 // maxiter iterations of preconditioned CG, each launch a no-op once the
-// solve has stopped.
-void launch_proj_cg(const ProjGeo& pg, const ProjWork& w, int maxiter, int slot, hipStream_t s);
+// solve has stopped; mode as for launch_proj_rhs.
+void launch_proj_cg(const ProjGeo& pg, const ProjWork& w, int maxiter, int mode, int slot, hipStream_t s);
 
 // ud = us - dtg grad_c3d(x), vd likewise, where the solve is active (in place allowed).
 template <typename real>

@@ -4,11 +4,13 @@
 // column with its neighbours at distance one and two along x and along y, so
 // it is held as nine float64 taps per column, built once on the host from the
 // wet levels and the metric.  One CG iteration is two launches: k_proj_A
-// (p = M r + beta p, Ap and p.Ap) and k_proj_B (x, r, r.M r, r.r and the
-// stopping test of jax.scipy.sparse.linalg.cg).  The global sums are formed by
-// the last block to arrive, in a fixed order, so results do not depend on
-// block scheduling; once a solve has stopped the remaining launches return at
-// once, which keeps the launch sequence fixed for hipGraph capture.
+// (stopping test, beta, p = M r + beta p, Ap and the p.Ap partials) and
+// k_proj_B (alpha, x, r and the r.M r, r.r partials).  Each launch folds the
+// previous launch's per-block partials itself, every block in the same fixed
+// order, so every block holds bit-identical scalars, results do not depend on
+// block scheduling, and no fence or atomic is needed inside a launch.  Once a
+// solve has stopped the remaining launches return at once, which keeps the
+// launch sequence fixed for hipGraph capture.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -18,6 +20,7 @@
 
 namespace zm {
 
+// block-wide sums of NV values (result valid in every thread)
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   for (int q = 0; q < NV; ++q)
@@ -27,33 +30,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
   if (lane == 0)
     for (int q = 0; q < NV; ++q) red[w * NV + q] = v[q];
   __syncthreads();
-  if (threadIdx.x == 0)
-    for (int q = 0; q < NV; ++q) v[q] = (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]);
+  for (int q = 0; q < NV; ++q) v[q] = (red[q] + red[NV + q]) + (red[2 * NV + q] + red[3 * NV + q]);
 }
 
-// True in every thread of the last block to arrive; the counter is re-armed.
-// The release fence before the arrival publishes this block's partials and the
-// acquire fence after it makes every other block's partials visible.
-__device__ __forceinline__ bool arrive_last(unsigned* cnt) {
-  __shared__ int last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(cnt, 1u);
-    last = prev == gridDim.x - 1;
-    if (last) atomicExch(cnt, 0u);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last != 0;
-}
-
-// sum over the blocks' partials in a fixed order (valid in thread 0)
+// sums of the previous launch's block partials [nb][stride], fixed order
 template <int NV>
-__device__ __forceinline__ void gather_partials(const double* partials, double (&t)[NV], double* red) {
+__device__ __forceinline__ void fold_partials(const double* __restrict__ part, int stride, int nb, double (&t)[NV],
+                                              double* red) {
   for (int q = 0; q < NV; ++q) t[q] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kProjBlock)
-    for (int q = 0; q < NV; ++q) t[q] += partials[b * 4 + q];
+  for (int b = threadIdx.x; b < nb; b += kProjBlock)
+    for (int q = 0; q < NV; ++q) t[q] += part[b * stride + q];
   block_sum<NV>(t, red);
 }
 
@@ -62,15 +48,18 @@ __device__ __forceinline__ int wrap_x(int i, int nx) {
   return i < 0 ? i + nx : i;
 }
 
+__device__ __forceinline__ int first_column(int nblocks) {
+  return (int)(xcd_tile(blockIdx.x, nblocks) * kProjBlock + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 template <typename real>
 __global__ __launch_bounds__(kProjBlock) void k_proj_rhs(Geo<real> g, ProjGeo pg, ProjWork w,
                                                          const real* __restrict__ u, const real* __restrict__ v,
                                                          double dtg, int mode) {
   __shared__ double red[4 * 3];
-  const int c = (int)(xcd_tile(blockIdx.x, gridDim.x) * kProjBlock + threadIdx.x);
   double val[3] = {0.0, 0.0, 0.0};
-  if (c < pg.n2) {
+  for (int c = first_column(pg.nblocks); c < pg.n2; c += pg.nblocks * kProjBlock) {
     const int j = c / pg.nx, i = c - j * pg.nx, row = j * pg.nx;
     const int e = row + wrap_x(i + 1, pg.nx), wc = row + wrap_x(i - 1, pg.nx);
     const bool hasN = j + 1 < pg.ny, hasS = j > 0;
@@ -93,59 +82,76 @@ __global__ __launch_bounds__(kProjBlock) void k_proj_rhs(Geo<real> g, ProjGeo pg
     const double b = -(cd / dtg) * a;
     w.x[c] = 0.0;
     w.r[c] = b;
-    val[0] = b * b;
-    val[1] = b * (b * pg.inv[c]);
-    val[2] = cd * cd * a;
+    val[0] += b * b;
+    val[1] += b * (b * pg.inv[c]);
+    val[2] += cd * cd * a;
   }
   block_sum<3>(val, red);
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) w.partials[blockIdx.x * 4 + q] = val[q];
-  ProjScalars* sc = w.sc;
-  if (arrive_last(&sc->cnt[0])) {
-    double t[3];
-    gather_partials<3>(w.partials, t, red);
-    if (threadIdx.x == 0) {
-      const double bb = t[0], rz = t[1], tn = t[2];
-      if (mode == 0) {
-        sc->floor2 = sc->tol2 * bb;
-        sc->threshold = sc->ttol2 * tn;
-        sc->active = 1;
-        for (int q = 0; q < 4; ++q) sc->its[q] = 0;
-      } else {
-        sc->active = tn > sc->threshold ? 1 : 0;
-      }
-      if (sc->active) {
-        const double atol2 = mode == 0 ? sc->tol2 * bb : fmax(sc->tol2 * bb, sc->floor2);
-        sc->gamma = rz;
-        sc->rr = bb;
-        sc->atol2 = atol2;
-        sc->done = (bb > atol2 && sc->maxiter > 0) ? 0 : 1;
-      } else {
-        sc->done = 1;
-      }
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 3; ++q) w.part_rhs[blockIdx.x * 4 + q] = val[q];
+    if (blockIdx.x == 0) {
+      w.sc->done = 0;
+      if (mode == 0)
+        for (int q = 0; q < 4; ++q) w.sc->its[q] = 0;
     }
   }
 }
 
-// p_new = M r + beta p_old (beta = 0 on the first iteration), Ap, p.Ap -> alpha
-__global__ __launch_bounds__(kProjBlock) void k_proj_A(ProjGeo pg, ProjWork w, int it,
+// stopping test and beta from the previous launch, then p_new = M r + beta
+// p_old (beta = 0 on the first iteration), Ap and the p.Ap partials
+__global__ __launch_bounds__(kProjBlock) void k_proj_A(ProjGeo pg, ProjWork w, int it, int mode,
                                                        const double* __restrict__ po, double* __restrict__ pn) {
-  __shared__ double red[4];
+  __shared__ double red[4 * 3];
   ProjScalars* sc = w.sc;
   if (sc->done) return;
-  const double beta = it == 0 ? 0.0 : sc->beta;
-  const int c = (int)(xcd_tile(blockIdx.x, gridDim.x) * kProjBlock + threadIdx.x);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double beta = 0.0;
+  bool stop;
+  if (it == 0) {
+    double t[3];  // b.b, b.M b, transport norm^2
+    fold_partials<3>(w.part_rhs, 4, pg.nblocks, t, red);
+    const double bb = t[0];
+    bool active;
+    double atol2;
+    if (mode == 0) {
+      active = true;
+      atol2 = sc->tol2 * bb;
+      if (lead) {
+        sc->floor2 = sc->tol2 * bb;
+        sc->threshold = sc->ttol2 * t[2];
+      }
+    } else {
+      active = t[2] > sc->threshold;
+      atol2 = fmax(sc->tol2 * bb, sc->floor2);
+    }
+    stop = !active || !(bb > atol2 && sc->maxiter > 0);
+    if (lead) {
+      sc->active = active ? 1 : 0;
+      sc->atol2 = atol2;
+      sc->gam[0] = t[1];
+    }
+  } else {
+    double t[2];  // r.M r, r.r after the previous iteration
+    fold_partials<2>(w.part_b, 2, pg.nblocks, t, red);
+    stop = !(t[1] > sc->atol2 && it < sc->maxiter);
+    beta = t[0] / sc->gam[(it - 1) & 1];
+    if (lead) sc->gam[it & 1] = t[0];
+  }
+  if (stop) {
+    if (lead) sc->done = 1;
+    return;
+  }
   double val[1] = {0.0};
-  if (c < pg.n2) {
+  const double* __restrict__ r = w.r;
+  const double* __restrict__ inv = pg.inv;
+  const double* __restrict__ C = pg.coef;
+  const long long n2 = pg.n2;
+  for (int c = first_column(pg.nblocks); c < pg.n2; c += pg.nblocks * kProjBlock) {
     const int j = c / pg.nx, i = c - j * pg.nx, row = j * pg.nx;
-    const double* __restrict__ r = w.r;
-    const double* __restrict__ inv = pg.inv;
     auto P = [&](int q) {
       const double z = r[q] * inv[q];
       return it == 0 ? z : z + beta * po[q];
     };
-    const long long n2 = pg.n2;
-    const double* __restrict__ C = pg.coef;
     const int n = j + 1 < pg.ny ? c + pg.nx : c, nn = j + 2 < pg.ny ? c + 2 * pg.nx : c;
     const int s = j > 0 ? c - pg.nx : c, ss = j > 1 ? c - 2 * pg.nx : c;
     const double pc = P(c);
@@ -160,49 +166,34 @@ __global__ __launch_bounds__(kProjBlock) void k_proj_A(ProjGeo pg, ProjWork w, i
     acc += C[8 * n2 + c] * P(ss);
     pn[c] = pc;
     w.Ap[c] = acc;
-    val[0] = pc * acc;
+    val[0] += pc * acc;
   }
   block_sum<1>(val, red);
-  if (threadIdx.x == 0) w.partials[blockIdx.x * 4] = val[0];
-  if (arrive_last(&sc->cnt[1])) {
-    double t[1];
-    gather_partials<1>(w.partials, t, red);
-    if (threadIdx.x == 0) sc->alpha = sc->gamma / t[0];
-  }
+  if (threadIdx.x == 0) w.part_a[blockIdx.x] = val[0];
 }
 
-// x += alpha p, r -= alpha Ap, r.M r and r.r -> beta, stopping test
+// alpha from the p.Ap partials, x += alpha p, r -= alpha Ap and the r.M r, r.r partials
 __global__ __launch_bounds__(kProjBlock) void k_proj_B(ProjGeo pg, ProjWork w, int it, int slot,
                                                        const double* __restrict__ pn) {
   __shared__ double red[4 * 2];
   ProjScalars* sc = w.sc;
   if (sc->done) return;
-  const double alpha = sc->alpha;
-  const int c = (int)(xcd_tile(blockIdx.x, gridDim.x) * kProjBlock + threadIdx.x);
+  double t[1];
+  fold_partials<1>(w.part_a, 1, pg.nblocks, t, red);
+  const double alpha = sc->gam[it & 1] / t[0];
   double val[2] = {0.0, 0.0};
-  if (c < pg.n2) {
+  for (int c = first_column(pg.nblocks); c < pg.n2; c += pg.nblocks * kProjBlock) {
     w.x[c] = w.x[c] + alpha * pn[c];
     const double rn = w.r[c] - alpha * w.Ap[c];
     w.r[c] = rn;
-    val[0] = rn * (rn * pg.inv[c]);
-    val[1] = rn * rn;
+    val[0] += rn * (rn * pg.inv[c]);
+    val[1] += rn * rn;
   }
   block_sum<2>(val, red);
   if (threadIdx.x == 0) {
-    w.partials[blockIdx.x * 4 + 0] = val[0];
-    w.partials[blockIdx.x * 4 + 1] = val[1];
-  }
-  if (arrive_last(&sc->cnt[2])) {
-    double t[2];
-    gather_partials<2>(w.partials, t, red);
-    if (threadIdx.x == 0) {
-      const double gn = t[0], rr = t[1];
-      sc->beta = gn / sc->gamma;
-      sc->gamma = gn;
-      sc->rr = rr;
-      sc->its[slot] = it + 1;
-      sc->done = (rr > sc->atol2 && it + 1 < sc->maxiter) ? 0 : 1;
-    }
+    w.part_b[blockIdx.x * 2 + 0] = val[0];
+    w.part_b[blockIdx.x * 2 + 1] = val[1];
+    if (blockIdx.x == 0) sc->its[slot] = it + 1;
   }
 }
 
@@ -241,11 +232,11 @@ void launch_proj_rhs(const Geo<real>& g, const ProjGeo& pg, const ProjWork& w, c
   hipLaunchKernelGGL(k_proj_rhs<real>, dim3(pg.nblocks), dim3(kProjBlock), 0, s, g, pg, w, u, v, dtg, mode);
 }
 
-void launch_proj_cg(const ProjGeo& pg, const ProjWork& w, int maxiter, int slot, hipStream_t s) {
+void launch_proj_cg(const ProjGeo& pg, const ProjWork& w, int maxiter, int mode, int slot, hipStream_t s) {
   for (int it = 0; it < maxiter; ++it) {
     const double* po = (it & 1) ? w.p1 : w.p0;
     double* pn = (it & 1) ? w.p0 : w.p1;
-    hipLaunchKernelGGL(k_proj_A, dim3(pg.nblocks), dim3(kProjBlock), 0, s, pg, w, it, po, pn);
+    hipLaunchKernelGGL(k_proj_A, dim3(pg.nblocks), dim3(kProjBlock), 0, s, pg, w, it, mode, po, pn);
     hipLaunchKernelGGL(k_proj_B, dim3(pg.nblocks), dim3(kProjBlock), 0, s, pg, w, it, slot, (const double*)pn);
   }
 }
