@@ -140,6 +140,42 @@ def learned_leg(case, plain_ms, steps, warmup):
                     "convection of the plain step by the implicit mixing operator's surrogate"}
 
 
+def projection_leg(case, plain_ms, tiles, steps, warmup, dtype):
+    """The same step with project_adv_vel (reference defaults: 150 CG
+    iterations per solve, up to two transport-checked refinements): time per
+    step, CG iterations, and the algorithmic bytes of one CG iteration
+    (k_proj_A: nine float64 taps, r, M, p_old read, p_new and Ap written = 112
+    B per column; k_proj_B: x, p, r, Ap, M read, x and r written = 56 B)."""
+    import torch
+
+    from zhenmode_amd.solver import make_solver_global
+
+    step, init_state, _ = make_solver_global(case.grid, case.physics, case.dt, dtype=dtype, project_adv_vel=True,
+                                             **case.solver_kwargs)
+    sol = step.solver
+    sol.set_tiles(*tiles[:2])
+    a = init_state(case.T_init, case.S_init)
+    b = sol.new_state()
+    for _ in range(warmup):
+        sol.step(a, out=b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sol.step(a, out=b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    its = sol.projection_iterations()
+    n2 = case.grid.nx * case.grid.ny
+    per_it_us = 1e3 * (ms - plain_ms) / max(sum(its), 1)
+    return {"ms_per_step": ms, "plain_ms_per_step": plain_ms, "cg_iterations_last_step": its,
+            "us_per_cg_iteration": per_it_us, "cg_bytes_per_iteration": 168 * n2,
+            "cg_GBps": 168 * n2 / (per_it_us * 1e-6) / 1e9,
+            "sypd": case.dt / (365.0 * ms * 1e-3),
+            "finite": bool(torch.isfinite(a.T).all())}
+
+
 def ensemble_leg(steps, warmup):
     """24 uneven members (8 x 180x80x30, 8 x 360x160x50, 8 x 480x214x40) in one
     hipGraph launch per ensemble step, tiles autotuned per member size."""
@@ -201,6 +237,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-projection", action="store_true")
     ap.add_argument("--tiles", default="", help="nu_block,seg: fixed launch shapes instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -319,6 +356,9 @@ def main():
     }
     if not args.no_learned:
         out["learned_mixed"] = learned_leg(case, ms_per_step, min(args.steps, 20), 3)
+    if not args.no_projection:
+        out["projected_step"] = projection_leg(case, ms_per_step, solver.get_tiles(), min(args.steps, 10), 2,
+                                               args.dtype)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(min(args.steps, 10), 2)
     if rank == 0 and not args.no_cpu_baseline:

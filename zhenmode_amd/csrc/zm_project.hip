@@ -97,6 +97,34 @@ __global__ __launch_bounds__(kProjBlock) void k_proj_rhs(Geo<real> g, ProjGeo pg
   }
 }
 
+// One column's operands of k_proj_A: the nine taps and M r, p_old at the
+// nine stencil points.  Loaded before the launch folds the previous partials,
+// so the loads overlap that fold.
+struct ColA {
+  double C[kProjTaps], z[kProjTaps], p[kProjTaps];
+};
+
+__device__ __forceinline__ void load_col_a(const ProjGeo& pg, const double* __restrict__ r,
+                                           const double* __restrict__ po, bool first, int c, ColA& a) {
+  const int j = c / pg.nx, i = c - j * pg.nx, row = j * pg.nx;
+  const int q[kProjTaps] = {c,
+                            row + wrap_x(i + 1, pg.nx),
+                            row + wrap_x(i - 1, pg.nx),
+                            row + wrap_x(i + 2, pg.nx),
+                            row + wrap_x(i - 2, pg.nx),
+                            j + 1 < pg.ny ? c + pg.nx : c,
+                            j > 0 ? c - pg.nx : c,
+                            j + 2 < pg.ny ? c + 2 * pg.nx : c,
+                            j > 1 ? c - 2 * pg.nx : c};
+  const long long n2 = pg.n2;
+#pragma unroll
+  for (int t = 0; t < kProjTaps; ++t) {
+    a.C[t] = pg.coef[t * n2 + c];
+    a.z[t] = r[q[t]] * pg.inv[q[t]];
+    a.p[t] = first ? 0.0 : po[q[t]];
+  }
+}
+
 // stopping test and beta from the previous launch, then p_new = M r + beta
 // p_old (beta = 0 on the first iteration), Ap and the p.Ap partials
 __global__ __launch_bounds__(kProjBlock) void k_proj_A(ProjGeo pg, ProjWork w, int it, int mode,
@@ -105,6 +133,9 @@ __global__ __launch_bounds__(kProjBlock) void k_proj_A(ProjGeo pg, ProjWork w, i
   ProjScalars* sc = w.sc;
   if (sc->done) return;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int c0 = first_column(pg.nblocks), stride = pg.nblocks * kProjBlock;
+  ColA a;
+  if (c0 < pg.n2) load_col_a(pg, w.r, po, it == 0, c0, a);
   double beta = 0.0;
   bool stop;
   if (it == 0) {
@@ -142,31 +173,17 @@ __global__ __launch_bounds__(kProjBlock) void k_proj_A(ProjGeo pg, ProjWork w, i
     return;
   }
   double val[1] = {0.0};
-  const double* __restrict__ r = w.r;
-  const double* __restrict__ inv = pg.inv;
-  const double* __restrict__ C = pg.coef;
-  const long long n2 = pg.n2;
-  for (int c = first_column(pg.nblocks); c < pg.n2; c += pg.nblocks * kProjBlock) {
-    const int j = c / pg.nx, i = c - j * pg.nx, row = j * pg.nx;
-    auto P = [&](int q) {
-      const double z = r[q] * inv[q];
-      return it == 0 ? z : z + beta * po[q];
-    };
-    const int n = j + 1 < pg.ny ? c + pg.nx : c, nn = j + 2 < pg.ny ? c + 2 * pg.nx : c;
-    const int s = j > 0 ? c - pg.nx : c, ss = j > 1 ? c - 2 * pg.nx : c;
-    const double pc = P(c);
-    double acc = C[c] * pc;
-    acc += C[n2 + c] * P(row + wrap_x(i + 1, pg.nx));
-    acc += C[2 * n2 + c] * P(row + wrap_x(i - 1, pg.nx));
-    acc += C[3 * n2 + c] * P(row + wrap_x(i + 2, pg.nx));
-    acc += C[4 * n2 + c] * P(row + wrap_x(i - 2, pg.nx));
-    acc += C[5 * n2 + c] * P(n);
-    acc += C[6 * n2 + c] * P(s);
-    acc += C[7 * n2 + c] * P(nn);
-    acc += C[8 * n2 + c] * P(ss);
-    pn[c] = pc;
+  for (int c = c0; c < pg.n2; c += stride) {
+    if (c != c0) load_col_a(pg, w.r, po, it == 0, c, a);
+    double pv[kProjTaps];
+#pragma unroll
+    for (int t = 0; t < kProjTaps; ++t) pv[t] = it == 0 ? a.z[t] : a.z[t] + beta * a.p[t];
+    double acc = a.C[0] * pv[0];
+#pragma unroll
+    for (int t = 1; t < kProjTaps; ++t) acc += a.C[t] * pv[t];
+    pn[c] = pv[0];
     w.Ap[c] = acc;
-    val[0] += pc * acc;
+    val[0] += pv[0] * acc;
   }
   block_sum<1>(val, red);
   if (threadIdx.x == 0) w.part_a[blockIdx.x] = val[0];
@@ -178,15 +195,31 @@ __global__ __launch_bounds__(kProjBlock) void k_proj_B(ProjGeo pg, ProjWork w, i
   __shared__ double red[4 * 2];
   ProjScalars* sc = w.sc;
   if (sc->done) return;
+  const int c0 = first_column(pg.nblocks), stride = pg.nblocks * kProjBlock;
+  double xv = 0.0, pv = 0.0, rv = 0.0, av = 0.0, mv = 0.0;
+  if (c0 < pg.n2) {
+    xv = w.x[c0];
+    pv = pn[c0];
+    rv = w.r[c0];
+    av = w.Ap[c0];
+    mv = pg.inv[c0];
+  }
   double t[1];
   fold_partials<1>(w.part_a, 1, pg.nblocks, t, red);
   const double alpha = sc->gam[it & 1] / t[0];
   double val[2] = {0.0, 0.0};
-  for (int c = first_column(pg.nblocks); c < pg.n2; c += pg.nblocks * kProjBlock) {
-    w.x[c] = w.x[c] + alpha * pn[c];
-    const double rn = w.r[c] - alpha * w.Ap[c];
+  for (int c = c0; c < pg.n2; c += stride) {
+    if (c != c0) {
+      xv = w.x[c];
+      pv = pn[c];
+      rv = w.r[c];
+      av = w.Ap[c];
+      mv = pg.inv[c];
+    }
+    w.x[c] = xv + alpha * pv;
+    const double rn = rv - alpha * av;
     w.r[c] = rn;
-    val[0] += rn * (rn * pg.inv[c]);
+    val[0] += rn * (rn * mv);
     val[1] += rn * rn;
   }
   block_sum<2>(val, red);
