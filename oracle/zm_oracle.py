@@ -30,6 +30,14 @@ How each part is pinned (checked by ``tests/test_oracle_*.py``):
     conservation on closed basins.
   * whole step: a resting uniform ocean is an exact fixed point; volume is
     conserved to round-off.
+  * production flags (conservative_kv, localize_conv, fct_adv,
+    monotone_adv): the reference's FCT/monotone assertions and the flux-form
+    telescoping identity (``tests/test_oracle_flags.py``).
+  * column-divergence projection (project_adv_vel): operator symmetry,
+    positive semi-definiteness and null constants, the stated diagonal against
+    the brute-force diagonal (a planted dropped term fails), divergence
+    removal to 1e-9, PCG against a dense solve, the resting-ocean fixed point
+    (``tests/test_oracle_projection.py``).
   Parity unpinned: the bit-level agreement of the composite step with the
   reference JAX executable (XLA's FFT, cumsum association and transcendental
   rounding cannot be reproduced without importing JAX, which is absent).
