@@ -257,6 +257,46 @@ int zm_monitor(zm_handle* h, const zm_state* state, int64_t step, double velocit
  * wet columns, 0. */
 int zm_budget(zm_handle* h, const zm_state* state, double* sums, void* stream);
 
+/* ---- shadow-step stage ledger --------------------------------------------
+ * Replaces make_budget_step(params) (diagnostics/budgets.py:244-270) and its
+ * _StageRecorder (:69-225): one step, identical to zm_step_dyn, that also
+ * writes the float64 audit of the step into the device array ledger
+ * [ZM_LEDGER_SIZE].  Metrics are (heat J, salt kg, displacement m3); stages
+ * are (linear_diffusion, sponge, nonlinear, free_surface, polar_cap_and_masks,
+ * dynamic_ice); sources are (prescribed_heat, bulk_heat, coastal_bulk_heat,
+ * temperature_restore, salinity_restore, prescribed_brine, sponge,
+ * ice_atmosphere_heat, ice_brine); processes are (advection, convection, gm,
+ * redi).  Entries the legacy mode-split step never books (sponge, dynamic
+ * ice, GM/Redi, split bottom drag, transport consistency) are zero.
+ * tau_x / tau_y / q_heat / t_atm may be NULL for the handle's frozen forcing.
+ * out must not alias in. */
+#define ZM_LEDGER_OBSERVED 0                          /* [3] */
+#define ZM_LEDGER_STAGES 3                            /* [6][3] */
+#define ZM_LEDGER_SOURCES 21                          /* [9][3] */
+#define ZM_LEDGER_DECOMPOSITION 48                    /* [3] observed - sum(stages) */
+#define ZM_LEDGER_BUDGET 51                           /* [3] observed - sum(sources) */
+#define ZM_LEDGER_ABS_DECOMPOSITION 54                /* [3] */
+#define ZM_LEDGER_ABS_BUDGET 57                       /* [3] */
+#define ZM_LEDGER_CHANGE_SCALE 60                     /* [3] sum of |stage changes| */
+#define ZM_LEDGER_PROCESSES 63                        /* [4][3] */
+#define ZM_LEDGER_PROCESS_SCALE 75                    /* [3] */
+#define ZM_LEDGER_ADVECTION_BOUNDARY 78               /* [3] surface advective flux */
+#define ZM_LEDGER_ADVECTION_BOUNDARY_RESIDUAL 81      /* [3] advection - boundary */
+#define ZM_LEDGER_ABS_ADVECTION_BOUNDARY_RESIDUAL 84  /* [3] */
+#define ZM_LEDGER_NONLINEAR_RESIDUAL 87               /* [3] nonlinear stage - sources[:6] - processes */
+#define ZM_LEDGER_ABS_NONLINEAR_RESIDUAL 90           /* [3] */
+#define ZM_LEDGER_SURFACE_DISPLACEMENT 93             /* [3] eta * surface tracer change */
+#define ZM_LEDGER_PROJECTION_NORMS 96                 /* [2] surface transport norm^2 before / after */
+#define ZM_LEDGER_PROJECTION_MAX 98                   /* worst sqrt(after / before) */
+#define ZM_LEDGER_TRANSPORT_CONSISTENCY 99            /* [4] */
+#define ZM_LEDGER_TRANSPORT_AUDITED 103
+#define ZM_LEDGER_DRAG_ENERGY 104
+#define ZM_LEDGER_DRAG_FACE_CHANGE 105
+#define ZM_LEDGER_DRAG_HALVES 106
+#define ZM_LEDGER_SIZE 107
+int zm_budget_step(zm_handle* h, const zm_state* in, const zm_state* out, const void* tau_x, const void* tau_y,
+                   const void* q_heat, const void* t_atm, double* ledger, void* stream);
+
 /* ---- column-divergence projection ----------------------------------------
  * Replaces _project_column_divergence(u, v, p, dt) (dynamics/projection.py:64-127):
  * (u_out, v_out) = (u, v) - dt g grad_c3d(psi) with psi the preconditioned CG

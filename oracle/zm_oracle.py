@@ -681,8 +681,10 @@ def bt_rho_pgf(T, S, p):
 # --------------------------------------------------------------------------
 # tendencies (reference dynamics/tendencies.py)
 # --------------------------------------------------------------------------
-def tracer_tendency(u, v, T, S, p):
-    """dT/dt, dS/dt, legacy defaults (tendencies.py:120-245)."""
+def tracer_tendency(u, v, T, S, p, rec=None):
+    """dT/dt, dS/dt, legacy defaults (tendencies.py:120-245); ``rec`` receives
+    the surface sources, process tendencies and surface advective fluxes
+    (tendencies.py:211-214)."""
     wm = p.wet_mask_z
     Fz = vertical_transport_iface(u, v, p)
     n_a = int(p.adv_nsub)
@@ -732,12 +734,17 @@ def tracer_tendency(u, v, T, S, p):
 
     dTdt = (adv_T + diff_h_T + diff_v_T + heat_T + bulk_T + cbulk_T + conv_T + rest_T) * wm
     dSdt = (adv_S + diff_h_S + diff_v_S + conv_S + rest_S + ice_salt) * wm
+    if rec is not None:
+        zero = np.zeros_like(T)
+        rec.surface_sources((heat_T, bulk_T, cbulk_T, rest_T, rest_S, ice_salt))
+        rec.nonlinear_terms(((adv_T, adv_S), (conv_T, conv_S), (zero, zero), (zero, zero)))
+        rec.advection_boundary_fluxes(Fz[..., 0] * T[..., 0], Fz[..., 0] * S[..., 0])
     return dTdt, dSdt, (diff_h_T, diff_h_S, diff_v_T, diff_v_S)
 
 
-def tracer_residual(u, v, T, S, p):
+def tracer_residual(u, v, T, S, p, rec=None):
     """Tendency minus the L-step's linear diffusion, add-then-subtract (tendencies.py:253-271)."""
-    dTdt, dSdt, (dhT, dhS, dvT, dvS) = tracer_tendency(u, v, T, S, p)
+    dTdt, dSdt, (dhT, dhS, dvT, dvS) = tracer_tendency(u, v, T, S, p, rec=rec)
     dTdt = dTdt - dhT
     dSdt = dSdt - dhS
     dTdt = dTdt - dvT
@@ -805,8 +812,9 @@ def nu_subcycle(u, v, p, dt_half):
     return u, v
 
 
-def linear_half_step(state, p, dt_half):
-    """Mode-split linear half-step (step.py:76-181)."""
+def linear_half_step(state, p, dt_half, rec=None):
+    """Mode-split linear half-step (step.py:76-181); ``rec`` records the
+    linear_diffusion and sponge stages (step.py:138-161)."""
     u, v = nu_subcycle(state.u, state.v, p, dt_half)
     T = state.T + tracer_diffusion(state.T, p) * dt_half
     S = state.S + tracer_diffusion(state.S, p) * dt_half
@@ -825,19 +833,25 @@ def linear_half_step(state, p, dt_half):
     v = v * wm + state.v * (1.0 - wm)
     T = T * wm + state.T * (1.0 - wm)
     S = S * wm + state.S * (1.0 - wm)
+    if rec is not None:
+        diffusion_state = State(u, v, T, S, state.eta, state.ice)
+        rec.stage("linear_diffusion", state, diffusion_state)
     decay = np.exp(-p.sponge_rate * dt_half)
     u = u * decay
     v = v * decay
     T = p.T_clim_3d + (T - p.T_clim_3d) * decay
     S = p.S_clim_3d + (S - p.S_clim_3d) * decay
+    if rec is not None:
+        rec.stage("sponge", diffusion_state, State(u, v, T, S, state.eta, state.ice))
+        rec.sponge_sources(diffusion_state, decay)
     u, v = coriolis_rotation(u, v, p.f, dt_half)
     return State(u, v, T, S, state.eta, state.ice)
 
 
-def explicit_full_step(state, p, dt):
+def explicit_full_step(state, p, dt, rec=None):
     """Legacy tracer-first RK2 with implicit bottom-drag strip (step.py:249-320)."""
     u, v, T, S, eta = state.u, state.v, state.T, state.S, state.eta
-    dT1, dS1 = tracer_residual(u, v, T, S, p)
+    dT1, dS1 = tracer_residual(u, v, T, S, p, rec=rec)
     T_pred = T + dT1 * dt
     S_pred = S + dS1 * dt
     du1, dv1 = momentum_residual(u, v, T_pred, S_pred, eta, p)
@@ -848,9 +862,11 @@ def explicit_full_step(state, p, dt):
     v_pred = v + dv1 * dt
     if getattr(p, "project_adv_vel", False):
         u_adv, v_adv = project_column_divergence(u_pred, v_pred, p, dt)
+        if rec is not None:
+            rec.column_projection(u_pred, v_pred, u_adv, v_adv)
     else:
         u_adv, v_adv = u_pred, v_pred
-    dT2, dS2 = tracer_residual(u_adv, v_adv, T_pred, S_pred, p)
+    dT2, dS2 = tracer_residual(u_adv, v_adv, T_pred, S_pred, p, rec=rec)
     T_new = T + 0.5 * (dT1 + dT2) * dt
     S_new = S + 0.5 * (dS1 + dS2) * dt
     du2, dv2 = momentum_residual(u, v, T_new, S_new, eta, p)

@@ -12,6 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libzhenmode_hip.so"
 
 ZM_OK, ZM_EINVAL, ZM_EHIP, ZM_ENOMEM = 0, 1, 2, 3
 STAGE_LINEAR, STAGE_NONLINEAR, STAGE_BAROTROPIC, STAGE_TRACER_RES, STAGE_MOMENTUM_RES = range(5)
+LEDGER_SIZE = 107  # ZM_LEDGER_SIZE: float64 entries written by zm_budget_step
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -67,7 +68,7 @@ EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", 
            "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
            "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
            "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget",
-           "zm_project_velocity", "zm_projection_iterations", "zm_projection_operator")
+           "zm_project_velocity", "zm_projection_iterations", "zm_projection_operator", "zm_budget_step")
 
 # epilogues of zm_gemm_bf16_nt
 EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
@@ -123,8 +124,9 @@ def load():
     lib.zm_project_velocity.argtypes = [vp, vp, vp, vp, vp, f64, vp, vp]
     lib.zm_projection_iterations.argtypes = [vp, vp, vp]
     lib.zm_projection_operator.argtypes = [ctypes.POINTER(ZmConfig), vp, vp]
+    lib.zm_budget_step.argtypes = [vp, ctypes.POINTER(ZmState), ctypes.POINTER(ZmState), vp, vp, vp, vp, vp, vp]
     for name in ("zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget", "zm_project_velocity",
-                 "zm_projection_iterations", "zm_projection_operator"):
+                 "zm_projection_iterations", "zm_projection_operator", "zm_budget_step"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("zm_vertical_mix", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
                  "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",

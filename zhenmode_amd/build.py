@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libzhenmode_hip.so"
-SOURCES = ["zm_kernels.hip", "zm_column.hip", "zm_runtime.hip", "zm_project.hip", "zm_host.hip"]
+SOURCES = ["zm_kernels.hip", "zm_column.hip", "zm_runtime.hip", "zm_project.hip", "zm_ledger.hip", "zm_host.hip"]
 ARCH = os.environ.get("ZHENMODE_OFFLOAD_ARCH", "gfx950")
 
 

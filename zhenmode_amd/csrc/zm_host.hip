@@ -20,6 +20,7 @@
 #include "zm_common.h"
 #include "zm_column.h"
 #include "zm_kernels.h"
+#include "zm_ledger.h"
 #include "zm_project.h"
 #include "zm_runtime.h"
 
@@ -68,6 +69,8 @@ struct SolverBase {
   virtual void project_velocity(const void* u, const void* v, void* uo, void* vo, double dt, int* its,
                                 hipStream_t s) = 0;
   virtual void projection_iterations(int* its, hipStream_t s) = 0;
+  virtual void budget_step(const zm_state& in, const zm_state& out, const void* const* forcing, double* ledger,
+                           hipStream_t s) = 0;
   virtual void set_tiles(int nu_block, int seg) = 0;
   virtual void get_tiles(int* nu_block, int* seg) const = 0;
   int nz_ = 0;
@@ -101,6 +104,10 @@ struct Solver : SolverBase {
   zm::ProjGeo pg{};
   zm::ProjWork pw{};
   real *ua = nullptr, *va = nullptr;  // projected advecting velocity
+  // stage ledger (allocated on the first budget step)
+  bool ledger_on = false;
+  double *led_raw = nullptr, *led_part = nullptr;
+  size_t led_part_n = 0;
   // workspace
   real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
   real *Fz1, *Fz2, *cT, *cS;
@@ -339,6 +346,65 @@ struct Solver : SolverBase {
     ZM_HIP(hipDeviceSynchronize());
   }
 
+  // ledger sums of one tracer residual evaluation, booked over dt/2
+  // (tendencies.py:211-214 with the recorder's default interval)
+  void ledger_tracer(const zm::Phys<real>& Pf, const real* u, const real* v, const real* T, const real* S,
+                     const real* Fz, hipStream_t s) {
+    zm::launch_tracer_ledger(g, Pf, u, v, T, S, Fz, (const real*)cT, (const real*)cS, (const int*)gate, area64,
+                             dzn64, led_part, s);
+    zm::launch_ledger_tracer_fold(led_part, (int)zm::tracer_ledger_blocks(g), dt64 / 2.0, led_raw, s);
+  }
+
+  // make_budget_step (diagnostics/budgets.py:244-270): the step with the
+  // recorder's stage differences (step.py:138-161, 365-368, 426-427, 452-456)
+  void budget_step(const zm_state& in, const zm_state& out, const void* const* forcing, double* ledger,
+                   hipStream_t s) override {
+    const void* ins[] = {in.u, in.v, in.T, in.S, in.eta, in.ice};
+    const void* outs[] = {out.u, out.v, out.T, out.S, out.eta, out.ice};
+    for (const void* a : ins)
+      for (const void* b : outs)
+        if (a == b) throw ConfigError{"zm_budget_step output must not alias its input"};
+    const size_t need = std::max<size_t>({(size_t)zm::tracer_ledger_blocks(g) * zm::kLedgerTracerSums,
+                                          (size_t)zm::kLedgerBlocks * zm::kLedgerDiffSums,
+                                          (size_t)zm::kLedgerBlocks * 2});
+    if (!led_raw) led_raw = dev_alloc<double>(zm::kRawSize);
+    if (led_part_n < need) {
+      led_part = dev_alloc<double>(need);
+      led_part_n = need;
+    }
+    const zm::Phys<real> Pf = forced(forcing);
+    const real* Ti = (const real*)in.T;
+    const real* Si = (const real*)in.S;
+    const real* ei = (const real*)in.eta;
+    const real* ci = (const real*)in.ice;
+    auto diff = [&](const real* Ta, const real* Sa, const real* ea, const real* Tb, const real* Sb,
+                    const real* eb, int slot) {
+      zm::launch_ledger_diff(g, area64, dzn64, Ta, Sa, ea, ci, Tb, Sb, eb, ci, slot, led_part, led_raw, s);
+    };
+    zm::launch_ledger_reset(led_raw, s);
+    linear((const real*)in.u, (const real*)in.v, Ti, Si, u1, v1, T1, S1, s);
+    diff(T1, S1, ei, Ti, Si, ei, zm::kStageLinearDiffusion);
+    ledger_on = true;
+    try {
+      nonlinear(Pf, ei, u2, v2, T2, S2, s);
+    } catch (...) {
+      ledger_on = false;
+      throw;
+    }
+    ledger_on = false;
+    diff(T2, S2, ei, T1, S1, ei, zm::kStageNonlinear);
+    linear(u2, v2, T2, S2, u3, v3, T3, S3, s);
+    diff(T3, S3, ei, T2, S2, ei, zm::kStageLinearDiffusion);
+    barotropic(Pf, ei, s);
+    diff(T3, S3, etaw, T3, S3, ei, zm::kStageFreeSurface);
+    post_barotropic(in, out, Pf, s);
+    diff((const real*)out.T, (const real*)out.S, (const real*)out.eta, T3, S3, etaw, zm::kStagePolarCap);
+    zm::launch_ledger_diff(g, area64, dzn64, (const real*)out.T, (const real*)out.S, (const real*)out.eta,
+                           (const real*)out.ice, Ti, Si, ei, ci, -1, led_part, led_raw, s);
+    zm::launch_ledger_finish(led_raw, ledger, s);
+    ZM_HIP(hipGetLastError());
+  }
+
   // operator taps, Jacobi scaling and solver buffers of the projection
   void setup_projection(const zm_config& c) {
     proj_on = c.project_adv_vel != 0;
@@ -461,6 +527,7 @@ struct Solver : SolverBase {
     zm::launch_fz(g, u1, v1, Fz1, s);
     zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, s);
     zm::launch_tracer_residual(g, Pf, 0, u1, v1, T1, S1, Fz1, cT, cS, gate, dT1, dS1, T1, S1, Tp, Sp, s);
+    if (ledger_on) ledger_tracer(Pf, u1, v1, T1, S1, Fz1, s);
     // momentum residual #1 at (u1, v1, T_pred, S_pred)
     zm::launch_pbc(g, Pf, Tp, Sp, pbc, s);
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
@@ -478,8 +545,14 @@ struct Solver : SolverBase {
       vad = va;
     }
     zm::launch_fz(g, uad, vad, Fz2, s);
+    if (ledger_on && proj_on) {
+      // surface transports of the predictor (into the free Fz1) and of its projection
+      zm::launch_fz(g, up, vp, Fz1, s);
+      zm::launch_ledger_transport(g, area64, (const real*)Fz1, (const real*)Fz2, led_part, led_raw, s);
+    }
     zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, s);
     zm::launch_tracer_residual(g, Pf, 1, uad, vad, Tp, Sp, Fz2, cT, cS, gate, dT1, dS1, T1, S1, To, So, s);
+    if (ledger_on) ledger_tracer(Pf, uad, vad, Tp, Sp, Fz2, s);
     // momentum residual #2 at (u1, v1, T_new, S_new); advection of (u1, v1) is reused
     zm::launch_pbc(g, Pf, To, So, pbc, s);
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
@@ -1256,6 +1329,13 @@ int zm_monitor(zm_handle* h, const zm_state* st, int64_t step, double velocity_l
 int zm_budget(zm_handle* h, const zm_state* st, double* sums, void* stream) {
   if (!h || !st || !sums) return fail(h, "invalid zm_budget arguments", ZM_EINVAL);
   return guarded(h, [&] { h->solver->budget(*st, sums, (hipStream_t)stream); });
+}
+
+int zm_budget_step(zm_handle* h, const zm_state* in, const zm_state* out, const void* tau_x, const void* tau_y,
+                   const void* q_heat, const void* t_atm, double* ledger, void* stream) {
+  if (!h || !in || !out || !ledger) return fail(h, "invalid zm_budget_step arguments", ZM_EINVAL);
+  const void* forcing[4] = {tau_x, tau_y, q_heat, t_atm};
+  return guarded(h, [&] { h->solver->budget_step(*in, *out, forcing, ledger, (hipStream_t)stream); });
 }
 
 int zm_project_velocity(zm_handle* h, const void* u, const void* v, void* u_out, void* v_out, double dt,

@@ -37,6 +37,21 @@ void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, c
                             const real* T, const real* S, const real* Fz, const real* cT, const real* cS,
                             const int* gate, real* dT1, real* dS1, const real* Tb, const real* Sb, real* To,
                             real* So, hipStream_t s);
+
+// Ledger sums of one tracer residual evaluation (no state is written): per
+// block, kLedgerTracerSums float64 values = volume integrals of the six
+// surface sources (prescribed heat, bulk heat, coastal bulk heat, coastal
+// temperature restoring, salinity restoring, brine), of the advection
+// tendency of T and S, their absolute values, of the convection tendency of T
+// and S and their absolute values, then the area integrals of Fz_top*T_top
+// and Fz_top*S_top.  lpart holds tracer_ledger_blocks(g) * kLedgerTracerSums.
+constexpr int kLedgerTracerSums = 16;
+template <typename real>
+unsigned tracer_ledger_blocks(const Geo<real>& g);
+template <typename real>
+void launch_tracer_ledger(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
+                          const real* S, const real* Fz, const real* cT, const real* cS, const int* gate,
+                          const double* area_row, const double* dzn64, double* lpart, hipStream_t s);
 template <typename real>
 void launch_pbc(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* pbc,
                 hipStream_t s);

@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __rest
 // d1_flux and d2dz2.
 // ---------------------------------------------------------------------------
 
-#define ZM_COLUMN_PROLOGUE                                                   \
+#define ZM_COLUMN_TILE                                                       \
   const int ntx_ = (g.nx + kTileI - 1) / kTileI;                             \
   const int nty_ = (g.ny + kTileJ - 1) / kTileJ;                             \
   long long t_ = xcd_tile(blockIdx.x, gridDim.x);                            \
@@ -396,14 +396,20 @@ __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __rest
   const int ty_ = (int)(t_ % nty_);                                          \
   const int seg_ = (int)(t_ / nty_);                                         \
   const int i = tx_ * kTileI + (int)(threadIdx.x % kTileI);                  \
-  const int j = ty_ * kTileJ + (int)(threadIdx.x / kTileI);                  \
-  if (i >= g.nx || j >= g.ny) return;                                        \
+  const int j = ty_ * kTileJ + (int)(threadIdx.x / kTileI);
+
+#define ZM_COLUMN_SEGMENT                                                    \
   const int k0 = seg_ * g.seg;                                               \
   const int k1 = k0 + g.seg < g.nz ? k0 + g.seg : g.nz;                      \
   const Nb n = make_nb(g, i, j);                                             \
   const int c2 = n.c2;                                                       \
   const long long n2 = g.n2;                                                 \
   const int kb = n.nwC > 0 ? n.nwC - 1 : 0;
+
+#define ZM_COLUMN_PROLOGUE                                                   \
+  ZM_COLUMN_TILE                                                             \
+  if (i >= g.nx || j >= g.ny) return;                                        \
+  ZM_COLUMN_SEGMENT
 
 template <typename real>
 __device__ __forceinline__ real lap_vals(real c, real e, real w, real nn, real s, bool ge, bool gw, bool gn,
@@ -697,14 +703,30 @@ __device__ __forceinline__ real adv_scalar(const real* __restrict__ X, const rea
   return -(hdiv + divz);
 }
 
-template <typename real, int ADV>
+//
+// LED = true is the ledger instantiation (diagnostics/budgets.py:141-186 fed by
+// tendencies.py:211-214): it writes no state but the per-block float64 sums
+// of the surface sources, the advection and convection tendencies (signed and
+// absolute, times the node volume) and the surface advective flux Fz_top*T_top
+// times the cell area, kLedgerTracerSums values per block in lpart.
+template <typename real, int ADV, bool LED>
 __global__ __launch_bounds__(256) void k_tracer_residual(
     Geo<real> g, Phys<real> P, int mode, const real* __restrict__ u, const real* __restrict__ v,
     const real* __restrict__ T, const real* __restrict__ S, const real* __restrict__ Fz,
     const real* __restrict__ cT, const real* __restrict__ cS, const int* __restrict__ gate,
     real* __restrict__ dT1, real* __restrict__ dS1, const real* __restrict__ Tb,
-    const real* __restrict__ Sb, real* __restrict__ To, real* __restrict__ So) {
-  ZM_COLUMN_PROLOGUE
+    const real* __restrict__ Sb, real* __restrict__ To, real* __restrict__ So,
+    const double* __restrict__ area_row, const double* __restrict__ dzn64, double* __restrict__ lpart) {
+  double lacc[LED ? kLedgerTracerSums : 1];
+  if constexpr (LED)
+    for (int q = 0; q < kLedgerTracerSums; ++q) lacc[q] = 0.0;
+  ZM_COLUMN_TILE
+  const bool inside = i < g.nx && j < g.ny;
+  if constexpr (!LED) {
+    if (!inside) return;
+  }
+  if (inside) {
+  ZM_COLUMN_SEGMENT
   const real idx = g.idx[j], idyc = g.idyc[j];
   const real cfN = g.cfn[j], cfS = n.hasS ? g.cfn[j - 1] : real(0);
   const real kc = P.kh2[c2];
@@ -800,9 +822,33 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
       const real convS = gated ? cS[o] : real(0);
       dTdt = advT + dhT + dvT;
       dSdt = advS + dhS + dvS;
+      if constexpr (LED) {
+        const double vol = area_row[j] * dzn64[k];
+        const double aT = (double)advT * vol, aS = (double)advS * vol;
+        const double qT = (double)convT * vol, qS = (double)convS * vol;
+        lacc[6] += aT;
+        lacc[7] += aS;
+        lacc[8] += fabs(aT);
+        lacc[9] += fabs(aS);
+        lacc[10] += qT;
+        lacc[11] += qS;
+        lacc[12] += fabs(qT);
+        lacc[13] += fabs(qS);
+      }
       if (k == 0) {
         const real hf = P.heat_factor;
         const real ta = P.t_atm[c2];
+        if constexpr (LED) {
+          const double vol = area_row[j] * dzn64[0];
+          lacc[0] += (double)(P.q_heat[c2] * hf) * vol;
+          lacc[1] += (double)(P.lambda_bulk * (ta - Tc) * hf) * vol;
+          lacc[2] += (double)(P.cbl[c2] * (ta - Tc) * hf) * vol;
+          lacc[3] += (double)(P.crc[c2] * (P.crT[c2] - Tc)) * vol;
+          lacc[4] += (double)(P.restore_coef_S * (P.s_ref2[c2] - Sc)) * vol;
+          lacc[5] += (double)(P.ice_salt_flux * (Tc <= P.ice_freeze ? real(1) : real(0))) * vol;
+          lacc[14] += (double)(fz0 * Tc) * area_row[j];
+          lacc[15] += (double)(fz0 * Sc) * area_row[j];
+        }
         dTdt = dTdt + P.q_heat[c2] * hf;
         dTdt = dTdt + P.lambda_bulk * (ta - Tc) * hf;
         dTdt = dTdt + P.cbl[c2] * (ta - Tc) * hf;
@@ -820,17 +866,31 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
     dSdt = dSdt - dhS;
     dTdt = dTdt - dvT;
     dSdt = dSdt - dvS;
-    if (mode == 0) {
-      dT1[o] = dTdt;
-      dS1[o] = dSdt;
-      To[o] = Tc + dTdt * P.dt;
-      So[o] = Sc + dSdt * P.dt;
-    } else {
-      To[o] = Tb[o] + real(0.5) * (dT1[o] + dTdt) * P.dt;
-      So[o] = Sb[o] + real(0.5) * (dS1[o] + dSdt) * P.dt;
+    if constexpr (!LED) {
+      if (mode == 0) {
+        dT1[o] = dTdt;
+        dS1[o] = dSdt;
+        To[o] = Tc + dTdt * P.dt;
+        So[o] = Sc + dSdt * P.dt;
+      } else {
+        To[o] = Tb[o] + real(0.5) * (dT1[o] + dTdt) * P.dt;
+        So[o] = Sb[o] + real(0.5) * (dS1[o] + dSdt) * P.dt;
+      }
     }
     Tm = Tc; Tc = Tp; Sm = Sc; Sc = Sp;
     fz0 = fz1;
+  }
+  }
+  if constexpr (LED) {
+    __shared__ double red[4][kLedgerTracerSums];
+    for (int q = 0; q < kLedgerTracerSums; ++q)
+      for (int off = 32; off > 0; off >>= 1) lacc[q] += __shfl_down(lacc[q], off, 64);
+    if ((threadIdx.x & 63) == 0)
+      for (int q = 0; q < kLedgerTracerSums; ++q) red[threadIdx.x >> 6][q] = lacc[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 0; q < kLedgerTracerSums; ++q)
+        lpart[(long long)blockIdx.x * kLedgerTracerSums + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
   }
 }
 
@@ -1715,15 +1775,39 @@ void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, c
                             const real* T, const real* S, const real* Fz, const real* cT, const real* cS,
                             const int* gate, real* dT1, real* dS1, const real* Tb, const real* Sb, real* To,
                             real* So, hipStream_t s) {
+  const double* nd = nullptr;
+  double* np_ = nullptr;
   if (P.adv_scheme == 2)
-    hipLaunchKernelGGL((k_tracer_residual<real, 2>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
-                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
+    hipLaunchKernelGGL((k_tracer_residual<real, 2, false>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
+                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So, nd, nd, np_);
   else if (P.adv_scheme == 1)
-    hipLaunchKernelGGL((k_tracer_residual<real, 1>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
-                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
+    hipLaunchKernelGGL((k_tracer_residual<real, 1, false>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
+                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So, nd, nd, np_);
   else
-    hipLaunchKernelGGL((k_tracer_residual<real, 0>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
-                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So);
+    hipLaunchKernelGGL((k_tracer_residual<real, 0, false>), dim3(segs3(g)), dim3(256), 0, s, g, P, mode, u, v,
+                     T, S, Fz, cT, cS, gate, dT1, dS1, Tb, Sb, To, So, nd, nd, np_);
+}
+
+template <typename real>
+unsigned tracer_ledger_blocks(const Geo<real>& g) {
+  return segs3(g);
+}
+
+template <typename real>
+void launch_tracer_ledger(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
+                          const real* S, const real* Fz, const real* cT, const real* cS, const int* gate,
+                          const double* area_row, const double* dzn64, double* lpart, hipStream_t s) {
+  real* nr = nullptr;
+  const real* cr = nullptr;
+  if (P.adv_scheme == 2)
+    hipLaunchKernelGGL((k_tracer_residual<real, 2, true>), dim3(segs3(g)), dim3(256), 0, s, g, P, 0, u, v, T,
+                     S, Fz, cT, cS, gate, nr, nr, cr, cr, nr, nr, area_row, dzn64, lpart);
+  else if (P.adv_scheme == 1)
+    hipLaunchKernelGGL((k_tracer_residual<real, 1, true>), dim3(segs3(g)), dim3(256), 0, s, g, P, 0, u, v, T,
+                     S, Fz, cT, cS, gate, nr, nr, cr, cr, nr, nr, area_row, dzn64, lpart);
+  else
+    hipLaunchKernelGGL((k_tracer_residual<real, 0, true>), dim3(segs3(g)), dim3(256), 0, s, g, P, 0, u, v, T,
+                     S, Fz, cT, cS, gate, nr, nr, cr, cr, nr, nr, area_row, dzn64, lpart);
 }
 
 template <typename real>
@@ -1839,6 +1923,10 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                              const real*, const real*, const real*, const real*,            \
                                              const real*, const real*, const int*, real*, real*,            \
                                              const real*, const real*, real*, real*, hipStream_t);           \
+  template unsigned tracer_ledger_blocks<real>(const Geo<real>&);                                          \
+  template void launch_tracer_ledger<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,   \
+                                           const real*, const real*, const real*, const real*, const real*, \
+                                           const int*, const double*, const double*, double*, hipStream_t); \
   template void launch_pbc<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, real*,      \
                                  hipStream_t);                                                               \
   template void launch_bt_rho<real>(const Geo<real>&, const Phys<real>&, const real*, real*, real*,         \
