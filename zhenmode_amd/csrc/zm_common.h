@@ -87,6 +87,15 @@ struct Phys {
 __device__ __forceinline__ int wrap_e(int i, int nx) { return i + 1 == nx ? 0 : i + 1; }
 __device__ __forceinline__ int wrap_w(int i, int nx) { return i == 0 ? nx - 1 : i - 1; }
 
+// Pin a loaded value to this point of the program: the compiler may otherwise
+// sink a load that feeds only one side of a select into a branch, which
+// serialises the memory latency of every guarded neighbour load.  Emits no
+// instruction.
+template <typename T>
+__device__ __forceinline__ void keep(const T& x) {
+  asm volatile("" ::"v"(x));
+}
+
 // Map a launch-order block id onto a tile id so that each of the 8 XCDs gets
 // one contiguous range of tiles (neighbouring tiles share halos through the
 // same L2).  Bijective for any block count.

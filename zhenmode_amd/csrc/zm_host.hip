@@ -112,6 +112,7 @@ struct Solver : SolverBase {
   real *u1, *v1, *T1, *S1, *nAu, *nAv, *nBu, *nBv, *Lu, *Lv, *DT, *DS;
   real *Fz1, *Fz2, *cT, *cS;
   int* gate;
+  unsigned long long* ubits;  // unstable interfaces of the gated columns (2 words per column)
   real *dT1, *dS1, *Tp, *Sp, *du1, *dv1, *up, *vp, *T2, *S2, *u2, *v2, *pbc, *au, *av, *axu, *axv;
   real *u3, *v3, *T3, *S3, *tmpA, *tmpB;
   real *Frx, *Fry, *ubt0, *vbt0, *ubt, *vbt, *etaw, *means;
@@ -339,6 +340,7 @@ struct Solver : SolverBase {
     Fz1 = dev_alloc<real>(n3 + n2);
     Fz2 = dev_alloc<real>(n3 + n2);
     gate = dev_alloc<int>(n2);
+    ubits = dev_alloc<unsigned long long>((size_t)2 * n2);
     real** f2[] = {&Frx, &Fry, &ubt0, &vbt0, &ubt, &vbt, &etaw};
     for (real** p : f2) *p = dev_alloc<real>(n2);
     means = dev_alloc<real>((size_t)4 * nz * (2 * (nb > 0 ? nb : 1)));
@@ -525,7 +527,7 @@ struct Solver : SolverBase {
                  hipStream_t s) {
     // tracer residual #1 at the N-step start (u1, v1, T1, S1)
     zm::launch_fz(g, u1, v1, Fz1, s);
-    zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, s);
+    zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, ubits, s);
     zm::launch_tracer_residual(g, Pf, 0, u1, v1, T1, S1, Fz1, cT, cS, gate, dT1, dS1, T1, S1, Tp, Sp, s);
     if (ledger_on) ledger_tracer(Pf, u1, v1, T1, S1, Fz1, s);
     // momentum residual #1 at (u1, v1, T_pred, S_pred)
@@ -550,7 +552,7 @@ struct Solver : SolverBase {
       zm::launch_fz(g, up, vp, Fz1, s);
       zm::launch_ledger_transport(g, area64, (const real*)Fz1, (const real*)Fz2, led_part, led_raw, s);
     }
-    zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, s);
+    zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, ubits, s);
     zm::launch_tracer_residual(g, Pf, 1, uad, vad, Tp, Sp, Fz2, cT, cS, gate, dT1, dS1, T1, S1, To, So, s);
     if (ledger_on) ledger_tracer(Pf, uad, vad, Tp, Sp, Fz2, s);
     // momentum residual #2 at (u1, v1, T_new, S_new); advection of (u1, v1) is reused
@@ -765,7 +767,7 @@ struct Solver : SolverBase {
         break;
       case ZM_STAGE_TRACER_RES:
         zm::launch_fz(g, ui, vi, Fz1, s);
-        zm::launch_conv(g, P, Ti, Si, cT, cS, gate, s);
+        zm::launch_conv(g, P, Ti, Si, cT, cS, gate, ubits, s);
         zm::launch_tracer_residual(g, P, 0, ui, vi, Ti, Si, Fz1, cT, cS, gate, (real*)out.T, (real*)out.S, Ti,
                                    Si, tmpA, tmpB, s);
         break;
@@ -856,7 +858,7 @@ struct Solver : SolverBase {
           zm::launch_linear_finish(g, P, u1, v1, T1, S1, nAu, nAv, Lu, Lv, DT, DS, u3, v3, T3, S3, s);
           break;
         case 8:
-          zm::launch_conv(g, P, T1, S1, cT, cS, gate, s);
+          zm::launch_conv(g, P, T1, S1, cT, cS, gate, ubits, s);
           break;
         case 9:
           zm::launch_fz(g, u1, v1, Fz1, s);

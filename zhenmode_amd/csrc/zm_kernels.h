@@ -30,8 +30,11 @@ void launch_linear_finish(const Geo<real>& g, const Phys<real>& P, const real* u
 template <typename real>
 void launch_fz(const Geo<real>& g, const real* u, const real* v, real* Fz, hipStream_t s);
 template <typename real>
+// Convective gate (gate[c] = 1 where a wet interface is unstable and
+// kappa_conv > 0; ubits[2c..2c+1] = the unstable interfaces when localize_conv)
+// and the conv sub-cycle of the gated columns into cT, cS.
 void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
-                 int* gate, hipStream_t s);
+                 int* gate, unsigned long long* ubits, hipStream_t s);
 template <typename real>
 void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* u, const real* v,
                             const real* T, const real* S, const real* Fz, const real* cT, const real* cS,

@@ -369,12 +369,31 @@ struct NuTile<double> {
 
 // Unit-coefficient D1 of T and S (first half of _horizontal_biharmonic_tracer).
 template <typename real>
+__device__ __forceinline__ real d1_vals(real c, real e, real w, real nn, real s, bool ge, bool gw, bool gn,
+                                        bool gs, real kfe, real kfw, real kfn, real kfs, real idx, real idy,
+                                        real cfN, real cfS, real idyc) {
+  const real zE = ge ? kfe * (e - c) * idx : real(0);
+  const real zW = gw ? kfw * (c - w) * idx : real(0);
+  const real mN = gn ? kfn * (nn - c) * idy * cfN : real(0);
+  const real mS = gs ? kfs * (c - s) * idy * cfS : real(0);
+  return (zE - zW) * idx + (mN - mS) * idyc;
+}
+
+template <typename real>
 __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __restrict__ T,
                                                  const real* __restrict__ S, real* __restrict__ dT,
                                                  real* __restrict__ dS) {
   ZM_CELL_PROLOGUE
-  dT[c] = d1_flux(T, plane, n, k, g, (const real*)nullptr);
-  dS[c] = d1_flux(S, plane, n, k, g, (const real*)nullptr);
+  // d1_flux with unit coefficients, neighbours loaded unconditionally
+  const bool wc = k < n.nwC;
+  const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
+  const bool gn = n.hasN && wc && k < n.nwN, gs = n.hasS && wc && k < n.nwS;
+  const real one = real(1);
+  const real idx = g.idx[n.j], idyc = g.idyc[n.j], cfN = g.cfn[n.j], cfS = n.hasS ? g.cfn[n.j - 1] : real(0);
+  dT[c] = d1_vals(T[plane + n.c2], T[plane + n.e2], T[plane + n.w2], T[plane + n.n2e], T[plane + n.s2e], ge, gw,
+                  gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc);
+  dS[c] = d1_vals(S[plane + n.c2], S[plane + n.e2], S[plane + n.w2], S[plane + n.n2e], S[plane + n.s2e], ge, gw,
+                  gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc);
 }
 
 
@@ -419,16 +438,6 @@ __device__ __forceinline__ real lap_vals(real c, real e, real w, real nn, real s
   return (fe - fw) * idx2 + (fn - fs) * idy2 + (-((nn - s) * hidy) * tr);
 }
 
-template <typename real>
-__device__ __forceinline__ real d1_vals(real c, real e, real w, real nn, real s, bool ge, bool gw, bool gn,
-                                        bool gs, real kfe, real kfw, real kfn, real kfs, real idx, real idy,
-                                        real cfN, real cfS, real idyc) {
-  const real zE = ge ? kfe * (e - c) * idx : real(0);
-  const real zW = gw ? kfw * (c - w) * idx : real(0);
-  const real mN = gn ? kfn * (nn - c) * idy * cfN : real(0);
-  const real mS = gs ? kfs * (c - s) * idy * cfS : real(0);
-  return (zE - zW) * idx + (mN - mS) * idyc;
-}
 
 // node-form d2/dz2 with the seafloor ghost fill, from the window (k-1, k, k+1);
 // ghost nodes below the deepest wet level (k > kb) return 0.
@@ -554,15 +563,23 @@ __global__ __launch_bounds__(256) void k_linear_finish(
 // ---------------------------------------------------------------------------
 template <typename real>
 __device__ __forceinline__ real div_h(const real* __restrict__ u, const real* __restrict__ v,
-                                      long long plane, const Nb& n, int k, const Geo<real>& g) {
-  const real uc = u[plane + n.c2];
+                                      long long plane, const Nb& n, int k, real idx, real idyc, real cfN,
+                                      real cfS) {
+  // every neighbour index is in bounds (x periodic, y edge-clamped): load
+  // unconditionally and select, so the loads of a level issue together; the
+  // row metrics come in registers (a store to Fz could alias the metric rows)
+  const real uc = u[plane + n.c2], ue = u[plane + n.e2], uw = u[plane + n.w2];
+  const real vc = v[plane + n.c2], vn = v[plane + n.n2e], vs = v[plane + n.s2e];
+  keep(ue);
+  keep(uw);
+  keep(vn);
+  keep(vs);
   const bool wc = k < n.nwC;
-  const real fxe = (wc && k < n.nwE) ? real(0.5) * (uc + u[plane + n.e2]) : real(0);
-  const real fxw = (wc && k < n.nwW) ? real(0.5) * (u[plane + n.w2] + uc) : real(0);
-  const real vc = v[plane + n.c2];
-  const real fyn = (n.hasN && wc && k < n.nwN) ? real(0.5) * (vc + v[plane + n.n2e]) * g.cfn[n.j] : real(0);
-  const real fys = (n.hasS && wc && k < n.nwS) ? real(0.5) * (v[plane + n.s2e] + vc) * g.cfn[n.j - 1] : real(0);
-  return (fxe - fxw) * g.idx[n.j] + (fyn - fys) * g.idyc[n.j];
+  const real fxe = (wc && k < n.nwE) ? real(0.5) * (uc + ue) : real(0);
+  const real fxw = (wc && k < n.nwW) ? real(0.5) * (uw + uc) : real(0);
+  const real fyn = (n.hasN && wc && k < n.nwN) ? real(0.5) * (vc + vn) * cfN : real(0);
+  const real fys = (n.hasS && wc && k < n.nwS) ? real(0.5) * (vs + vc) * cfS : real(0);
+  return (fxe - fxw) * idx + (fyn - fys) * idyc;
 }
 
 template <typename real>
@@ -573,11 +590,18 @@ __global__ __launch_bounds__(256) void k_fz(Geo<real> g, const real* __restrict_
   const int j = c2 / g.nx, i = c2 - j * g.nx;
   const Nb n = make_nb(g, i, j);
   real acc = 0;
+  const real idx = g.idx[j], idyc = g.idyc[j], cfN = g.cfn[j], cfS = n.hasS ? g.cfn[j - 1] : real(0);
   Fz[(long long)g.nz * g.n2 + c2] = real(0);
+  // dz of the next level is read before this level's store (the store could
+  // alias the metric array, which would otherwise serialise every level)
+  real dzk = g.dzn[g.nz - 1];
+#pragma unroll 10
   for (int k = g.nz - 1; k >= 0; --k) {
     const long long plane = (long long)k * g.n2;
-    acc = acc + div_h(u, v, plane, n, k, g) * g.dzn[k];
+    const real dz_next = g.dzn[k > 0 ? k - 1 : 0];
+    acc = acc + div_h(u, v, plane, n, k, idx, idyc, cfN, cfS) * dzk;
     Fz[plane + c2] = acc;
+    dzk = dz_next;
   }
 }
 
@@ -593,10 +617,46 @@ __device__ __forceinline__ real rho_anom(real T, real S, const Phys<real>& P) {
 }
 
 template <typename real>
+__global__ __launch_bounds__(256) void k_conv_gate(Geo<real> g, Phys<real> P, const real* __restrict__ T,
+                                                   const real* __restrict__ S, int* __restrict__ gate,
+                                                   unsigned long long* __restrict__ ubits) {
+  // pass 1: the column gate and the unstable interfaces k+1/2 (k < 128), one
+  // column per thread with no LDS, so every column is tested at full occupancy
+  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  if (c2 >= g.n2) return;
+  const int nw = g.nwet[c2];
+  const long long n2 = g.n2;
+  bool unstable = false;
+  unsigned long long ub0 = 0, ub1 = 0;
+  real rprev = rho_anom(T[c2], S[c2], P);
+  // full-depth loop with a wet test, so the loads of several levels are in
+  // flight together; levels at or below nw never set a bit
+#pragma unroll 10
+  for (int k = 1; k < g.nz; ++k) {
+    const real Tk = T[(long long)k * n2 + c2], Sk = S[(long long)k * n2 + c2];
+    keep(Tk);
+    keep(Sk);
+    const real r = rho_anom(Tk, Sk, P);
+    const bool un = k < nw && rprev > r;
+    unstable |= un;
+    if (k - 1 < 64) ub0 |= (unsigned long long)un << (k - 1);
+    else ub1 |= (unsigned long long)un << ((k - 65) & 63);
+    rprev = r;
+  }
+  const bool on = unstable && P.kappa_conv > real(0);
+  gate[c2] = on ? 1 : 0;
+  if (on && P.local_conv) {
+    ubits[2 * (long long)c2] = ub0;
+    ubits[2 * (long long)c2 + 1] = ub1;
+  }
+}
+
+template <typename real>
 __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const real* __restrict__ T,
                                              const real* __restrict__ S, real* __restrict__ cT,
-                                             real* __restrict__ cS, int* __restrict__ gate) {
-  // LDS columns [k][lane]: the state being mixed and the sum of the substep terms
+                                             real* __restrict__ cS, const int* __restrict__ gate,
+                                             const unsigned long long* __restrict__ ubits) {
+  // pass 2, gated columns only: the working column sits in LDS as [k][lane]
   extern __shared__ unsigned char zm_lds[];
   real* tT = reinterpret_cast<real*>(zm_lds);
   real* tS = tT + (size_t)g.nz * 64;
@@ -604,7 +664,7 @@ __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const re
   real* aS = aT + (size_t)g.nz * 64;
   const int lane = threadIdx.x;
   const int c2 = blockIdx.x * 64 + lane;
-  if (c2 >= g.n2) return;
+  if (c2 >= g.n2 || !gate[c2]) return;
   const int nw = g.nwet[c2];
   const int n2 = g.n2;
   // stage the wet part of the column (independent loads, issued together)
@@ -612,26 +672,11 @@ __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const re
     tT[k * 64 + lane] = T[(long long)k * n2 + c2];
     tS[k * 64 + lane] = S[(long long)k * n2 + c2];
   }
-  bool unstable = false;
-  unsigned long long ub0 = 0, ub1 = 0;  // unstable interfaces k+1/2 (k < 128), frozen over the sub-cycle
-  if (nw >= 2) {
-    real rprev = rho_anom(tT[lane], tS[lane], P);
-    for (int k = 1; k < nw; ++k) {
-      const real r = rho_anom(tT[k * 64 + lane], tS[k * 64 + lane], P);
-      const bool un = rprev > r;
-      unstable |= un;
-      if (un) {
-        if (k - 1 < 64) ub0 |= 1ull << (k - 1);
-        else ub1 |= 1ull << (k - 65);
-      }
-      rprev = r;
-    }
+  unsigned long long ub0 = 0, ub1 = 0;  // unstable interfaces, frozen over the sub-cycle
+  if (P.local_conv) {
+    ub0 = ubits[2 * (long long)c2];
+    ub1 = ubits[2 * (long long)c2 + 1];
   }
-  if (!unstable || P.kappa_conv <= real(0)) {
-    gate[c2] = 0;
-    return;
-  }
-  gate[c2] = 1;
   const int nc = P.conv_nsub;
   const real kap = nc > 1 ? P.kappa_conv / real(nc) : P.kappa_conv;
   const real h = P.dt / real(nc);
@@ -907,9 +952,13 @@ __global__ __launch_bounds__(256) void k_pbc(Geo<real> g, Phys<real> P, const re
   real rprev = 0 < nw ? rho_anom(T[c2], S[c2], P) : real(0);
   real acc = 0;
   pbc[c2] = real(0);
+#pragma unroll 8
   for (int k = 1; k < g.nz; ++k) {
     const long long o = (long long)k * g.n2 + c2;
-    const real r = k < nw ? rho_anom(T[o], S[o], P) : real(0);
+    const real To = T[o], So = S[o];  // loaded on every level, used on wet ones
+    keep(To);
+    keep(So);
+    const real r = k < nw ? rho_anom(To, So, P) : real(0);
     acc = acc + real(kG) * (real(0.5) * (rprev + r)) * g.dz3[k - 1];
     pbc[o] = acc;
     rprev = r;
@@ -930,13 +979,19 @@ __global__ __launch_bounds__(256) void k_bt_rho(Geo<real> g, Phys<real> P, const
   const Nb n = make_nb(g, i, j);
   auto grad = [&](int k, real& gx, real& gy) {
     const long long pl = (long long)k * g.n2;
-    const real pn = n.hasN ? pbc[pl + n.c2 + g.nx] : real(0);
-    const real ps = n.hasS ? pbc[pl + n.c2 - g.nx] : real(0);
-    grad_cons(pbc[pl + n.c2], pbc[pl + n.e2], pbc[pl + n.w2], pn, ps, n, k, g, gx, gy);
+    const real pc = pbc[pl + n.c2], pe = pbc[pl + n.e2], pw = pbc[pl + n.w2];
+    const real pn = pbc[pl + n.n2e];  // edge-clamped: unused at the walls
+    const real ps = pbc[pl + n.s2e];
+    keep(pe);
+    keep(pw);
+    keep(pn);
+    keep(ps);
+    grad_cons(pc, pe, pw, pn, ps, n, k, g, gx, gy);
   };
   real gxp, gyp;
   grad(0, gxp, gyp);
   real sx = 0, sy = 0;
+#pragma unroll 8
   for (int k = 0; k + 1 < g.nz; ++k) {
     real gxn, gyn;
     grad(k + 1, gxn, gyn);
@@ -952,6 +1007,7 @@ __global__ __launch_bounds__(256) void k_bt_rho(Geo<real> g, Phys<real> P, const
   if (ubt0) {
     real su = 0, sv = 0;
     real up = u[c2], vp = v[c2];
+#pragma unroll 8
     for (int k = 0; k + 1 < g.nz; ++k) {
       const long long o = (long long)(k + 1) * g.n2 + c2;
       const real un = u[o], vn = v[o];
@@ -1003,17 +1059,24 @@ __global__ __launch_bounds__(256) void k_mom_adv(Geo<real> g, const real* __rest
   real um = k0 > 0 ? u[o - n2] : real(0), uc = u[o];
   real vm = k0 > 0 ? v[o - n2] : real(0), vc = v[o];
   for (int k = k0; k < k1; ++k, o += n2) {
+    // all loads in bounds (the level below is clamped at the last level):
+    // issued unconditionally, then selected
     const bool more = k + 1 < g.nz;
-    const real up = more ? u[o + n2] : uc, vp = more ? v[o + n2] : vc;
+    const long long onext = more ? o + n2 : o;
+    const real un_ = u[onext], vn_ = v[onext];
+    const real up = more ? un_ : uc, vp = more ? vn_ : vc;
     const long long pl = o - c2;
+    const real fz = Fz[o];
+    const real uE = u[pl + n.e2], uW = u[pl + n.w2], uN = u[pl + n.n2e], uS = u[pl + n.s2e];
+    const real vE = v[pl + n.e2], vW = v[pl + n.w2], vN = v[pl + n.n2e], vS = v[pl + n.s2e];
     const bool wc = k < n.nwC;
     const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
     const bool gn = wc && k < n.nwN, gs = wc && k < n.nwS;
-    const real w = wc ? -Fz[o] : real(0);
-    const real dudx = hidx * ((ge ? u[pl + n.e2] - uc : real(0)) + (gw ? uc - u[pl + n.w2] : real(0)));
-    const real dudy = hidy * ((gn ? u[pl + n.n2e] - uc : real(0)) + (gs ? uc - u[pl + n.s2e] : real(0)));
-    const real dvdx = hidx * ((ge ? v[pl + n.e2] - vc : real(0)) + (gw ? vc - v[pl + n.w2] : real(0)));
-    const real dvdy = hidy * ((gn ? v[pl + n.n2e] - vc : real(0)) + (gs ? vc - v[pl + n.s2e] : real(0)));
+    const real w = wc ? -fz : real(0);
+    const real dudx = hidx * ((ge ? uE - uc : real(0)) + (gw ? uc - uW : real(0)));
+    const real dudy = hidy * ((gn ? uN - uc : real(0)) + (gs ? uc - uS : real(0)));
+    const real dvdx = hidx * ((ge ? vE - vc : real(0)) + (gw ? vc - vW : real(0)));
+    const real dvdy = hidy * ((gn ? vN - vc : real(0)) + (gs ? vc - vS : real(0)));
     const real dudz = ddz_win(um, uc, up, k, kb, g);
     const real dvdz = ddz_win(vm, vc, vp, k, kb, g);
     au[o] = -(uc * dudx + vc * dudy + w * dudz);
@@ -1757,7 +1820,8 @@ void launch_fz(const Geo<real>& g, const real* u, const real* v, real* Fz, hipSt
 
 template <typename real>
 void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
-                 int* gate, hipStream_t s) {
+                 int* gate, unsigned long long* ubits, hipStream_t s) {
+  hipLaunchKernelGGL(k_conv_gate<real>, dim3(blocks_for(g.n2, 256)), dim3(256), 0, s, g, P, T, S, gate, ubits);
   const size_t lds = (size_t)4 * g.nz * 64 * sizeof(real);
   if (lds > 65536) {
     static bool raised = false;  // once per instantiation (one per real type)
@@ -1767,7 +1831,8 @@ void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const r
       raised = true;
     }
   }
-  hipLaunchKernelGGL(k_conv<real>, dim3(blocks_for(g.n2, 64)), dim3(64), lds, s, g, P, T, S, cT, cS, gate);
+  hipLaunchKernelGGL(k_conv<real>, dim3(blocks_for(g.n2, 64)), dim3(64), lds, s, g, P, T, S, cT, cS,
+                     (const int*)gate, (const unsigned long long*)ubits);
 }
 
 template <typename real>
@@ -1918,7 +1983,7 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                            real*, hipStream_t);                                              \
   template void launch_fz<real>(const Geo<real>&, const real*, const real*, real*, hipStream_t);            \
   template void launch_conv<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, real*,     \
-                                  real*, int*, hipStream_t);                                                 \
+                                  real*, int*, unsigned long long*, hipStream_t);                            \
   template void launch_tracer_residual<real>(const Geo<real>&, const Phys<real>&, int, const real*,         \
                                              const real*, const real*, const real*, const real*,            \
                                              const real*, const real*, const int*, real*, real*,            \
