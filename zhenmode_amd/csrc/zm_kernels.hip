@@ -1495,25 +1495,32 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
   const real* wet = P.wet2;
   real en[kRowChunk];
   real ssum = 0, wsum = 0;
+  // row metrics in registers: the selects below then only mix loaded values
+  const bool hasN = j + 1 < g.ny, hasS = j > 0;
+  const real idx = g.idx[j], idyc = g.idyc[j], cfN = g.cfn[j], cfS = g.cfn[hasS ? j - 1 : 0];
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
-    const int i = threadIdx.x + q * 256;
-    en[q] = 0;
-    if (i >= g.nx) continue;
+    // lanes past the row end repeat its last column and book nothing
+    const bool act = threadIdx.x + q * 256 < g.nx;
+    const int i = act ? threadIdx.x + q * 256 : g.nx - 1;
     const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
-    const bool wc = wet[c2] != real(0);
-    const bool we = wet[e2] != real(0), ww = wet[w2] != real(0);
-    const real fe = (wc && we) ? real(0.5) * (ubt[c2] + ubt[e2]) : real(0);
-    const real fw = (ww && wc) ? real(0.5) * (ubt[w2] + ubt[c2]) : real(0);
-    const real divx = (fe - fw) * g.idx[j];
-    real fn = 0, fs = 0;
-    if (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) fn = real(0.5) * (vbt[c2] + vbt[c2 + g.nx]) * g.cfn[j];
-    if (j > 0 && wc && wet[c2 - g.nx] != real(0)) fs = real(0.5) * (vbt[c2 - g.nx] + vbt[c2]) * g.cfn[j - 1];
-    const real divy = (fn - fs) * g.idyc[j];
-    const real e = wc ? eta_row[i] - dtH * (divx + divy) : real(0);
-    en[q] = e;
-    ssum += e;
-    wsum += wet[c2];
+    const int n2 = hasN ? c2 + g.nx : c2, s2 = hasS ? c2 - g.nx : c2;  // clamped: unused at the walls
+    // every index is in bounds: load unconditionally, then select
+    const real wC = wet[c2], wE = wet[e2], wW = wet[w2], wN = wet[n2], wS = wet[s2];
+    const real uC = ubt[c2], uE = ubt[e2], uW = ubt[w2];
+    const real vC = vbt[c2], vN = vbt[n2], vS = vbt[s2];
+    const real eC = eta_row[i];
+    const bool wc = wC != real(0);
+    const real fe = (wc && wE != real(0)) ? real(0.5) * (uC + uE) : real(0);
+    const real fw = (wW != real(0) && wc) ? real(0.5) * (uW + uC) : real(0);
+    const real divx = (fe - fw) * idx;
+    const real fn = (hasN && wc && wN != real(0)) ? real(0.5) * (vC + vN) * cfN : real(0);
+    const real fs = (hasS && wc && wS != real(0)) ? real(0.5) * (vS + vC) * cfS : real(0);
+    const real divy = (fn - fs) * idyc;
+    const real e = wc ? eC - dtH * (divx + divy) : real(0);
+    en[q] = act ? e : real(0);
+    ssum += act ? e : real(0);
+    wsum += act ? wC : real(0);
   }
   const real wb = cap_weight(P, j, g.ny);
   real zm = 0;
@@ -1546,33 +1553,44 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
   const real* wet = P.wet2;
   real un[kRowChunk], vn[kRowChunk];
   real su = 0, sv = 0, sw = 0;
+  // row data in registers: the selects below then only mix loaded values
+  const bool hasN = j + 1 < g.ny, hasS = j > 0;
+  const real idx = g.idx[j], idyc = g.idyc[j], cfN = g.cfn[j], cfS = g.cfs[j];
+  const real* en_safe = hasN ? en_row : ec_row;  // rows past the walls are never read
+  const real* es_safe = hasS ? es_row : ec_row;
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
-    const int i = threadIdx.x + q * 256;
-    un[q] = vn[q] = 0;
-    if (i >= g.nx) continue;
+    // lanes past the row end repeat its last column and book nothing
+    const bool act = threadIdx.x + q * 256 < g.nx;
+    const int i = act ? threadIdx.x + q * 256 : g.nx - 1;
     const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
-    const bool wc = wet[c2] != real(0);
-    const real ec = ec_row[i];
-    const real dxp = (wc && wet[e2] != real(0)) ? ec_row[wrap_e(i, g.nx)] - ec : real(0);
-    const real dxm = (wc && wet[w2] != real(0)) ? ec - ec_row[wrap_w(i, g.nx)] : real(0);
-    const real gx = g.idx[j] * real(0.5) * (dxp + dxm);
-    const real dyp = (j + 1 < g.ny && wc && wet[c2 + g.nx] != real(0)) ? (en_row[i] - ec) * g.cfn[j] : real(0);
-    const real dym = (j > 0 && wc && wet[c2 - g.nx] != real(0)) ? (ec - es_row[i]) * g.cfs[j] : real(0);
-    const real gy = g.idyc[j] * real(0.5) * (dyp + dym);
+    // every index is in bounds (rows j +- 1 clamped to row j at the walls):
+    // load unconditionally, then select
+    const real wC = wet[c2], wE = wet[e2], wW = wet[w2];
+    const real wN = wet[hasN ? c2 + g.nx : c2], wS = wet[hasS ? c2 - g.nx : c2];
+    const real ec = ec_row[i], eE = ec_row[wrap_e(i, g.nx)], eW = ec_row[wrap_w(i, g.nx)];
+    const real eN = en_safe[i], eS = es_safe[i];
+    const real u0 = ubt[c2], v0 = vbt[c2];
+    const bool wc = wC != real(0);
+    const real dxp = (wc && wE != real(0)) ? eE - ec : real(0);
+    const real dxm = (wc && wW != real(0)) ? ec - eW : real(0);
+    const real gx = idx * real(0.5) * (dxp + dxm);
+    const real dyp = (hasN && wc && wN != real(0)) ? (eN - ec) * cfN : real(0);
+    const real dym = (hasS && wc && wS != real(0)) ? (ec - eS) * cfS : real(0);
+    const real gy = idyc * real(0.5) * (dyp + dym);
     const real Fx = Frx[c2] + P.tau_x[c2] * P.inv_rho0H;
     const real Fy = Fry[c2] + P.tau_y[c2] * P.inv_rho0H;
-    const real us = ubt[c2] + dt_bt * (real(-kG) * gx + Fx);
-    const real vs = vbt[c2] + dt_bt * (real(-kG) * gy + Fy);
+    const real us = u0 + dt_bt * (real(-kG) * gx + Fx);
+    const real vs = v0 + dt_bt * (real(-kG) * gy + Fy);
     real a = (us + fd * vs) / den;
     real b = (vs - fd * us) / den;
     a = wc ? a * drag : real(0);
     b = wc ? b * drag : real(0);
-    un[q] = a;
-    vn[q] = b;
-    su += a;
-    sv += b;
-    sw += wet[c2];
+    un[q] = act ? a : real(0);
+    vn[q] = act ? b : real(0);
+    su += act ? a : real(0);
+    sv += act ? b : real(0);
+    sw += act ? wC : real(0);
   }
   const real wb = cap_weight(P, j, g.ny);
   real zu = 0, zv = 0;
