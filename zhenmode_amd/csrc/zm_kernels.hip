@@ -513,8 +513,10 @@ __global__ __launch_bounds__(256) void k_linear_finish(
   real Sm = k0 > 0 ? S0[o - n2] : real(0), Sc = S0[o];
   for (int k = k0; k < k1; ++k, o += n2) {
     const bool more = k + 1 < g.nz;
-    const real up = more ? u0[o + n2] : uc, vp = more ? v0[o + n2] : vc;
-    const real Tp = more ? T0[o + n2] : Tc, Sp = more ? S0[o + n2] : Sc;
+    const long long onext = more ? o + n2 : o;  // in bounds at the last level
+    const real un_ = u0[onext], vn_ = v0[onext], Tn_ = T0[onext], Sn_ = S0[onext];
+    const real up = more ? un_ : uc, vp = more ? vn_ : vc;
+    const real Tp = more ? Tn_ : Tc, Sp = more ? Sn_ : Sc;
     const long long pl = o - c2;
     const bool wc = k < n.nwC;
     const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
@@ -798,7 +800,9 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
   real fz0 = Fz[o];
   for (int k = k0; k < k1; ++k, o += n2) {
     const bool more = k + 1 < g.nz;
-    const real Tp = more ? T[o + n2] : Tc, Sp = more ? S[o + n2] : Sc;
+    const long long onext = more ? o + n2 : o;  // in bounds at the last level
+    const real Tn_ = T[onext], Sn_ = S[onext];
+    const real Tp = more ? Tn_ : Tc, Sp = more ? Sn_ : Sc;
     const real fz1 = Fz[o + n2];
     const long long pl = o - c2;
     const bool wc = k < n.nwC;
@@ -810,12 +814,16 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
     const real dhS = d1_vals(Sc, SE, SW, SN, SS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
     const real dvT = vdiff_win(Tm, Tc, Tp, k, n.nwC, kb, kv, P.cons_kv != 0, g);
     const real dvS = vdiff_win(Sm, Sc, Sp, k, n.nwC, kb, kv, P.cons_kv != 0, g);
+    // the velocity and convection loads of the level are in bounds: issued
+    // unconditionally, used on wet nodes
+    const real uc = u[o], vc = v[o];
+    const real uE = u[pl + n.e2], uW = u[pl + n.w2], vN = v[pl + n.n2e], vS = v[pl + n.s2e];
+    const real cTo = cT[o], cSo = cS[o];
     real dTdt = 0, dSdt = 0;
     if (wc) {
       // centred face-gated horizontal fluxes (transport.py:224-290)
-      const real uc = u[o], vc = v[o];
-      const real uxe = h * (uc + u[pl + n.e2]), uxw = h * (u[pl + n.w2] + uc);
-      const real vyn = h * (vc + v[pl + n.n2e]), vys = h * (v[pl + n.s2e] + vc);
+      const real uxe = h * (uc + uE), uxw = h * (uW + uc);
+      const real vyn = h * (vc + vN), vys = h * (vS + vc);
       const real ufe = ge ? uxe : real(0);
       const real ufw = gw ? uxw : real(0);
       const real vfn = gn ? vyn : real(0);
@@ -863,8 +871,8 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
       }
       const real advT = -(hT + (dnT - upT) * g.idzn[k]);
       const real advS = -(hS + (dnS - upS) * g.idzn[k]);
-      const real convT = gated ? cT[o] : real(0);
-      const real convS = gated ? cS[o] : real(0);
+      const real convT = gated ? cTo : real(0);
+      const real convS = gated ? cSo : real(0);
       dTdt = advT + dhT + dvT;
       dSdt = advS + dhS + dvS;
       if constexpr (LED) {
@@ -1362,7 +1370,8 @@ __global__ __launch_bounds__(256) void k_mom_residual(
   // column constants: eta at the five points, 2-D forcing
   const real rg = real(kRho0 * kG);
   const real eC = eta[c2], eE = eta[n.e2], eW = eta[n.w2];
-  const real eN = n.hasN ? eta[c2 + g.nx] : real(0), eS = n.hasS ? eta[c2 - g.nx] : real(0);
+  const real eNl = eta[n.n2e], eSl = eta[n.s2e];  // edge-clamped, selected below
+  const real eN = n.hasN ? eNl : real(0), eS = n.hasS ? eSl : real(0);
   const real frx = Frx[c2], fry = Fry[c2];
   const real tx = P.tau_x[c2], ty = P.tau_y[c2];
   const real sfx = tx * P.wind_factor, sfy = ty * P.wind_factor;
@@ -1373,25 +1382,27 @@ __global__ __launch_bounds__(256) void k_mom_residual(
   real vm = k0 > 0 ? v[o - n2] : real(0), vc = v[o];
   for (int k = k0; k < k1; ++k, o += n2) {
     const bool more = k + 1 < g.nz;
-    const real up = more ? u[o + n2] : uc, vp = more ? v[o + n2] : vc;
+    const long long onext = more ? o + n2 : o;  // in bounds at the last level
+    const real un_ = u[onext], vn_ = v[onext];
+    const real up = more ? un_ : uc, vp = more ? vn_ : vc;
     const long long pl = o - c2;
     const bool wc = k < n.nwC;
     const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
     const bool gnE = wc && k < n.nwN, gsE = wc && k < n.nwS;
     const bool gn = n.hasN && gnE, gs = n.hasS && gsE;
-    real advu = 0, advv = 0;
-    if (wc) {
-      advu = (ax[o + rm2] + real(4) * ax[o + rm1] + real(6) * ax[o] + real(4) * ax[o + rp1] + ax[o + rp2]) /
-             real(16);
-      advv = (ay[o + rm2] + real(4) * ay[o + rm1] + real(6) * ay[o] + real(4) * ay[o + rp1] + ay[o + rp2]) /
-             real(16);
-    }
+    // all loads of the level are in bounds (rows clamped): issued unconditionally
+    const real a0 = ax[o + rm2], a1 = ax[o + rm1], a2 = ax[o], a3 = ax[o + rp1], a4 = ax[o + rp2];
+    const real b0 = ay[o + rm2], b1 = ay[o + rm1], b2 = ay[o], b3 = ay[o + rp1], b4 = ay[o + rp2];
+    const real pbC = pbc[o], pbE = pbc[pl + n.e2], pbW = pbc[pl + n.w2], pbN = pbc[pl + n.n2e], pbS = pbc[pl + n.s2e];
+    const real advu_ = (a0 + real(4) * a1 + real(6) * a2 + real(4) * a3 + a4) / real(16);
+    const real advv_ = (b0 + real(4) * b1 + real(6) * b2 + real(4) * b3 + b4) / real(16);
+    const real advu = wc ? advu_ : real(0), advv = wc ? advv_ : real(0);
     // hydrostatic pressure p = rho0 g eta + p_bc at the five points
-    const real pc = rg * eC + pbc[o];
-    const real pe = rg * eE + pbc[pl + n.e2];
-    const real pw = rg * eW + pbc[pl + n.w2];
-    const real pn = n.hasN ? rg * eN + pbc[o + g.nx] : real(0);
-    const real ps = n.hasS ? rg * eS + pbc[o - g.nx] : real(0);
+    const real pc = rg * eC + pbC;
+    const real pe = rg * eE + pbE;
+    const real pw = rg * eW + pbW;
+    const real pn = n.hasN ? rg * eN + pbN : real(0);
+    const real ps = n.hasS ? rg * eS + pbS : real(0);
     const real gpx = idx * half * ((ge ? pe - pc : real(0)) + (gw ? pc - pw : real(0)));
     const real gpy = idyc * half * ((gn ? (pn - pc) * cfn : real(0)) + (gs ? (pc - ps) * cfs : real(0)));
     const real pgfx = -gpx / real(kRho0), pgfy = -gpy / real(kRho0);
