@@ -50,6 +50,7 @@ struct Geo {
   real ih0top2, ih0bot2;  // 1/h0^2 at top and bottom
   const int* nwet;        // wet levels per column [n2]
   int seg;                // levels per thread in the column-marching kernels
+  int nu_chunk;           // rows per wave of the streaming nu_h kernel (0: register-tile kernel)
 };
 
 // Scalars and 2-D fields of the physics.

@@ -230,6 +230,8 @@ struct Solver : SolverBase {
     ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
     g.nwet = nw;
     g.seg = 10;
+    g.nu_chunk = 32;
+    if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
     dzn64 = dev_alloc<double>(nz);
     ZM_HIP(hipMemcpy(dzn64, c.dz_node, nz * sizeof(double), hipMemcpyHostToDevice));
     dzi64 = dev_alloc<double>(nz > 1 ? nz - 1 : 1);

@@ -356,6 +356,193 @@ __global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __res
   }
 }
 
+// Streaming form of the temporally blocked nu_h sub-cycle.  A wave owns 64
+// longitudes of one level and marches north through a chunk of rows.  Level l
+// of the NS substeps keeps a window of three consecutive rows in registers;
+// in one march step every level l computes its row of level l + 1 from the
+// window as it stood at the start of the step, so the 2 NS computations of a
+// step are independent, and the new rows then enter the windows of the level
+// above (level 0: the loaded row).  Level l + 1 therefore trails level l by
+// two rows: in step t it computes row t - 2 l - 1.  Rows are only recomputed
+// at the chunk ends (NS - l extra rows per side at level l) and input rows
+// are prefetched kNuAhead rows ahead.  East/west neighbours come from the
+// adjacent lanes (DPP), so as in k_nu_strip lanes [NS, 64 - NS) are exact.
+// Every node receives the arithmetic of lap_h / k_nu_substep (lap_open where
+// the row's interior lanes have all four faces open, lap_strip otherwise).
+constexpr int kNuAhead = 6;  // a multiple of the period of the three-row window
+
+template <typename real, int NS>
+struct NuStream {
+  real W0u[NS], W1u[NS], W2u[NS];  // level l: rows (south, centre, north) of its next computation
+  real W0v[NS], W1v[NS], W2v[NS];
+  real qu[kNuAhead], qv[kNuAhead];  // prefetched input rows
+  int qm[kNuAhead];                 // their wet bit at this level
+  real qi[kNuAhead], qt[kNuAhead];  // their row metrics 1/dx^2, tan(lat)/R
+  real mi[2 * NS], mt[2 * NS];      // metrics of rows t, t - 1, ..., t - 2 NS + 1
+  unsigned hist;                    // bit b: wet at row t - b (this lane)
+  unsigned long long openh;         // bit b: row t - 1 - b fully open (uniform)
+};
+
+template <typename real, int NS, int SLOT>
+__device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, int t, int k, int i,
+                                               int lane, int qe, int jb, int je, const real* __restrict__ up,
+                                               const real* __restrict__ vp, real* __restrict__ uop,
+                                               real* __restrict__ vop, bool store_lane, int io, real nu_h,
+                                               real dts) {
+  const bool have = t < qe;  // uniform
+  // row t enters the level-0 window
+  st.W0u[0] = st.W1u[0];
+  st.W1u[0] = st.W2u[0];
+  st.W2u[0] = st.qu[SLOT];
+  st.W0v[0] = st.W1v[0];
+  st.W1v[0] = st.W2v[0];
+  st.W2v[0] = st.qv[SLOT];
+  const unsigned mc = have ? (unsigned)st.qm[SLOT] : 0u;
+#pragma unroll
+  for (int l = 2 * NS - 1; l > 0; --l) {
+    st.mi[l] = st.mi[l - 1];
+    st.mt[l] = st.mt[l - 1];
+  }
+  st.mi[0] = st.qi[SLOT];
+  st.mt[0] = st.qt[SLOT];
+  // prefetch row t + kNuAhead into this slot
+  if (t + kNuAhead < qe) {
+    const int c2 = (t + kNuAhead) * g.nx + i;
+    st.qu[SLOT] = up[c2];
+    st.qv[SLOT] = vp[c2];
+    st.qm[SLOT] = k < g.nwet[c2] ? 1 : 0;
+    st.qi[SLOT] = g.idx2[t + kNuAhead];
+    st.qt[SLOT] = g.tanR[t + kNuAhead];
+  }
+  st.hist = (st.hist << 1) | mc;
+  const unsigned hE = (unsigned)dpp_from_right((int)st.hist), hW = (unsigned)dpp_from_left((int)st.hist);
+  // row t - 1 has all its gates now: open flag over the interior lanes
+  {
+    const int r = t - 1;
+    const unsigned c = (st.hist >> 1) & 1u;
+    const unsigned n = r == g.ny - 1 ? 1u : st.hist & 1u;
+    const unsigned s = r == 0 ? 1u : (st.hist >> 2) & 1u;
+    const unsigned full = c & (hE >> 1) & (hW >> 1) & n & s & 1u;
+    const bool ok = lane == 0 || lane == 63 || full != 0u;
+    st.openh = (st.openh << 1) | (__ballot(ok) == ~0ull ? 1ull : 0ull);
+  }
+  const real idy2 = g.idy2, hidy = g.hidy;
+  real Ou[NS], Ov[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const int r = t - 2 * l - 1;  // row of level l + 1 computed now
+    Ou[l] = Ov[l] = real(0);
+    const bool need = r >= 0 && r < g.ny && r >= jb - (NS - l - 1) && r < je + (NS - l - 1);  // uniform
+    if (need) {
+      const real x = st.W1u[l], y = st.W1v[l];
+      const bool atN = r == g.ny - 1, atS = r == 0;  // the wall row is its own mirror neighbour
+      real xn = st.W2u[l], xs = st.W0u[l], yn = st.W2v[l], ys = st.W0v[l];
+      if (atN || atS) {  // uniform, two rows of the domain
+        xn = atN ? x : xn;
+        xs = atS ? x : xs;
+        yn = atN ? y : yn;
+        ys = atS ? y : ys;
+      }
+      const real xe = dpp_from_right(x), xw = dpp_from_left(x);
+      const real ye = dpp_from_right(y), yw = dpp_from_left(y);
+      const real id2 = st.mi[2 * l + 1], tr = st.mt[2 * l + 1];
+      real Lx, Ly;
+      if ((st.openh >> (2 * l)) & 1ull) {
+        Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
+        Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+      } else {
+        const int b = 2 * l + 1;
+        const unsigned c = (st.hist >> b) & 1u;
+        const bool fe_on = (c & (hE >> b)) & 1u, fw_on = (c & (hW >> b)) & 1u;
+        const bool fn_on = c & (atN ? 1u : (st.hist >> (b - 1)) & 1u);
+        const bool fs_on = c & (atS ? 1u : (st.hist >> (b + 1)) & 1u);
+        Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+        Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+      }
+      Ou[l] = nu_update(x, Lx, nu_h, dts);
+      Ov[l] = nu_update(y, Ly, nu_h, dts);
+    }
+  }
+  // the new rows enter the windows of the level above
+#pragma unroll
+  for (int l = 1; l < NS; ++l) {
+    st.W0u[l] = st.W1u[l];
+    st.W1u[l] = st.W2u[l];
+    st.W2u[l] = Ou[l - 1];
+    st.W0v[l] = st.W1v[l];
+    st.W1v[l] = st.W2v[l];
+    st.W2v[l] = Ov[l - 1];
+  }
+  const int rs = t - 2 * NS + 1;  // level NS row
+  if (rs >= jb && rs < je && store_lane) {
+    uop[rs * g.nx + io] = Ou[NS - 1];
+    vop[rs * g.nx + io] = Ov[NS - 1];
+  }
+}
+
+template <typename real, int NS>
+__global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __restrict__ u,
+                                                   const real* __restrict__ v, real* __restrict__ uo,
+                                                   real* __restrict__ vo, real nu_h, real dts, int chunk) {
+  constexpr int OX = 64 - 2 * NS;
+  static_assert(2 * NS + 1 <= 32, "wet history is one 32-bit word");
+  const int lane = threadIdx.x & 63;
+  const int ntx = (g.nx + OX - 1) / OX, nch = (g.ny + chunk - 1) / chunk;
+  const long long ntiles = (long long)ntx * nch * g.nz;
+  long long t = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (t >= ntiles) return;  // uniform per wave
+  const int tx = __builtin_amdgcn_readfirstlane((int)(t % ntx));
+  t /= ntx;
+  const int ch = __builtin_amdgcn_readfirstlane((int)(t % nch));
+  const int k = __builtin_amdgcn_readfirstlane((int)(t / nch));
+  int i = (tx * OX - NS + lane) % g.nx;
+  if (i < 0) i += g.nx;
+  const int jb = ch * chunk, je = jb + chunk < g.ny ? jb + chunk : g.ny;
+  const int qs = jb - NS > 0 ? jb - NS : 0, qe = je + NS < g.ny ? je + NS : g.ny;  // input rows [qs, qe)
+  const long long plane = (long long)k * g.n2;
+  const real* __restrict__ up = u + plane;
+  const real* __restrict__ vp = v + plane;
+  real* __restrict__ uop = uo + plane;
+  real* __restrict__ vop = vo + plane;
+  const int io = tx * OX + (lane - NS);
+  const bool store_lane = lane >= NS && lane < 64 - NS && io < g.nx;
+  NuStream<real, NS> st;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) st.W0u[l] = st.W1u[l] = st.W2u[l] = st.W0v[l] = st.W1v[l] = st.W2v[l] = real(0);
+#pragma unroll
+  for (int l = 0; l < 2 * NS; ++l) st.mi[l] = st.mt[l] = real(0);
+#pragma unroll
+  for (int q = 0; q < kNuAhead; ++q) {
+    st.qu[q] = st.qv[q] = st.qi[q] = st.qt[q] = real(0);
+    st.qm[q] = 0;
+    if (qs + q < qe) {
+      const int c2 = (qs + q) * g.nx + i;
+      st.qu[q] = up[c2];
+      st.qv[q] = vp[c2];
+      st.qm[q] = k < g.nwet[c2] ? 1 : 0;
+      st.qi[q] = g.idx2[qs + q];
+      st.qt[q] = g.tanR[qs + q];
+    }
+  }
+  st.hist = 0;
+  st.openh = 0;
+  const int tend = qe + 2 * NS - 1;
+  static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
+  for (int t0 = qs; t0 < tend; t0 += kNuAhead) {
+    nu_stream_step<real, NS, 0>(st, g, t0, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    if (t0 + 1 >= tend) break;
+    nu_stream_step<real, NS, 1>(st, g, t0 + 1, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    if (t0 + 2 >= tend) break;
+    nu_stream_step<real, NS, 2>(st, g, t0 + 2, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    if (t0 + 3 >= tend) break;
+    nu_stream_step<real, NS, 3>(st, g, t0 + 3, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    if (t0 + 4 >= tend) break;
+    nu_stream_step<real, NS, 4>(st, g, t0 + 4, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    if (t0 + 5 >= tend) break;
+    nu_stream_step<real, NS, 5>(st, g, t0 + 5, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+  }
+}
+
 template <typename real>
 struct NuTile;
 template <>
@@ -1821,6 +2008,19 @@ static void launch_nu_strip(const Geo<real>& g, const real* u, const real* v, re
 template <typename real>
 void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
                      int nsub, hipStream_t s) {
+  if (nsub >= 2 && nsub <= 4 && g.nu_chunk > 0) {
+    const int chunk = g.nu_chunk;
+    auto go = [&](auto kern, int ns) {
+      const int OX = 64 - 2 * ns;
+      const long long waves = (long long)((g.nx + OX - 1) / OX) * ((g.ny + chunk - 1) / chunk) * g.nz;
+      const unsigned blocks = (unsigned)((waves + 3) / 4);
+      hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, g, u, v, uo, vo, nu_h, dts, chunk);
+    };
+    if (nsub == 2) go(k_nu_stream<real, 2>, 2);
+    else if (nsub == 3) go(k_nu_stream<real, 3>, 3);
+    else go(k_nu_stream<real, 4>, 4);
+    return;
+  }
   using T = NuTile<real>;
   if (nsub <= 3)
     launch_nu_strip<real, T::H, 3>(g, u, v, uo, vo, nu_h, dts, nsub, s);
