@@ -1925,6 +1925,13 @@ __global__ __launch_bounds__(256) void k_final(Geo<real> g, Phys<real> P, const 
   ZM_TILE_COORDS
   const int c2 = j * g.nx + i;
   const bool wc = k < g.nwet[c2];
+  if (!wc) {  // land and ghost nodes hold their pre-step values
+    uo[c] = uin[c];
+    vo[c] = (j == 0 || j == g.ny - 1) ? real(0) : vin[c];
+    To[c] = Tin[c];
+    So[c] = Sin[c];
+    return;
+  }
   real u = u3[c] + (ubt[c2] - ubt0[c2]);
   real v = v3[c] + (vbt[c2] - vbt0[c2]);
   real T = T3[c], S = S3[c];
@@ -1938,12 +1945,6 @@ __global__ __launch_bounds__(256) void k_final(Geo<real> g, Phys<real> P, const 
     v = wb * means[o + stride] + om * v;
     T = wb * means[o + 2 * stride] + om * T;
     S = wb * means[o + 3 * stride] + om * S;
-  }
-  if (!wc) {
-    u = uin[c];
-    v = vin[c];
-    T = Tin[c];
-    S = Sin[c];
   }
   if (j == 0 || j == g.ny - 1) v = real(0);
   uo[c] = u;
