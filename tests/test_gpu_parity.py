@@ -109,24 +109,33 @@ def test_residual_stages_match_oracle(dtype):
         assert e <= tol, (name, e)
 
 
-def test_blocked_nu_subcycle_equals_single_substeps():
-    print("case g4deg nu blocking float64", flush=True)
+@pytest.mark.parametrize("dtype,block,chunk", [("float64", "", ""), ("float32", "", ""), ("float32", "3", "8"),
+                                               ("float32", "2", "16"), ("float64", "4", "0")],
+                         ids=["f64-stream4", "f32-stream4", "f32-stream3-chunk8", "f32-stream2-chunk16",
+                              "f64-tile4"])
+def test_blocked_nu_subcycle_equals_single_substeps(dtype, block, chunk):
+    """The temporally blocked nu_h kernels (streaming chunks of `chunk` rows,
+    or the register-tile kernel for chunk 0) give the single-substep bits:
+    4-degree grid, 40 rows, so chunk boundaries and both walls are crossed."""
+    print(f"case g4deg nu blocking {dtype} block={block or 'max'} chunk={chunk or 'default'}", flush=True)
     case = make_case("g4deg")
-    s = None
     outs = []
-    for block in ("1", ""):
-        if block:
-            os.environ["ZHENMODE_NU_BLOCK"] = block
-        else:
-            os.environ.pop("ZHENMODE_NU_BLOCK", None)
-        step, init_state, _ = make_solver_global(case.grid, case.physics, case.dt, dtype="float64",
-                                                 **case.solver_kwargs)
-        if block == "":
-            assert step.solver.nu_substeps_per_launch() > 1
+    for single in (True, False):
+        env = {"ZHENMODE_NU_BLOCK": "1"} if single else {"ZHENMODE_NU_BLOCK": block, "ZHENMODE_NU_CHUNK": chunk}
+        for k, v in env.items():
+            if v:
+                os.environ[k] = v
+        try:
+            step, init_state, _ = make_solver_global(case.grid, case.physics, case.dt, dtype=dtype,
+                                                     **case.solver_kwargs)
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+        if not single:
+            assert step.solver.nu_substeps_per_launch() == (int(block) if block else 4)
         s = init_state(case.T_init, case.S_init)
         s = step(step(s))
         outs.append(to_numpy_state(s))
-    os.environ.pop("ZHENMODE_NU_BLOCK", None)
     for f in FIELDS:  # identical arithmetic per node: bit-equal
         np.testing.assert_array_equal(getattr(outs[1], f), getattr(outs[0], f), err_msg=f)
 
