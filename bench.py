@@ -341,11 +341,11 @@ def main():
                    "event_ms_per_step": ev_ms, "finite": finite,
                    "tiles": {"nu_block": tiles[0], "seg": tiles[1]}},
         "roofline": {"bound": "hbm",
-                     "kernel": (f"k_nu_strip ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
+                     "kernel": (f"k_nu_stream ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
                                 else "k_nu_substep (one nu_h Laplacian substep of u and v)"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_pmc_traffic(f"nu_strip_{args.case}_pmc.txt"),
+                     "traffic": read_pmc_traffic(f"nu_stream_{args.case}_pmc.txt"),
                      "bytes_per_launch": nu_bytes, "launch_ms": nu_ms,
                      "single_substep_launch_ms": nu_single_ms,
                      "single_substep_GBps": 4 * n3 * word / (nu_single_ms * 1e-3) / 1e9,
