@@ -51,6 +51,8 @@ struct Geo {
   const int* nwet;        // wet levels per column [n2]
   int seg;                // levels per thread in the column-marching kernels
   int nu_chunk;           // rows per wave of the streaming nu_h kernel (0: register-tile kernel)
+  const unsigned long long* nu_wet;  // streaming nu_h kernel: wet words [nz][chunks][nx]
+  const real* nu_met;                // streaming nu_h kernel: (1/dx^2, tan(lat)/R) [ny + 16][2]
 };
 
 // Scalars and 2-D fields of the physics.

@@ -232,6 +232,31 @@ struct Solver : SolverBase {
     g.seg = 10;
     g.nu_chunk = 32;
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
+    g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
+    {
+      // streaming nu_h kernel: padded row metrics and per-(level, chunk, column) wet words
+      std::vector<double> met(2 * (size_t)(ny + 2 * zm::kNuMetPad), 0.0);
+      for (int j = 0; j < ny; ++j) {
+        met[2 * (j + zm::kNuMetPad)] = idx2[j];
+        met[2 * (j + zm::kNuMetPad) + 1] = tanR[j];
+      }
+      g.nu_met = upload(met);
+      const int chunk = g.nu_chunk > 0 ? g.nu_chunk : 32;
+      const int nch = (ny + chunk - 1) / chunk;
+      std::vector<unsigned long long> wet((size_t)nz * nch * nx, 0ull);
+      for (int ch = 0; ch < nch; ++ch)
+        for (int b = 0; b < 64; ++b) {
+          const int j = ch * chunk - zm::kNuWetBase + b;
+          if (j < 0 || j >= ny) continue;
+          for (int i = 0; i < nx; ++i) {
+            const int nwc = std::min(c.nwet[(size_t)j * nx + i], nz);
+            for (int k = 0; k < nwc; ++k) wet[((size_t)k * nch + ch) * nx + i] |= 1ull << b;
+          }
+        }
+      unsigned long long* dw = dev_alloc<unsigned long long>(wet.size());
+      ZM_HIP(hipMemcpy(dw, wet.data(), wet.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+      g.nu_wet = dw;
+    }
     dzn64 = dev_alloc<double>(nz);
     ZM_HIP(hipMemcpy(dzn64, c.dz_node, nz * sizeof(double), hipMemcpyHostToDevice));
     dzi64 = dev_alloc<double>(nz > 1 ? nz - 1 : 1);

@@ -369,27 +369,66 @@ __global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __res
 // adjacent lanes (DPP), so as in k_nu_strip lanes [NS, 64 - NS) are exact.
 // Every node receives the arithmetic of lap_h / k_nu_substep (lap_open where
 // the row's interior lanes have all four faces open, lap_strip otherwise).
+//
+// Per-wave data prepared at set-up (never written by a step):
+//  * nu_wet[(k * nch + ch) * nx + i]: bit b = node (i, jb - kNuWetBase + b, k)
+//    is wet (0 outside the domain), jb = ch * chunk; the four face gates of
+//    every row follow from it once per wave, and the rows whose interior
+//    lanes are all open from one wave-wide AND;
+//  * nu_met[2 (j + kNuMetPad) + {0, 1}] = 1/dx^2, tan(lat)/R of row j, padded
+//    so that every march step reads its rows' metrics unconditionally through
+//    scalar loads.
 constexpr int kNuAhead = 6;  // a multiple of the period of the three-row window
+
+template <typename real>
+using nu_met_ptr = const __attribute__((address_space(4))) real*;  // read through the scalar cache
 
 template <typename real, int NS>
 struct NuStream {
   real W0u[NS], W1u[NS], W2u[NS];  // level l: rows (south, centre, north) of its next computation
   real W0v[NS], W1v[NS], W2v[NS];
   real qu[kNuAhead], qv[kNuAhead];  // prefetched input rows
-  int qm[kNuAhead];                 // their wet bit at this level
-  real qi[kNuAhead], qt[kNuAhead];  // their row metrics 1/dx^2, tan(lat)/R
-  real mi[2 * NS], mt[2 * NS];      // metrics of rows t, t - 1, ..., t - 2 NS + 1
-  unsigned hist;                    // bit b: wet at row t - b (this lane)
-  unsigned long long openh;         // bit b: row t - 1 - b fully open (uniform)
 };
 
+// face gates of this lane, bit b = row wb + b (the wall row is its own mirror)
+struct NuGates {
+  unsigned long long e, w, n, s;
+};
+
+// One level-row of the streaming kernel: row r of level l + 1 from the window
+// (xs, x, xn) of level l; b = r's bit in the gate words.
+template <typename real>
+__device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y, real yn, real ys, bool open,
+                                               const NuGates& G, int b, real id2, real tr, real idy2, real hidy,
+                                               real nu_h, real dts, real& ou, real& ov) {
+  const real xe = dpp_from_right(x), xw = dpp_from_left(x);
+  const real ye = dpp_from_right(y), yw = dpp_from_left(y);
+  real Lx, Ly;
+  if (open) {
+    Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
+    Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+  } else {
+    const bool fe_on = (G.e >> b) & 1ull, fw_on = (G.w >> b) & 1ull;
+    const bool fn_on = (G.n >> b) & 1ull, fs_on = (G.s >> b) & 1ull;
+    Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+    Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+  }
+  ou = nu_update(x, Lx, nu_h, dts);
+  ov = nu_update(y, Ly, nu_h, dts);
+}
+
+// Uniform per-wave bounds of the streaming kernel.
+struct NuRange {
+  int wb, qe, jb, je;
+};
+
+// One march step t.
 template <typename real, int NS, int SLOT>
-__device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, int t, int k, int i,
-                                               int lane, int qe, int jb, int je, const real* __restrict__ up,
-                                               const real* __restrict__ vp, real* __restrict__ uop,
-                                               real* __restrict__ vop, bool store_lane, int io, real nu_h,
-                                               real dts) {
-  const bool have = t < qe;  // uniform
+__device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> met,
+                                               const NuGates& G, unsigned long long openw, int t, const NuRange& R,
+                                               const real* __restrict__ up, const real* __restrict__ vp,
+                                               real* __restrict__ uop, real* __restrict__ vop, int i, int io,
+                                               bool store_lane, real nu_h, real dts) {
   // row t enters the level-0 window
   st.W0u[0] = st.W1u[0];
   st.W1u[0] = st.W2u[0];
@@ -397,70 +436,26 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
   st.W0v[0] = st.W1v[0];
   st.W1v[0] = st.W2v[0];
   st.W2v[0] = st.qv[SLOT];
-  const unsigned mc = have ? (unsigned)st.qm[SLOT] : 0u;
-#pragma unroll
-  for (int l = 2 * NS - 1; l > 0; --l) {
-    st.mi[l] = st.mi[l - 1];
-    st.mt[l] = st.mt[l - 1];
-  }
-  st.mi[0] = st.qi[SLOT];
-  st.mt[0] = st.qt[SLOT];
-  // prefetch row t + kNuAhead into this slot
-  if (t + kNuAhead < qe) {
-    const int c2 = (t + kNuAhead) * g.nx + i;
-    st.qu[SLOT] = up[c2];
-    st.qv[SLOT] = vp[c2];
-    st.qm[SLOT] = k < g.nwet[c2] ? 1 : 0;
-    st.qi[SLOT] = g.idx2[t + kNuAhead];
-    st.qt[SLOT] = g.tanR[t + kNuAhead];
-  }
-  st.hist = (st.hist << 1) | mc;
-  const unsigned hE = (unsigned)dpp_from_right((int)st.hist), hW = (unsigned)dpp_from_left((int)st.hist);
-  // row t - 1 has all its gates now: open flag over the interior lanes
-  {
-    const int r = t - 1;
-    const unsigned c = (st.hist >> 1) & 1u;
-    const unsigned n = r == g.ny - 1 ? 1u : st.hist & 1u;
-    const unsigned s = r == 0 ? 1u : (st.hist >> 2) & 1u;
-    const unsigned full = c & (hE >> 1) & (hW >> 1) & n & s & 1u;
-    const bool ok = lane == 0 || lane == 63 || full != 0u;
-    st.openh = (st.openh << 1) | (__ballot(ok) == ~0ull ? 1ull : 0ull);
+  // prefetch row t + kNuAhead into this slot (uniform row bases)
+  if (t + kNuAhead < R.qe) {
+    const int ro = (t + kNuAhead) * g.nx;
+    st.qu[SLOT] = (up + ro)[(unsigned)i];
+    st.qv[SLOT] = (vp + ro)[(unsigned)i];
   }
   const real idy2 = g.idy2, hidy = g.hidy;
   real Ou[NS], Ov[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     const int r = t - 2 * l - 1;  // row of level l + 1 computed now
+    const real id2 = met[2 * (r + kNuMetPad)], tr = met[2 * (r + kNuMetPad) + 1];
     Ou[l] = Ov[l] = real(0);
-    const bool need = r >= 0 && r < g.ny && r >= jb - (NS - l - 1) && r < je + (NS - l - 1);  // uniform
+    const bool need = r >= 0 && r < g.ny && r >= R.jb - (NS - l - 1) && r < R.je + (NS - l - 1);  // uniform
     if (need) {
       const real x = st.W1u[l], y = st.W1v[l];
       const bool atN = r == g.ny - 1, atS = r == 0;  // the wall row is its own mirror neighbour
-      real xn = st.W2u[l], xs = st.W0u[l], yn = st.W2v[l], ys = st.W0v[l];
-      if (atN || atS) {  // uniform, two rows of the domain
-        xn = atN ? x : xn;
-        xs = atS ? x : xs;
-        yn = atN ? y : yn;
-        ys = atS ? y : ys;
-      }
-      const real xe = dpp_from_right(x), xw = dpp_from_left(x);
-      const real ye = dpp_from_right(y), yw = dpp_from_left(y);
-      const real id2 = st.mi[2 * l + 1], tr = st.mt[2 * l + 1];
-      real Lx, Ly;
-      if ((st.openh >> (2 * l)) & 1ull) {
-        Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
-        Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
-      } else {
-        const int b = 2 * l + 1;
-        const unsigned c = (st.hist >> b) & 1u;
-        const bool fe_on = (c & (hE >> b)) & 1u, fw_on = (c & (hW >> b)) & 1u;
-        const bool fn_on = c & (atN ? 1u : (st.hist >> (b - 1)) & 1u);
-        const bool fs_on = c & (atS ? 1u : (st.hist >> (b + 1)) & 1u);
-        Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
-        Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
-      }
-      Ou[l] = nu_update(x, Lx, nu_h, dts);
-      Ov[l] = nu_update(y, Ly, nu_h, dts);
+      const int b = r - R.wb;
+      nu_stream_node(x, atN ? x : st.W2u[l], atS ? x : st.W0u[l], y, atN ? y : st.W2v[l], atS ? y : st.W0v[l],
+                     ((openw >> b) & 1ull) != 0ull, G, b, id2, tr, idy2, hidy, nu_h, dts, Ou[l], Ov[l]);
     }
   }
   // the new rows enter the windows of the level above
@@ -474,9 +469,10 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
     st.W2v[l] = Ov[l - 1];
   }
   const int rs = t - 2 * NS + 1;  // level NS row
-  if (rs >= jb && rs < je && store_lane) {
-    uop[rs * g.nx + io] = Ou[NS - 1];
-    vop[rs * g.nx + io] = Ov[NS - 1];
+  if (rs >= R.jb && rs < R.je && store_lane) {
+    const int ro = rs * g.nx;
+    (uop + ro)[(unsigned)io] = Ou[NS - 1];
+    (vop + ro)[(unsigned)io] = Ov[NS - 1];
   }
 }
 
@@ -485,7 +481,7 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
                                                    const real* __restrict__ v, real* __restrict__ uo,
                                                    real* __restrict__ vo, real nu_h, real dts, int chunk) {
   constexpr int OX = 64 - 2 * NS;
-  static_assert(2 * NS + 1 <= 32, "wet history is one 32-bit word");
+  static_assert(NS <= kNuWetBase && 2 * NS - 1 < kNuMetPad, "wet word and metric padding cover NS substeps");
   const int lane = threadIdx.x & 63;
   const int ntx = (g.nx + OX - 1) / OX, nch = (g.ny + chunk - 1) / chunk;
   const long long ntiles = (long long)ntx * nch * g.nz;
@@ -499,6 +495,7 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   if (i < 0) i += g.nx;
   const int jb = ch * chunk, je = jb + chunk < g.ny ? jb + chunk : g.ny;
   const int qs = jb - NS > 0 ? jb - NS : 0, qe = je + NS < g.ny ? je + NS : g.ny;  // input rows [qs, qe)
+  const int wb = jb - kNuWetBase;
   const long long plane = (long long)k * g.n2;
   const real* __restrict__ up = u + plane;
   const real* __restrict__ vp = v + plane;
@@ -510,37 +507,55 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
 #pragma unroll
   for (int l = 0; l < NS; ++l) st.W0u[l] = st.W1u[l] = st.W2u[l] = st.W0v[l] = st.W1v[l] = st.W2v[l] = real(0);
 #pragma unroll
-  for (int l = 0; l < 2 * NS; ++l) st.mi[l] = st.mt[l] = real(0);
-#pragma unroll
   for (int q = 0; q < kNuAhead; ++q) {
-    st.qu[q] = st.qv[q] = st.qi[q] = st.qt[q] = real(0);
-    st.qm[q] = 0;
+    st.qu[q] = st.qv[q] = real(0);
     if (qs + q < qe) {
-      const int c2 = (qs + q) * g.nx + i;
-      st.qu[q] = up[c2];
-      st.qv[q] = vp[c2];
-      st.qm[q] = k < g.nwet[c2] ? 1 : 0;
-      st.qi[q] = g.idx2[qs + q];
-      st.qt[q] = g.tanR[qs + q];
+      const int ro = (qs + q) * g.nx;
+      st.qu[q] = (up + ro)[i];
+      st.qv[q] = (vp + ro)[i];
     }
   }
-  st.hist = 0;
-  st.openh = 0;
+  // face gates: east/west from the adjacent lanes' wet words, north/south
+  // from the adjacent bits; the wall rows are their own mirror neighbours
+  const unsigned long long W = g.nu_wet[((long long)k * nch + ch) * g.nx + i];
+  const int bn = g.ny - 1 - wb, bs = -wb;
+  const unsigned long long wallN = bn >= 0 && bn < 64 ? 1ull << bn : 0ull;
+  const unsigned long long wallS = bs >= 0 && bs < 64 ? 1ull << bs : 0ull;
+  NuGates G;
+  G.e = W & dpp_from_right(W);
+  G.w = W & dpp_from_left(W);
+  G.n = W & ((W >> 1) | wallN);
+  G.s = W & ((W << 1) | wallS);
+  // rows where every interior lane has all four faces open (uniform)
+  unsigned long long full = (lane == 0 || lane == 63) ? ~0ull : (G.e & G.w & G.n & G.s);
+  for (int o = 1; o < 64; o <<= 1) full &= (unsigned long long)__shfl_xor((long long)full, o, 64);
+  const unsigned long long openw =
+      ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(full >> 32)) << 32) |
+      (unsigned)__builtin_amdgcn_readfirstlane((int)(full & 0xffffffffull));
+  const nu_met_ptr<real> met = (nu_met_ptr<real>)g.nu_met;
+  NuRange R;
+  R.wb = wb;
+  R.qe = qe;
+  R.jb = jb;
+  R.je = je;
   const int tend = qe + 2 * NS - 1;
   static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
+#define ZM_NU_STEP(S) \
+  nu_stream_step<real, NS, S>(st, g, met, G, openw, t0 + S, R, up, vp, uop, vop, i, io, store_lane, nu_h, dts)
   for (int t0 = qs; t0 < tend; t0 += kNuAhead) {
-    nu_stream_step<real, NS, 0>(st, g, t0, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    ZM_NU_STEP(0);
     if (t0 + 1 >= tend) break;
-    nu_stream_step<real, NS, 1>(st, g, t0 + 1, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    ZM_NU_STEP(1);
     if (t0 + 2 >= tend) break;
-    nu_stream_step<real, NS, 2>(st, g, t0 + 2, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    ZM_NU_STEP(2);
     if (t0 + 3 >= tend) break;
-    nu_stream_step<real, NS, 3>(st, g, t0 + 3, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    ZM_NU_STEP(3);
     if (t0 + 4 >= tend) break;
-    nu_stream_step<real, NS, 4>(st, g, t0 + 4, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    ZM_NU_STEP(4);
     if (t0 + 5 >= tend) break;
-    nu_stream_step<real, NS, 5>(st, g, t0 + 5, k, i, lane, qe, jb, je, up, vp, uop, vop, store_lane, io, nu_h, dts);
+    ZM_NU_STEP(5);
   }
+#undef ZM_NU_STEP
 }
 
 template <typename real>
