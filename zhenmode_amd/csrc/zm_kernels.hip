@@ -417,15 +417,18 @@ __device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y,
   ov = nu_update(y, Ly, nu_h, dts);
 }
 
-// Uniform per-wave bounds of the streaming kernel.
+// Uniform per-wave bounds of the streaming kernel: level l computes its row
+// at step t iff 0 <= t - c[l] < n[l].
+template <int NS>
 struct NuRange {
   int wb, qe, jb, je;
+  int c[NS], n[NS];
 };
 
 // One march step t.
 template <typename real, int NS, int SLOT>
 __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> met,
-                                               const NuGates& G, unsigned long long openw, int t, const NuRange& R,
+                                               const NuGates& G, unsigned long long openw, int t, const NuRange<NS>& R,
                                                const real* __restrict__ up, const real* __restrict__ vp,
                                                real* __restrict__ uop, real* __restrict__ vop, int i, int io,
                                                bool store_lane, real nu_h, real dts) {
@@ -448,13 +451,23 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
   for (int l = 0; l < NS; ++l) {
     const int r = t - 2 * l - 1;  // row of level l + 1 computed now
     const real id2 = met[2 * (r + kNuMetPad)], tr = met[2 * (r + kNuMetPad) + 1];
-    Ou[l] = Ov[l] = real(0);
-    const bool need = r >= 0 && r < g.ny && r >= R.jb - (NS - l - 1) && r < R.je + (NS - l - 1);  // uniform
-    if (need) {
-      const real x = st.W1u[l], y = st.W1v[l];
-      const bool atN = r == g.ny - 1, atS = r == 0;  // the wall row is its own mirror neighbour
+    // a row outside the level's range is never read by a computed row of
+    // the level above: it keeps the centre value
+    Ou[l] = st.W1u[l];
+    Ov[l] = st.W1v[l];
+    if ((unsigned)(t - R.c[l]) < (unsigned)R.n[l]) {  // uniform
+      // the wall row is its own mirror neighbour; the slot written is not
+      // read by another computed row (W0 leaves the window, W2 holds row ny)
+      if (__builtin_expect(r == 0, 0)) {
+        st.W0u[l] = st.W1u[l];
+        st.W0v[l] = st.W1v[l];
+      }
+      if (__builtin_expect(r == g.ny - 1, 0)) {
+        st.W2u[l] = st.W1u[l];
+        st.W2v[l] = st.W1v[l];
+      }
       const int b = r - R.wb;
-      nu_stream_node(x, atN ? x : st.W2u[l], atS ? x : st.W0u[l], y, atN ? y : st.W2v[l], atS ? y : st.W0v[l],
+      nu_stream_node(st.W1u[l], st.W2u[l], st.W0u[l], st.W1v[l], st.W2v[l], st.W0v[l],
                      ((openw >> b) & 1ull) != 0ull, G, b, id2, tr, idy2, hidy, nu_h, dts, Ou[l], Ov[l]);
     }
   }
@@ -533,11 +546,18 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
       ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(full >> 32)) << 32) |
       (unsigned)__builtin_amdgcn_readfirstlane((int)(full & 0xffffffffull));
   const nu_met_ptr<real> met = (nu_met_ptr<real>)g.nu_met;
-  NuRange R;
+  NuRange<NS> R;
   R.wb = wb;
   R.qe = qe;
   R.jb = jb;
   R.je = je;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {  // rows [max(jb - (NS - l - 1), 0), min(je + NS - l - 1, ny)) at step r + 2 l + 1
+    const int lo = jb - (NS - l - 1) > 0 ? jb - (NS - l - 1) : 0;
+    const int hi = je + NS - l - 1 < g.ny ? je + NS - l - 1 : g.ny;
+    R.c[l] = __builtin_amdgcn_readfirstlane(lo + 2 * l + 1);
+    R.n[l] = __builtin_amdgcn_readfirstlane(hi - lo);
+  }
   const int tend = qe + 2 * NS - 1;
   static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
 #define ZM_NU_STEP(S) \
