@@ -396,22 +396,30 @@ struct NuGates {
 };
 
 // One level-row of the streaming kernel: row r of level l + 1 from the window
-// (xs, x, xn) of level l; b = r's bit in the gate words.
+// (xs, x, xn) of level l; b = r's bit in the gate words.  cls (uniform): 1 =
+// every interior lane has all four faces open, 2 = every face is closed
+// (only the metric term remains: lap_strip with zero face differences), 0 =
+// mixed.
 template <typename real>
-__device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y, real yn, real ys, bool open,
+__device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y, real yn, real ys, int cls,
                                                const NuGates& G, int b, real id2, real tr, real idy2, real hidy,
                                                real nu_h, real dts, real& ou, real& ov) {
-  const real xe = dpp_from_right(x), xw = dpp_from_left(x);
-  const real ye = dpp_from_right(y), yw = dpp_from_left(y);
   real Lx, Ly;
-  if (open) {
-    Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
-    Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+  if (cls == 2) {
+    Lx = lap_sum(real(0), real(0), xn - xs, id2, idy2, hidy, tr);
+    Ly = lap_sum(real(0), real(0), yn - ys, id2, idy2, hidy, tr);
   } else {
-    const bool fe_on = (G.e >> b) & 1ull, fw_on = (G.w >> b) & 1ull;
-    const bool fn_on = (G.n >> b) & 1ull, fs_on = (G.s >> b) & 1ull;
-    Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
-    Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+    const real xe = dpp_from_right(x), xw = dpp_from_left(x);
+    const real ye = dpp_from_right(y), yw = dpp_from_left(y);
+    if (cls == 1) {
+      Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
+      Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+    } else {
+      const bool fe_on = (G.e >> b) & 1ull, fw_on = (G.w >> b) & 1ull;
+      const bool fn_on = (G.n >> b) & 1ull, fs_on = (G.s >> b) & 1ull;
+      Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+      Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+    }
   }
   ou = nu_update(x, Lx, nu_h, dts);
   ov = nu_update(y, Ly, nu_h, dts);
@@ -428,7 +436,8 @@ struct NuRange {
 // One march step t.
 template <typename real, int NS, int SLOT>
 __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> met,
-                                               const NuGates& G, unsigned long long openw, int t, const NuRange<NS>& R,
+                                               const NuGates& G, unsigned long long openw, unsigned long long dryw,
+                                               int t, const NuRange<NS>& R,
                                                const real* __restrict__ up, const real* __restrict__ vp,
                                                real* __restrict__ uop, real* __restrict__ vop, int i, int io,
                                                bool store_lane, real nu_h, real dts) {
@@ -467,8 +476,9 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
         st.W2v[l] = st.W1v[l];
       }
       const int b = r - R.wb;
-      nu_stream_node(st.W1u[l], st.W2u[l], st.W0u[l], st.W1v[l], st.W2v[l], st.W0v[l],
-                     ((openw >> b) & 1ull) != 0ull, G, b, id2, tr, idy2, hidy, nu_h, dts, Ou[l], Ov[l]);
+      const int cls = ((openw >> b) & 1ull) ? 1 : (((dryw >> b) & 1ull) ? 2 : 0);
+      nu_stream_node(st.W1u[l], st.W2u[l], st.W0u[l], st.W1v[l], st.W2v[l], st.W0v[l], cls, G, b, id2, tr, idy2,
+                     hidy, nu_h, dts, Ou[l], Ov[l]);
     }
   }
   // the new rows enter the windows of the level above
@@ -539,12 +549,20 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   G.w = W & dpp_from_left(W);
   G.n = W & ((W >> 1) | wallN);
   G.s = W & ((W << 1) | wallS);
-  // rows where every interior lane has all four faces open (uniform)
-  unsigned long long full = (lane == 0 || lane == 63) ? ~0ull : (G.e & G.w & G.n & G.s);
-  for (int o = 1; o < 64; o <<= 1) full &= (unsigned long long)__shfl_xor((long long)full, o, 64);
+  // rows where every interior lane has all four faces open, or all closed (uniform)
+  const bool edge = lane == 0 || lane == 63;
+  unsigned long long full = edge ? ~0ull : (G.e & G.w & G.n & G.s);
+  unsigned long long some = edge ? 0ull : (G.e | G.w | G.n | G.s);
+  for (int o = 1; o < 64; o <<= 1) {
+    full &= (unsigned long long)__shfl_xor((long long)full, o, 64);
+    some |= (unsigned long long)__shfl_xor((long long)some, o, 64);
+  }
   const unsigned long long openw =
       ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(full >> 32)) << 32) |
       (unsigned)__builtin_amdgcn_readfirstlane((int)(full & 0xffffffffull));
+  const unsigned long long dryw =
+      ~(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(some >> 32)) << 32) |
+        (unsigned)__builtin_amdgcn_readfirstlane((int)(some & 0xffffffffull)));
   const nu_met_ptr<real> met = (nu_met_ptr<real>)g.nu_met;
   NuRange<NS> R;
   R.wb = wb;
@@ -561,7 +579,7 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const int tend = qe + 2 * NS - 1;
   static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
 #define ZM_NU_STEP(S) \
-  nu_stream_step<real, NS, S>(st, g, met, G, openw, t0 + S, R, up, vp, uop, vop, i, io, store_lane, nu_h, dts)
+  nu_stream_step<real, NS, S>(st, g, met, G, openw, dryw, t0 + S, R, up, vp, uop, vop, i, io, store_lane, nu_h, dts)
   for (int t0 = qs; t0 < tend; t0 += kNuAhead) {
     ZM_NU_STEP(0);
     if (t0 + 1 >= tend) break;
