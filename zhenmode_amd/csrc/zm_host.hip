@@ -230,7 +230,7 @@ struct Solver : SolverBase {
     ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
     g.nwet = nw;
     g.seg = 10;
-    g.nu_chunk = 32;
+    g.nu_chunk = 36;
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
     g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
     {

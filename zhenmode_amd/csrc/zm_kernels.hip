@@ -395,6 +395,32 @@ struct NuGates {
   unsigned long long e, w, n, s;
 };
 
+// Raw buffer access (gfx9 descriptor word 3 = 0x00020000, stride 0): the base
+// address sits in an SGPR descriptor, each access adds a 32-bit per-lane
+// byte offset and a uniform byte offset (the row), so no per-lane 64-bit
+// pointer is kept.  Offsets stay below 2^31 bytes (one level plane).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ float buf_ld(float*, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double buf_ld(double*, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(float x, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_st(double x, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  using b64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64, x), r, voff, soff, 0);
+}
+
+// Buffer descriptors of the streaming kernel's input and output level planes.
+struct NuBufs {
+  __amdgpu_buffer_rsrc_t u, v, uo, vo;
+};
+
 // One level-row of the streaming kernel: row r of level l + 1 from the window
 // (xs, x, xn) of level l; b = r's bit in the gate words.  cls (uniform): 1 =
 // every interior lane has all four faces open, 2 = every face is closed
@@ -438,9 +464,8 @@ template <typename real, int NS, int SLOT>
 __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> met,
                                                const NuGates& G, unsigned long long openw, unsigned long long dryw,
                                                int t, const NuRange<NS>& R,
-                                               const real* __restrict__ up, const real* __restrict__ vp,
-                                               real* __restrict__ uop, real* __restrict__ vop, int i, int io,
-                                               bool store_lane, real nu_h, real dts) {
+                                               const NuBufs& B, unsigned ib, unsigned iob, bool store_lane,
+                                               real nu_h, real dts) {
   // row t enters the level-0 window
   st.W0u[0] = st.W1u[0];
   st.W1u[0] = st.W2u[0];
@@ -450,9 +475,9 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
   st.W2v[0] = st.qv[SLOT];
   // prefetch row t + kNuAhead into this slot (uniform row bases)
   if (t + kNuAhead < R.qe) {
-    const int ro = (t + kNuAhead) * g.nx;
-    st.qu[SLOT] = (up + ro)[(unsigned)i];
-    st.qv[SLOT] = (vp + ro)[(unsigned)i];
+    const int ro = (t + kNuAhead) * g.nx * (int)sizeof(real);
+    st.qu[SLOT] = buf_ld((real*)nullptr, B.u, ib, ro);
+    st.qv[SLOT] = buf_ld((real*)nullptr, B.v, ib, ro);
   }
   const real idy2 = g.idy2, hidy = g.hidy;
   real Ou[NS], Ov[NS];
@@ -493,9 +518,9 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
   }
   const int rs = t - 2 * NS + 1;  // level NS row
   if (rs >= R.jb && rs < R.je && store_lane) {
-    const int ro = rs * g.nx;
-    (uop + ro)[(unsigned)io] = Ou[NS - 1];
-    (vop + ro)[(unsigned)io] = Ov[NS - 1];
+    const int ro = rs * g.nx * (int)sizeof(real);
+    buf_st(Ou[NS - 1], B.uo, iob, ro);
+    buf_st(Ov[NS - 1], B.vo, iob, ro);
   }
 }
 
@@ -520,12 +545,15 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const int qs = jb - NS > 0 ? jb - NS : 0, qe = je + NS < g.ny ? je + NS : g.ny;  // input rows [qs, qe)
   const int wb = jb - kNuWetBase;
   const long long plane = (long long)k * g.n2;
-  const real* __restrict__ up = u + plane;
-  const real* __restrict__ vp = v + plane;
-  real* __restrict__ uop = uo + plane;
-  real* __restrict__ vop = vo + plane;
+  NuBufs B;
+  B.u = buf_rsrc(u + plane);
+  B.v = buf_rsrc(v + plane);
+  B.uo = buf_rsrc(uo + plane);
+  B.vo = buf_rsrc(vo + plane);
   const int io = tx * OX + (lane - NS);
   const bool store_lane = lane >= NS && lane < 64 - NS && io < g.nx;
+  // 32-bit byte offsets of this lane's column (row bases stay uniform)
+  const unsigned ib = (unsigned)i * (unsigned)sizeof(real), iob = store_lane ? (unsigned)io * (unsigned)sizeof(real) : 0u;
   NuStream<real, NS> st;
 #pragma unroll
   for (int l = 0; l < NS; ++l) st.W0u[l] = st.W1u[l] = st.W2u[l] = st.W0v[l] = st.W1v[l] = st.W2v[l] = real(0);
@@ -533,9 +561,9 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   for (int q = 0; q < kNuAhead; ++q) {
     st.qu[q] = st.qv[q] = real(0);
     if (qs + q < qe) {
-      const int ro = (qs + q) * g.nx;
-      st.qu[q] = (up + ro)[i];
-      st.qv[q] = (vp + ro)[i];
+      const int ro = (qs + q) * g.nx * (int)sizeof(real);
+      st.qu[q] = buf_ld((real*)nullptr, B.u, ib, ro);
+      st.qv[q] = buf_ld((real*)nullptr, B.v, ib, ro);
     }
   }
   // face gates: east/west from the adjacent lanes' wet words, north/south
@@ -579,7 +607,7 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const int tend = qe + 2 * NS - 1;
   static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
 #define ZM_NU_STEP(S) \
-  nu_stream_step<real, NS, S>(st, g, met, G, openw, dryw, t0 + S, R, up, vp, uop, vop, i, io, store_lane, nu_h, dts)
+  nu_stream_step<real, NS, S>(st, g, met, G, openw, dryw, t0 + S, R, B, ib, iob, store_lane, nu_h, dts)
   for (int t0 = qs; t0 < tend; t0 += kNuAhead) {
     ZM_NU_STEP(0);
     if (t0 + 1 >= tend) break;
