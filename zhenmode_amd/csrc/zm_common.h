@@ -99,6 +99,14 @@ __device__ __forceinline__ void keep(const T& x) {
   asm volatile("" ::"v"(x));
 }
 
+// Read-only set-up table element through the scalar cache (constant address
+// space): uniform indices become scalar loads that the compiler may hoist past
+// the kernel's own stores.  Only for tables no kernel writes.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // Map a launch-order block id onto a tile id so that each of the 8 XCDs gets
 // one contiguous range of tiles (neighbouring tiles share halos through the
 // same L2).  Bijective for any block count.

@@ -106,7 +106,7 @@ __device__ __forceinline__ real d2dz2(const real* __restrict__ f, int c2, int n2
   auto val = [&](int kk) { return f[(long long)(kk < kb ? kk : kb) * n2 + c2]; };
   if (k == 0) return real(2) * (val(1) - val(0)) * g.ih0top2;
   if (k == g.nz - 1) return real(2) * (val(g.nz - 2) - val(g.nz - 1)) * g.ih0bot2;
-  return (val(k + 1) * g.hm[k - 1] + val(k - 1) * g.hp[k - 1] - val(k) * g.hmp[k - 1]) * g.iden[k - 1];
+  return (val(k + 1) * ldc(g.hm, k - 1) + val(k - 1) * ldc(g.hp, k - 1) - val(k) * ldc(g.hmp, k - 1)) * ldc(g.iden, k - 1);
 }
 
 // _gradient_conservative_3d (numerics/horizontal.py:126-151) of a field given
@@ -715,7 +715,7 @@ __device__ __forceinline__ real d2z_win(real xm, real xc, real xp, int k, int kb
   if (k == g.nz - 1) return kb >= g.nz - 1 ? real(2) * (xm - xc) * g.ih0bot2 : real(0);
   if (k > kb) return real(0);
   const real up = k < kb ? xp : xc;
-  return (up * g.hm[k - 1] + xm * g.hp[k - 1] - xc * g.hmp[k - 1]) * g.iden[k - 1];
+  return (up * ldc(g.hm, k - 1) + xm * ldc(g.hp, k - 1) - xc * ldc(g.hmp, k - 1)) * ldc(g.iden, k - 1);
 }
 
 // one-sided/centered d/dz of the ghost-filled column (numerics/vertical.py:267-272)
@@ -724,7 +724,7 @@ __device__ __forceinline__ real ddz_win(real xm, real xc, real xp, int k, int kb
   if (k == 0) return ((kb >= 1 ? xp : xc) - xc) * g.idz_top;
   if (k == g.nz - 1) return kb >= g.nz - 1 ? (xc - xm) * g.idz_bot : real(0);
   if (k > kb) return real(0);
-  return ((k < kb ? xp : xc) - xm) * g.iddz[k - 1];
+  return ((k < kb ? xp : xc) - xm) * ldc(g.iddz, k - 1);
 }
 
 // kappa_v vertical diffusion of a tracer at level k (physics/vertical.py:45-53):
@@ -736,9 +736,9 @@ __device__ __forceinline__ real vdiff_win(real xm, real xc, real xp, int k, int 
                                           const Geo<real>& g) {
   if (!cons) return kv * d2z_win(xm, xc, xp, k, kb, g);
   if (k >= nw) return real(0);
-  const real fup = k > 0 ? -kv * (xc - xm) * g.idzi[k - 1] : real(0);
-  const real fdn = k + 1 < nw ? -kv * (xp - xc) * g.idzi[k] : real(0);
-  return (fup - fdn) * g.idzn[k];
+  const real fup = k > 0 ? -kv * (xc - xm) * ldc(g.idzi, k - 1) : real(0);
+  const real fdn = k + 1 < nw ? -kv * (xp - xc) * ldc(g.idzi, k) : real(0);
+  return (fup - fdn) * ldc(g.idzn, k);
 }
 
 // minmod slope from wet neighbours (transport.py:199-213)
@@ -864,11 +864,11 @@ __global__ __launch_bounds__(256) void k_fz(Geo<real> g, const real* __restrict_
   Fz[(long long)g.nz * g.n2 + c2] = real(0);
   // dz of the next level is read before this level's store (the store could
   // alias the metric array, which would otherwise serialise every level)
-  real dzk = g.dzn[g.nz - 1];
+  real dzk = ldc(g.dzn, g.nz - 1);
 #pragma unroll 10
   for (int k = g.nz - 1; k >= 0; --k) {
     const long long plane = (long long)k * g.n2;
-    const real dz_next = g.dzn[k > 0 ? k - 1 : 0];
+    const real dz_next = ldc(g.dzn, k > 0 ? k - 1 : 0);
     acc = acc + div_h(u, v, plane, n, k, idx, idyc, cfN, cfS) * dzk;
     Fz[plane + c2] = acc;
     dzk = dz_next;
@@ -958,11 +958,11 @@ __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const re
         const real ot = tT[k * 64 + lane], os = tS[k * 64 + lane];
         const bool open = !P.local_conv || ((k < 64 ? (ub0 >> k) : (ub1 >> (k - 64))) & 1ull);
         if (k + 1 < nw && open) {
-          fT = (-kap * (tT[(k + 1) * 64 + lane] - ot)) * g.idzi[k];
-          fS = (-kap * (tS[(k + 1) * 64 + lane] - os)) * g.idzi[k];
+          fT = (-kap * (tT[(k + 1) * 64 + lane] - ot)) * ldc(g.idzi, k);
+          fS = (-kap * (tS[(k + 1) * 64 + lane] - os)) * ldc(g.idzi, k);
         }
-        et = (fTprev - fT) * g.idzn[k];
-        es = (fSprev - fS) * g.idzn[k];
+        et = (fTprev - fT) * ldc(g.idzn, k);
+        es = (fSprev - fS) * ldc(g.idzn, k);
         tT[k * 64 + lane] = ot + et * h;
         tS[k * 64 + lane] = os + es * h;
       }
@@ -1014,7 +1014,7 @@ __device__ __forceinline__ real adv_scalar(const real* __restrict__ X, const rea
     const real fzn = Fz[plane + g.n2 + n.c2];
     dn = fzn * (fzn > real(0) ? xc : X[plane + g.n2 + n.c2]);
   }
-  const real divz = (dn - up) * g.idzn[k];
+  const real divz = (dn - up) * ldc(g.idzn, k);
   return -(hdiv + divz);
 }
 
@@ -1137,8 +1137,8 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
         dnT = fz1 * (fz1 > real(0) ? Tc : Tp);
         dnS = fz1 * (fz1 > real(0) ? Sc : Sp);
       }
-      const real advT = -(hT + (dnT - upT) * g.idzn[k]);
-      const real advS = -(hS + (dnS - upS) * g.idzn[k]);
+      const real advT = -(hT + (dnT - upT) * ldc(g.idzn, k));
+      const real advS = -(hS + (dnS - upS) * ldc(g.idzn, k));
       const real convT = gated ? cTo : real(0);
       const real convS = gated ? cSo : real(0);
       dTdt = advT + dhT + dvT;
@@ -1235,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_pbc(Geo<real> g, Phys<real> P, const re
     keep(To);
     keep(So);
     const real r = k < nw ? rho_anom(To, So, P) : real(0);
-    acc = acc + real(kG) * (real(0.5) * (rprev + r)) * g.dz3[k - 1];
+    acc = acc + real(kG) * (real(0.5) * (rprev + r)) * ldc(g.dz3, k - 1);
     pbc[o] = acc;
     rprev = r;
   }
@@ -1272,8 +1272,8 @@ __global__ __launch_bounds__(256) void k_bt_rho(Geo<real> g, Phys<real> P, const
     real gxn, gyn;
     grad(k + 1, gxn, gyn);
     if (k + 1 < n.nwC) {
-      sx = sx + real(0.5) * (gxp + gxn) * g.dz3[k];
-      sy = sy + real(0.5) * (gyp + gyn) * g.dz3[k];
+      sx = sx + real(0.5) * (gxp + gxn) * ldc(g.dz3, k);
+      sy = sy + real(0.5) * (gyp + gyn) * ldc(g.dz3, k);
     }
     gxp = gxn;
     gyp = gyn;
@@ -1287,8 +1287,8 @@ __global__ __launch_bounds__(256) void k_bt_rho(Geo<real> g, Phys<real> P, const
     for (int k = 0; k + 1 < g.nz; ++k) {
       const long long o = (long long)(k + 1) * g.n2 + c2;
       const real un = u[o], vn = v[o];
-      su = su + real(0.5) * (up + un) * g.dznorm[k];
-      sv = sv + real(0.5) * (vp + vn) * g.dznorm[k];
+      su = su + real(0.5) * (up + un) * ldc(g.dznorm, k);
+      sv = sv + real(0.5) * (vp + vn) * ldc(g.dznorm, k);
       up = un;
       vp = vn;
     }
@@ -1322,7 +1322,7 @@ __device__ __forceinline__ real d_dz_filled(const real* __restrict__ f, int c2, 
   auto val = [&](int kk) { return f[(long long)(kk < kb ? kk : kb) * n2 + c2]; };
   if (k == 0) return (val(1) - val(0)) * g.idz_top;
   if (k == g.nz - 1) return (val(g.nz - 1) - val(g.nz - 2)) * g.idz_bot;
-  return (val(k + 1) - val(k - 1)) * g.iddz[k - 1];
+  return (val(k + 1) - val(k - 1)) * ldc(g.iddz, k - 1);
 }
 
 template <typename real>
