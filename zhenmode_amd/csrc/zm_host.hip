@@ -116,6 +116,7 @@ struct Solver : SolverBase {
   real *dT1, *dS1, *Tp, *Sp, *du1, *dv1, *up, *vp, *T2, *S2, *u2, *v2, *pbc, *au, *av, *axu, *axv;
   real *u3, *v3, *T3, *S3, *tmpA, *tmpB;
   real *Frx, *Fry, *ubt0, *vbt0, *ubt, *vbt, *etaw, *means;
+  real *ubt2, *vbt2, *etaw2;  // the other half of the barotropic ping-pong
   hipStream_t own = nullptr;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
@@ -368,7 +369,7 @@ struct Solver : SolverBase {
     Fz2 = dev_alloc<real>(n3 + n2);
     gate = dev_alloc<int>(n2);
     ubits = dev_alloc<unsigned long long>((size_t)2 * n2);
-    real** f2[] = {&Frx, &Fry, &ubt0, &vbt0, &ubt, &vbt, &etaw};
+    real** f2[] = {&Frx, &Fry, &ubt0, &vbt0, &ubt, &vbt, &etaw, &ubt2, &vbt2, &etaw2};
     for (real** p : f2) *p = dev_alloc<real>(n2);
     means = dev_alloc<real>((size_t)4 * nz * (2 * (nb > 0 ? nb : 1)));
     ZM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
@@ -598,11 +599,22 @@ struct Solver : SolverBase {
     zm::launch_copy((const real*)vbt0, vbt, n2, s);
   }
 
+  // n_sub forward-backward substeps, one launch each, alternating between
+  // (etaw, ubt, vbt) and (etaw2, ubt2, vbt2) so that the last substep lands in
+  // the former; the first reads eta_in, ubt0, vbt0 directly
   void barotropic(const zm::Phys<real>& Pf, const real* eta_in, hipStream_t s) {
-    bt_prepare(Pf, eta_in, s);
+    zm::launch_pbc(g, Pf, T3, S3, pbc, s);
+    zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, u3, v3, ubt0, vbt0, s);
+    const real *ei = eta_in, *ui = ubt0, *vi = vbt0;
     for (int q = 0; q < n_sub; ++q) {
-      zm::launch_bt_eta(g, Pf, etaw, ubt, vbt, dtH, s);
-      zm::launch_bt_vel(g, Pf, etaw, ubt, vbt, Frx, Fry, fdt, fden, dt_bt, drag, s);
+      const bool last_half = ((n_sub - 1 - q) & 1) == 0;
+      real* eo = last_half ? etaw : etaw2;
+      real* uo = last_half ? ubt : ubt2;
+      real* vo = last_half ? vbt : vbt2;
+      zm::launch_bt_sub(g, Pf, ei, ui, vi, eo, uo, vo, Frx, Fry, fdt, fden, dtH, dt_bt, drag, s);
+      ei = eo;
+      ui = uo;
+      vi = vo;
     }
   }
 
@@ -900,8 +912,7 @@ struct Solver : SolverBase {
           zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, u1, v1, T1, S1, tmpA, tmpB, u2, v2, s);
           break;
         case 4:
-          zm::launch_bt_eta(g, P, etaw, ubt, vbt, dtH, s);
-          zm::launch_bt_vel(g, P, etaw, ubt, vbt, Frx, Fry, fdt, fden, dt_bt, drag, s);
+          zm::launch_bt_sub(g, P, etaw, ubt, vbt, etaw2, ubt2, vbt2, Frx, Fry, fdt, fden, dtH, dt_bt, drag, s);
           break;
         case 14:
           zm::launch_dealias_x(nx, N1, N2, kept, (int)(ny * nz), twr, twi, au, av, axu, axv, s);

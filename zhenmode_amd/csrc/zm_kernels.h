@@ -106,6 +106,13 @@ struct BtMember {
   int row0;
 };
 
+// One whole barotropic substep in one launch, from (eta_in, u_in, v_in) into
+// separate output buffers (same arithmetic as launch_bt_eta + launch_bt_vel).
+template <typename real>
+void launch_bt_sub(const Geo<real>& g, const Phys<real>& P, const real* eta_in, const real* u_in,
+                   const real* v_in, real* eta_out, real* u_out, real* v_out, const real* Frx, const real* Fry,
+                   const real* fdt, const real* fden, real dtH, real dt_bt, real drag, hipStream_t s);
+
 // One barotropic substep (eta then velocity kernel) for every member of a
 // device-resident table.
 template <typename real>

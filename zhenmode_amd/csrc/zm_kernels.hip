@@ -1742,17 +1742,18 @@ __global__ __launch_bounds__(256) void k_mom_residual(
 // latitude row; rows of the polar-cap bands reduce their zonal means in LDS
 // (_apply_polar_cap legacy, horizontal.py:387-451).
 // ---------------------------------------------------------------------------
+// Sum over the 256 threads of one row group (t = thread within the group, red
+// = the group's 4-entry LDS scratch), broadcast to the group.  Both barriers
+// are block-wide: every thread of the block must make the same calls.
 template <typename real>
-__device__ __forceinline__ real block_sum(real x, real* red) {
-  // 256-thread sum, result broadcast to all threads
+__device__ __forceinline__ real group_sum(real x, real* red, int t) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = t & 63, wid = t >> 6;
   __syncthreads();
   if (lane == 0) red[wid] = x;
   __syncthreads();
   real s = 0;
-  const int nw = (blockDim.x + 63) >> 6;
-  for (int w = 0; w < nw; ++w) s += red[w];
+  for (int w = 0; w < 4; ++w) s += red[w];
   return s;
 }
 
@@ -1766,11 +1767,15 @@ __device__ __forceinline__ real cap_weight(const Phys<real>& P, int j, int ny) {
 constexpr int kRowChunk = 4;  // cells per thread per row (nx <= 1024)
 
 // eta' = eta - dt H div(ubt) on row j with the polar-cap blend of band rows;
-// reads eta_row[i] (row j of eta), writes out[i].  out may alias eta_row.
-template <typename real>
-__device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>& P, int j,
+// reads eta_row[i] (row j of eta), writes out[i] (and out2[i] unless null).
+// out may alias eta_row.  t: thread within the 256-thread row group; ALLRED:
+// reduce on every row so that the block-wide barriers match across groups
+// (the zonal mean is only used on band rows).
+template <typename real, bool ALLRED = false>
+__device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
                                            const real* __restrict__ eta_row, const real* __restrict__ ubt,
-                                           const real* __restrict__ vbt, real dtH, real* out, real* red) {
+                                           const real* __restrict__ vbt, real dtH, real* out, real* red,
+                                           real* out2 = nullptr) {
   const real* wet = P.wet2;
   real en[kRowChunk];
   real ssum = 0, wsum = 0;
@@ -1780,8 +1785,8 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
     // lanes past the row end repeat its last column and book nothing
-    const bool act = threadIdx.x + q * 256 < g.nx;
-    const int i = act ? threadIdx.x + q * 256 : g.nx - 1;
+    const bool act = t + q * 256 < g.nx;
+    const int i = act ? t + q * 256 : g.nx - 1;
     const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
     const int n2 = hasN ? c2 + g.nx : c2, s2 = hasS ? c2 - g.nx : c2;  // clamped: unused at the walls
     // every index is in bounds: load unconditionally, then select
@@ -1803,19 +1808,20 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
   }
   const real wb = cap_weight(P, j, g.ny);
   real zm = 0;
-  if (wb >= real(0)) {
-    const real s = block_sum(ssum, red);
-    const real w = block_sum(wsum, red + 4);
+  if (ALLRED || wb >= real(0)) {
+    const real s = group_sum(ssum, red, t);
+    const real w = group_sum(wsum, red + 4, t);
     zm = s / (w > real(1) ? w : real(1));
   }
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
-    const int i = threadIdx.x + q * 256;
+    const int i = t + q * 256;
     if (i >= g.nx) continue;
     const int c2 = j * g.nx + i;
     real e = en[q];
     if (wb >= real(0)) e = wet[c2] != real(0) ? wb * zm + (real(1) - wb) * e : real(0);
     out[i] = e;
+    if (out2) out2[i] = e;
   }
 }
 
@@ -1823,12 +1829,13 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
 // pointers; en / es are only read when that row exists): gradient, forcing,
 // legacy implicit Coriolis, drag, mask, polar-cap blend.  Outputs may alias
 // the inputs.
-template <typename real>
-__device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>& P, int j, const real* ec_row,
-                                           const real* en_row, const real* es_row, const real* __restrict__ ubt,
-                                           const real* __restrict__ vbt, real* uo, real* vo,
-                                           const real* __restrict__ Frx, const real* __restrict__ Fry, real fd,
-                                           real den, real dt_bt, real drag, real* red) {
+template <typename real, bool ALLRED = false>
+__device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
+                                           const real* ec_row, const real* en_row, const real* es_row,
+                                           const real* __restrict__ ubt, const real* __restrict__ vbt, real* uo,
+                                           real* vo, const real* __restrict__ Frx, const real* __restrict__ Fry,
+                                           real fd, real den, real dt_bt, real drag, real* red,
+                                           bool store = true) {
   const real* wet = P.wet2;
   real un[kRowChunk], vn[kRowChunk];
   real su = 0, sv = 0, sw = 0;
@@ -1840,8 +1847,8 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
     // lanes past the row end repeat its last column and book nothing
-    const bool act = threadIdx.x + q * 256 < g.nx;
-    const int i = act ? threadIdx.x + q * 256 : g.nx - 1;
+    const bool act = t + q * 256 < g.nx;
+    const int i = act ? t + q * 256 : g.nx - 1;
     const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
     // every index is in bounds (rows j +- 1 clamped to row j at the walls):
     // load unconditionally, then select
@@ -1873,17 +1880,18 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
   }
   const real wb = cap_weight(P, j, g.ny);
   real zu = 0, zv = 0;
-  if (wb >= real(0)) {
-    const real s1 = block_sum(su, red);
-    const real s2 = block_sum(sv, red + 4);
-    const real w = block_sum(sw, red + 8);
+  if (ALLRED || wb >= real(0)) {
+    const real s1 = group_sum(su, red, t);
+    const real s2 = group_sum(sv, red + 4, t);
+    const real w = group_sum(sw, red + 8, t);
     const real d = w > real(1) ? w : real(1);
     zu = s1 / d;
     zv = s2 / d;
   }
+  if (!store) return;
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
-    const int i = threadIdx.x + q * 256;
+    const int i = t + q * 256;
     if (i >= g.nx) continue;
     const int c2 = j * g.nx + i;
     real a = un[q], b = vn[q];
@@ -1903,7 +1911,7 @@ __global__ __launch_bounds__(256) void k_bt_eta(Geo<real> g, Phys<real> P, real*
                                                 real dtH) {
   __shared__ real red[8];
   const int j = blockIdx.x;
-  bt_eta_row(g, P, j, eta + (long long)j * g.nx, ubt, vbt, dtH, eta + (long long)j * g.nx, red);
+  bt_eta_row(g, P, j, (int)threadIdx.x, eta + (long long)j * g.nx, ubt, vbt, dtH, eta + (long long)j * g.nx, red);
 }
 
 template <typename real>
@@ -1915,7 +1923,54 @@ __global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const
   __shared__ real red[16];
   const int j = blockIdx.x;
   const real* ec = eta + (long long)j * g.nx;
-  bt_vel_row(g, P, j, ec, ec + g.nx, ec - g.nx, ubt, vbt, ubt, vbt, Frx, Fry, fdt[j], fden[j], dt_bt, drag, red);
+  bt_vel_row(g, P, j, (int)threadIdx.x, ec, ec + g.nx, ec - g.nx, ubt, vbt, ubt, vbt, Frx, Fry, fdt[j], fden[j], dt_bt,
+             drag, red);
+}
+
+// One whole forward-backward substep per launch (the arithmetic of k_bt_eta
+// followed by k_bt_vel).  A block of 4 row groups owns rows j0, j0 + 1: group
+// r computes eta' of row j0 - 1 + r into LDS (the two owned rows also to
+// eta_out), then groups 0 and 1 update ubt, vbt of the owned rows from the
+// LDS rows.  Inputs and outputs are separate buffers (ping-pong across
+// substeps), so a block's halo rows of eta' equal those computed by their
+// owners.  Groups without a row of their own work on a clamped row and store
+// nothing, so every group reaches the same block-wide barriers.
+// One whole forward-backward substep per launch (the arithmetic of k_bt_eta
+// followed by k_bt_vel).  A block of 4 row groups owns rows j0, j0 + 1: group
+// r computes eta' of row j0 - 1 + r into LDS (the two owned rows also to
+// eta_out), then groups 0 and 1 update ubt, vbt of the owned rows from the
+// LDS rows.  Inputs and outputs are separate buffers (ping-pong across
+// substeps), so a block's halo rows of eta' equal those computed by their
+// owners.  Every group reduces on every row (the zonal means are used on the
+// polar-cap band rows only) so that all groups meet the same block-wide
+// barriers; groups without a row of their own work on a clamped row and
+// store nothing.
+template <typename real>
+__global__ __launch_bounds__(1024) void k_bt_sub(Geo<real> g, Phys<real> P, const real* __restrict__ eta_in,
+                                                 const real* __restrict__ u_in, const real* __restrict__ v_in,
+                                                 real* __restrict__ eta_out, real* __restrict__ u_out,
+                                                 real* __restrict__ v_out, const real* __restrict__ Frx,
+                                                 const real* __restrict__ Fry, const real* __restrict__ fdt,
+                                                 const real* __restrict__ fden, real dtH, real dt_bt, real drag) {
+  __shared__ real erow[4][kRowChunk * 256];
+  __shared__ real red[4][16];
+  const int grp = (int)threadIdx.x >> 8, t = (int)threadIdx.x & 255;
+  const int j0 = (int)blockIdx.x * 2;
+  {
+    const int je = j0 - 1 + grp;
+    const bool in = je >= 0 && je < g.ny;
+    const int jc = in ? je : (je < 0 ? 0 : g.ny - 1);
+    const bool own = in && (grp == 1 || grp == 2);
+    bt_eta_row<real, true>(g, P, jc, t, eta_in + (long long)jc * g.nx, u_in, v_in, dtH, &erow[grp][0], &red[grp][0],
+                           own ? eta_out + (long long)jc * g.nx : (real*)nullptr);
+  }
+  __syncthreads();
+  const int jv = j0 + (grp & 1);
+  const bool own = grp < 2 && jv < g.ny;
+  const int jc = jv < g.ny ? jv : g.ny - 1;
+  const int sl = jc - (j0 - 1);  // LDS slot of row jc (1 or 2)
+  bt_vel_row<real, true>(g, P, jc, t, &erow[sl][0], &erow[sl + 1][0], &erow[sl - 1][0], u_in, v_in, u_out, v_out, Frx,
+                         Fry, fdt[jc], fden[jc], dt_bt, drag, &red[grp][0], own);
 }
 
 // The substep's two kernels for many independent grids in one launch each:
@@ -1937,7 +1992,7 @@ __global__ __launch_bounds__(256) void k_bt_eta_multi(const BtMember<real>* __re
   const BtMember<real>& m = mem[bt_member_of(mem, nmem, (int)blockIdx.x)];
   const int j = (int)blockIdx.x - m.row0;
   real* row = m.eta + (long long)j * m.g.nx;
-  bt_eta_row(m.g, m.P, j, row, m.ubt, m.vbt, m.dtH, row, red);
+  bt_eta_row(m.g, m.P, j, (int)threadIdx.x, row, m.ubt, m.vbt, m.dtH, row, red);
 }
 
 template <typename real>
@@ -1946,8 +2001,8 @@ __global__ __launch_bounds__(256) void k_bt_vel_multi(const BtMember<real>* __re
   const BtMember<real>& m = mem[bt_member_of(mem, nmem, (int)blockIdx.x)];
   const int j = (int)blockIdx.x - m.row0;
   const real* ec = m.eta + (long long)j * m.g.nx;
-  bt_vel_row(m.g, m.P, j, ec, ec + m.g.nx, ec - m.g.nx, m.ubt, m.vbt, m.ubt, m.vbt, m.Frx, m.Fry, m.fdt[j],
-             m.fden[j], m.dt_bt, m.drag, red);
+  bt_vel_row(m.g, m.P, j, (int)threadIdx.x, ec, ec + m.g.nx, ec - m.g.nx, m.ubt, m.vbt, m.ubt, m.vbt, m.Frx, m.Fry,
+             m.fdt[j], m.fden[j], m.dt_bt, m.drag, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1977,11 +2032,12 @@ __global__ __launch_bounds__(256) void k_cap_means(Geo<real> g, Phys<real> P, co
       sw += real(1);
     }
   }
-  su = block_sum(su, red);
-  sv = block_sum(sv, red + 4);
-  sT = block_sum(sT, red + 8);
-  sS = block_sum(sS, red + 12);
-  sw = block_sum(sw, red + 16);
+  const int t = (int)threadIdx.x;  // one 256-thread group per block
+  su = group_sum(su, red, t);
+  sv = group_sum(sv, red + 4, t);
+  sT = group_sum(sT, red + 8, t);
+  sS = group_sum(sS, red + 12, t);
+  sw = group_sum(sw, red + 16, t);
   if (threadIdx.x == 0) {
     const real d = sw > real(1) ? sw : real(1);
     const long long o = (long long)k * 2 * P.nb + band_row;
@@ -2246,6 +2302,14 @@ void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, rea
 }
 
 template <typename real>
+void launch_bt_sub(const Geo<real>& g, const Phys<real>& P, const real* eta_in, const real* u_in,
+                   const real* v_in, real* eta_out, real* u_out, real* v_out, const real* Frx, const real* Fry,
+                   const real* fdt, const real* fden, real dtH, real dt_bt, real drag, hipStream_t s) {
+  hipLaunchKernelGGL(k_bt_sub<real>, dim3((unsigned)((g.ny + 1) / 2)), dim3(1024), 0, s, g, P, eta_in, u_in, v_in,
+                     eta_out, u_out, v_out, Frx, Fry, fdt, fden, dtH, dt_bt, drag);
+}
+
+template <typename real>
 void launch_bt_multi(const BtMember<real>* table, int nmem, int total_rows, hipStream_t s) {
   hipLaunchKernelGGL(k_bt_eta_multi<real>, dim3(total_rows), dim3(256), 0, s, table, nmem);
   hipLaunchKernelGGL(k_bt_vel_multi<real>, dim3(total_rows), dim3(256), 0, s, table, nmem);
@@ -2323,6 +2387,9 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                     const real*, const real*, const real*, const real*, real, real,         \
                                     hipStream_t);                                                            \
   template void launch_bt_multi<real>(const BtMember<real>*, int, int, hipStream_t);                        \
+  template void launch_bt_sub<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, const real*,  \
+                                    real*, real*, real*, const real*, const real*, const real*, const real*,   \
+                                    real, real, real, hipStream_t);                                             \
   template void launch_cap_means<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,       \
                                        const real*, const real*, const real*, const real*, const real*,     \
                                        const real*, real*, hipStream_t);                                     \
