@@ -135,6 +135,9 @@ __device__ __forceinline__ void cell_of(long long c, int n2, int nx, int& i, int
 // workgroup's L1; tiles are numbered i-fastest and mapped XCD-contiguously.
 constexpr int kTileI = 64, kTileJ = 4;
 
+// Levels a column-scan kernel loads ahead of the level it sums.
+constexpr int kColAhead = 6;
+
 #define ZM_TILE_COORDS                                                      \
   const int ntx_ = (g.nx + kTileI - 1) / kTileI;                            \
   const int nty_ = (g.ny + kTileJ - 1) / kTileJ;                            \
@@ -1228,16 +1231,33 @@ __global__ __launch_bounds__(256) void k_pbc(Geo<real> g, Phys<real> P, const re
   real rprev = 0 < nw ? rho_anom(T[c2], S[c2], P) : real(0);
   real acc = 0;
   pbc[c2] = real(0);
-#pragma unroll 8
-  for (int k = 1; k < g.nz; ++k) {
-    const long long o = (long long)k * g.n2 + c2;
-    const real To = T[o], So = S[o];  // loaded on every level, used on wet ones
-    keep(To);
-    keep(So);
-    const real r = k < nw ? rho_anom(To, So, P) : real(0);
-    acc = acc + real(kG) * (real(0.5) * (rprev + r)) * ldc(g.dz3, k - 1);
-    pbc[o] = acc;
-    rprev = r;
+  // T, S of every level are loaded kColAhead levels ahead (clamped to the
+  // last level), so the loads of later levels are in flight while this one
+  // is summed and no load sits behind a wet-level branch
+  constexpr int D = kColAhead;
+  real Tq[D], Sq[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const long long o = (long long)(1 + d < g.nz ? 1 + d : g.nz - 1) * g.n2 + c2;
+    Tq[d] = T[o];
+    Sq[d] = S[o];
+  }
+  for (int k0 = 1; k0 < g.nz; k0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 + d;
+      if (k < g.nz) {  // uniform
+        const real To = Tq[d], So = Sq[d];
+        const long long on = (long long)(k + D < g.nz ? k + D : g.nz - 1) * g.n2 + c2;
+        Tq[d] = T[on];
+        Sq[d] = S[on];
+        const long long o = (long long)k * g.n2 + c2;
+        const real r = k < nw ? rho_anom(To, So, P) : real(0);
+        acc = acc + real(kG) * (real(0.5) * (rprev + r)) * ldc(g.dz3, k - 1);
+        pbc[o] = acc;
+        rprev = r;
+      }
+    }
   }
 }
 
@@ -1253,30 +1273,39 @@ __global__ __launch_bounds__(256) void k_bt_rho(Geo<real> g, Phys<real> P, const
   if (c2 >= g.n2) return;
   const int j = c2 / g.nx, i = c2 - j * g.nx;
   const Nb n = make_nb(g, i, j);
-  auto grad = [&](int k, real& gx, real& gy) {
-    const long long pl = (long long)k * g.n2;
-    const real pc = pbc[pl + n.c2], pe = pbc[pl + n.e2], pw = pbc[pl + n.w2];
-    const real pn = pbc[pl + n.n2e];  // edge-clamped: unused at the walls
-    const real ps = pbc[pl + n.s2e];
-    keep(pe);
-    keep(pw);
-    keep(pn);
-    keep(ps);
-    grad_cons(pc, pe, pw, pn, ps, n, k, g, gx, gy);
+  // the five pbc values of each level are loaded kColAhead levels ahead
+  // (clamped to the last level; the north/south rows edge-clamped and unused
+  // at the walls), so later levels' loads are in flight during the sum
+  constexpr int D = kColAhead;
+  real qc[D], qe[D], qw[D], qn[D], qs[D];
+  auto fetch = [&](int k, int d) {
+    const long long pl = (long long)(k < g.nz ? k : g.nz - 1) * g.n2;
+    qc[d] = pbc[pl + n.c2];
+    qe[d] = pbc[pl + n.e2];
+    qw[d] = pbc[pl + n.w2];
+    qn[d] = pbc[pl + n.n2e];
+    qs[d] = pbc[pl + n.s2e];
   };
-  real gxp, gyp;
-  grad(0, gxp, gyp);
+#pragma unroll
+  for (int d = 0; d < D; ++d) fetch(d, d);
+  real gxp = 0, gyp = 0;
   real sx = 0, sy = 0;
-#pragma unroll 8
-  for (int k = 0; k + 1 < g.nz; ++k) {
-    real gxn, gyn;
-    grad(k + 1, gxn, gyn);
-    if (k + 1 < n.nwC) {
-      sx = sx + real(0.5) * (gxp + gxn) * ldc(g.dz3, k);
-      sy = sy + real(0.5) * (gyp + gyn) * ldc(g.dz3, k);
+  for (int k0 = 0; k0 < g.nz; k0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 + d;
+      if (k < g.nz) {  // uniform
+        real gxn, gyn;
+        grad_cons(qc[d], qe[d], qw[d], qn[d], qs[d], n, k, g, gxn, gyn);
+        fetch(k + D, d);
+        if (k > 0 && k < n.nwC) {
+          sx = sx + real(0.5) * (gxp + gxn) * ldc(g.dz3, k - 1);
+          sy = sy + real(0.5) * (gyp + gyn) * ldc(g.dz3, k - 1);
+        }
+        gxp = gxn;
+        gyp = gyn;
+      }
     }
-    gxp = gxn;
-    gyp = gyn;
   }
   Fx[c2] = -sx * P.inv_rho0H;
   Fy[c2] = -sy * P.inv_rho0H;
