@@ -90,15 +90,6 @@ struct Phys {
 __device__ __forceinline__ int wrap_e(int i, int nx) { return i + 1 == nx ? 0 : i + 1; }
 __device__ __forceinline__ int wrap_w(int i, int nx) { return i == 0 ? nx - 1 : i - 1; }
 
-// Pin a loaded value to this point of the program: the compiler may otherwise
-// sink a load that feeds only one side of a select into a branch, which
-// serialises the memory latency of every guarded neighbour load.  Emits no
-// instruction.
-template <typename T>
-__device__ __forceinline__ void keep(const T& x) {
-  asm volatile("" ::"v"(x));
-}
-
 // Read-only set-up table element through the scalar cache (constant address
 // space): uniform indices become scalar loads that the compiler may hoist past
 // the kernel's own stores.  Only for tables no kernel writes.

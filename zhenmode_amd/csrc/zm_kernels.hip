@@ -834,19 +834,11 @@ __global__ __launch_bounds__(256) void k_linear_finish(
 // Fz: _divergence_h (horizontal.py:255-281) accumulated from the seafloor up
 // (_vertical_transport_iface, transport.py:110-133).  Fz has nz+1 levels.
 // ---------------------------------------------------------------------------
+// _divergence_h of one level from its six loaded velocities: face gates per
+// level, the wrapped y neighbour closed at the walls.
 template <typename real>
-__device__ __forceinline__ real div_h(const real* __restrict__ u, const real* __restrict__ v,
-                                      long long plane, const Nb& n, int k, real idx, real idyc, real cfN,
-                                      real cfS) {
-  // every neighbour index is in bounds (x periodic, y edge-clamped): load
-  // unconditionally and select, so the loads of a level issue together; the
-  // row metrics come in registers (a store to Fz could alias the metric rows)
-  const real uc = u[plane + n.c2], ue = u[plane + n.e2], uw = u[plane + n.w2];
-  const real vc = v[plane + n.c2], vn = v[plane + n.n2e], vs = v[plane + n.s2e];
-  keep(ue);
-  keep(uw);
-  keep(vn);
-  keep(vs);
+__device__ __forceinline__ real div_vals(real uc, real ue, real uw, real vc, real vn, real vs, const Nb& n, int k,
+                                         real idx, real idyc, real cfN, real cfS) {
   const bool wc = k < n.nwC;
   const real fxe = (wc && k < n.nwE) ? real(0.5) * (uc + ue) : real(0);
   const real fxw = (wc && k < n.nwW) ? real(0.5) * (uw + uc) : real(0);
@@ -867,14 +859,33 @@ __global__ __launch_bounds__(256) void k_fz(Geo<real> g, const real* __restrict_
   Fz[(long long)g.nz * g.n2 + c2] = real(0);
   // dz of the next level is read before this level's store (the store could
   // alias the metric array, which would otherwise serialise every level)
-  real dzk = ldc(g.dzn, g.nz - 1);
-#pragma unroll 10
-  for (int k = g.nz - 1; k >= 0; --k) {
-    const long long plane = (long long)k * g.n2;
-    const real dz_next = ldc(g.dzn, k > 0 ? k - 1 : 0);
-    acc = acc + div_h(u, v, plane, n, k, idx, idyc, cfN, cfS) * dzk;
-    Fz[plane + c2] = acc;
-    dzk = dz_next;
+  // the six velocities of each level are loaded kColAhead levels ahead
+  // (clamped to level 0), so later levels are in flight during the sum
+  constexpr int D = kColAhead;
+  real qc[D], qe[D], qw[D], qvc[D], qn[D], qs[D];
+  auto fetch = [&](int k, int d) {
+    const long long pl = (long long)(k >= 0 ? k : 0) * g.n2;
+    qc[d] = u[pl + n.c2];
+    qe[d] = u[pl + n.e2];
+    qw[d] = u[pl + n.w2];
+    qvc[d] = v[pl + n.c2];
+    qn[d] = v[pl + n.n2e];
+    qs[d] = v[pl + n.s2e];
+  };
+#pragma unroll
+  for (int d = 0; d < D; ++d) fetch(g.nz - 1 - d, d);
+  for (int k0 = g.nz - 1; k0 >= 0; k0 -= D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 - d;
+      if (k >= 0) {  // uniform
+        const real dv = div_vals(qc[d], qe[d], qw[d], qvc[d], qn[d], qs[d], n, k, idx, idyc, cfN, cfS);
+        fetch(k - D, d);
+        const long long plane = (long long)k * g.n2;
+        acc = acc + dv * ldc(g.dzn, k);
+        Fz[plane + c2] = acc;
+      }
+    }
   }
 }
 
@@ -902,19 +913,34 @@ __global__ __launch_bounds__(256) void k_conv_gate(Geo<real> g, Phys<real> P, co
   bool unstable = false;
   unsigned long long ub0 = 0, ub1 = 0;
   real rprev = rho_anom(T[c2], S[c2], P);
-  // full-depth loop with a wet test, so the loads of several levels are in
-  // flight together; levels at or below nw never set a bit
-#pragma unroll 10
-  for (int k = 1; k < g.nz; ++k) {
-    const real Tk = T[(long long)k * n2 + c2], Sk = S[(long long)k * n2 + c2];
-    keep(Tk);
-    keep(Sk);
-    const real r = rho_anom(Tk, Sk, P);
-    const bool un = k < nw && rprev > r;
-    unstable |= un;
-    if (k - 1 < 64) ub0 |= (unsigned long long)un << (k - 1);
-    else ub1 |= (unsigned long long)un << ((k - 65) & 63);
-    rprev = r;
+  // full-depth loop with a wet test; T, S are loaded kColAhead levels ahead
+  // (clamped to the last level) so later levels are in flight while this one
+  // is tested; levels at or below nw never set a bit
+  constexpr int D = kColAhead;
+  real Tq[D], Sq[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const long long o = (long long)(1 + d < g.nz ? 1 + d : g.nz - 1) * n2 + c2;
+    Tq[d] = T[o];
+    Sq[d] = S[o];
+  }
+  for (int k0 = 1; k0 < g.nz; k0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 + d;
+      if (k < g.nz) {  // uniform
+        const real Tk = Tq[d], Sk = Sq[d];
+        const long long on = (long long)(k + D < g.nz ? k + D : g.nz - 1) * n2 + c2;
+        Tq[d] = T[on];
+        Sq[d] = S[on];
+        const real r = rho_anom(Tk, Sk, P);
+        const bool un = k < nw && rprev > r;
+        unstable |= un;
+        if (k - 1 < 64) ub0 |= (unsigned long long)un << (k - 1);
+        else ub1 |= (unsigned long long)un << ((k - 65) & 63);
+        rprev = r;
+      }
+    }
   }
   const bool on = unstable && P.kappa_conv > real(0);
   gate[c2] = on ? 1 : 0;
@@ -940,10 +966,23 @@ __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const re
   if (c2 >= g.n2 || !gate[c2]) return;
   const int nw = g.nwet[c2];
   const int n2 = g.n2;
-  // stage the wet part of the column (independent loads, issued together)
-  for (int k = 0; k < nw; ++k) {
-    tT[k * 64 + lane] = T[(long long)k * n2 + c2];
-    tS[k * 64 + lane] = S[(long long)k * n2 + c2];
+  // stage the wet part of the column, eight levels per batch of loads (the
+  // indices clamped to the deepest wet level, nw >= 1 on a gated column)
+  for (int k0 = 0; k0 < nw; k0 += 8) {
+    real tq[8], sq[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const long long o = (long long)(k0 + d < nw ? k0 + d : nw - 1) * n2 + c2;
+      tq[d] = T[o];
+      sq[d] = S[o];
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      if (k0 + d < nw) {
+        tT[(k0 + d) * 64 + lane] = tq[d];
+        tS[(k0 + d) * 64 + lane] = sq[d];
+      }
+    }
   }
   unsigned long long ub0 = 0, ub1 = 0;  // unstable interfaces, frozen over the sub-cycle
   if (P.local_conv) {
