@@ -70,9 +70,9 @@ def land_grid(nx=24, ny=20, nz=6, seed=7):
                    ocean_mask=ocean, land_mask=~ocean)
 
 
-def land_case(seed=7, dt=600.0, ny=20):
+def land_case(seed=7, dt=600.0, ny=20, nx=24, nz=6):
     """A land_grid with every process of the legacy step switched on."""
-    grid = land_grid(ny=ny, seed=seed)
+    grid = land_grid(nx=nx, ny=ny, nz=nz, seed=seed)
     nx, ny, nz = grid.nx, grid.ny, grid.nz
     physics = replace(PhysicsConfig(), nu_h=2.0e5, nu_v=1e-3, kappa_h=500.0, kappa_v=1e-4,
                       kappa_conv=0.5, nu_bi=1.0e15, kappa_bi=1.0e15, r_bot=2e-3)

@@ -109,6 +109,31 @@ def test_residual_stages_match_oracle(dtype):
         assert e <= tol, (name, e)
 
 
+@pytest.mark.parametrize("nx", [720, 800, 480, 360, 77])
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_dealiased_momentum_residual_at_production_row_lengths(nx, dtype):
+    """The x de-alias inside the momentum residual at the row lengths of the
+    production grids (Stockham FFT over radices 2, 3, 4, 5: a block's rows
+    transformed together, the last block partial) and at a length with other
+    prime factors (four-step DFT), against the oracle's numpy FFT projection."""
+    print(f"case land {nx}x6x3 momentum residual {dtype}", flush=True)
+    grid, physics, dt, kwargs, T, S = land_case(nx=nx, ny=6, nz=3)
+    kwargs = dict(kwargs, polar_cap_rows=0, polar_cap_taper=0)  # six rows leave no room for the cap bands
+    step, _, _ = make_solver_global(grid, physics, dt, dtype=dtype, mode_split=True, **kwargs)
+    p = oracle_params(grid, physics, dt, kwargs)
+    s0 = perturbed_state(p, T, S)
+    du, dv = O.momentum_residual(s0.u, s0.v, s0.T, s0.S, s0.eta, p)
+    bm = p.bottom_mask * p.wet_mask_z
+    du, dv = du + p.r_bot * s0.u * bm, dv + p.r_bot * s0.v * bm
+    gm = to_numpy_state(step.solver.stage(_lib.STAGE_MOMENTUM_RES, JaxStateG(*s0)))
+    wet = p.wet_mask_z > 0
+    tol = 1e-10 if dtype == "float64" else 2e-3
+    for got, ref, name in ((gm.u, du, "du"), (gm.v, dv, "dv")):
+        e = rel(got[wet], ref[wet])
+        print(name, f"{e:.1e}", flush=True)
+        assert e <= tol, (name, e)
+
+
 @pytest.mark.parametrize("dtype,block,chunk", [("float64", "", ""), ("float32", "", ""), ("float32", "3", "8"),
                                                ("float32", "2", "16"), ("float64", "4", "0")],
                          ids=["f64-stream4", "f32-stream4", "f32-stream3-chunk8", "f32-stream2-chunk16",
