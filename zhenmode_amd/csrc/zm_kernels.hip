@@ -1810,20 +1810,31 @@ __global__ __launch_bounds__(256) void k_mom_residual(
 // latitude row; rows of the polar-cap bands reduce their zonal means in LDS
 // (_apply_polar_cap legacy, horizontal.py:387-451).
 // ---------------------------------------------------------------------------
-// Sum over the 256 threads of one row group (t = thread within the group, red
-// = the group's 4-entry LDS scratch), broadcast to the group.  Both barriers
+// Sums over the 256 threads of one row group (t = thread within the group,
+// red = the group's LDS scratch of 4 entries per value), broadcast to the
+// group; each value is summed in the same order (wave tree, then the four
+// wave sums in turn), NV values sharing one pair of barriers.  Both barriers
 // are block-wide: every thread of the block must make the same calls.
-template <typename real>
-__device__ __forceinline__ real group_sum(real x, real* red, int t) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+template <typename real, int NV>
+__device__ __forceinline__ void group_sums(real (&x)[NV], real* red, int t) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    for (int o = 32; o > 0; o >>= 1) x[q] += __shfl_down(x[q], o, 64);
   const int lane = t & 63, wid = t >> 6;
   __syncthreads();
-  if (lane == 0) red[wid] = x;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[4 * q + wid] = x[q];
+  }
   __syncthreads();
-  real s = 0;
-  for (int w = 0; w < 4; ++w) s += red[w];
-  return s;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    real s = 0;
+    for (int w = 0; w < 4; ++w) s += red[4 * q + w];
+    x[q] = s;
+  }
 }
+
 
 template <typename real>
 __device__ __forceinline__ real cap_weight(const Phys<real>& P, int j, int ny) {
@@ -1877,9 +1888,9 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
   const real wb = cap_weight(P, j, g.ny);
   real zm = 0;
   if (ALLRED || wb >= real(0)) {
-    const real s = group_sum(ssum, red, t);
-    const real w = group_sum(wsum, red + 4, t);
-    zm = s / (w > real(1) ? w : real(1));
+    real sw2[2] = {ssum, wsum};
+    group_sums<real, 2>(sw2, red, t);
+    zm = sw2[0] / (sw2[1] > real(1) ? sw2[1] : real(1));
   }
 #pragma unroll
   for (int q = 0; q < kRowChunk; ++q) {
@@ -1949,12 +1960,11 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
   const real wb = cap_weight(P, j, g.ny);
   real zu = 0, zv = 0;
   if (ALLRED || wb >= real(0)) {
-    const real s1 = group_sum(su, red, t);
-    const real s2 = group_sum(sv, red + 4, t);
-    const real w = group_sum(sw, red + 8, t);
-    const real d = w > real(1) ? w : real(1);
-    zu = s1 / d;
-    zv = s2 / d;
+    real s3[3] = {su, sv, sw};
+    group_sums<real, 3>(s3, red, t);
+    const real d = s3[2] > real(1) ? s3[2] : real(1);
+    zu = s3[0] / d;
+    zv = s3[1] / d;
   }
   if (!store) return;
 #pragma unroll
@@ -2101,11 +2111,13 @@ __global__ __launch_bounds__(256) void k_cap_means(Geo<real> g, Phys<real> P, co
     }
   }
   const int t = (int)threadIdx.x;  // one 256-thread group per block
-  su = group_sum(su, red, t);
-  sv = group_sum(sv, red + 4, t);
-  sT = group_sum(sT, red + 8, t);
-  sS = group_sum(sS, red + 12, t);
-  sw = group_sum(sw, red + 16, t);
+  real s5[5] = {su, sv, sT, sS, sw};
+  group_sums<real, 5>(s5, red, t);
+  su = s5[0];
+  sv = s5[1];
+  sT = s5[2];
+  sS = s5[3];
+  sw = s5[4];
   if (threadIdx.x == 0) {
     const real d = sw > real(1) ? sw : real(1);
     const long long o = (long long)k * 2 * P.nb + band_row;
