@@ -198,22 +198,25 @@ def ensemble_leg(steps, warmup):
         else:
             sol.set_tiles(*tuned[key][:2])
     ens = Ensemble(sols)
-    a = states
-    for _ in range(warmup):
-        a = ens.step(a)
+    # two preallocated state sets stepped into each other, so that the timed
+    # steps replay the two captured graphs (one per pointer set)
+    bufs = [[s.new_state() for s in sols] for _ in range(2)]
+    a = ens.step(states, out=bufs[0])
+    for w in range(max(warmup, 2)):
+        a = ens.step(a, out=bufs[(w + 1) % 2])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        a = ens.step(a)
+    for n in range(steps):
+        a = ens.step(a, out=bufs[(n + 1 + max(warmup, 2)) % 2])
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / steps
-    b = states
-    for _ in range(warmup):
-        b = [s.step(x) for s, x in zip(sols, b)]
+    b = [s.step(x, out=y) for s, x, y in zip(sols, states, bufs[0])]
+    for w in range(max(warmup, 2)):
+        b = [s.step(x, out=y) for s, x, y in zip(sols, b, bufs[(w + 1) % 2])]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        b = [s.step(x) for s, x in zip(sols, b)]
+    for n in range(steps):
+        b = [s.step(x, out=y) for s, x, y in zip(sols, b, bufs[(n + 1 + max(warmup, 2)) % 2])]
     torch.cuda.synchronize()
     ms_sep = 1e3 * (time.perf_counter() - t0) / steps
     dt = members[0].dt
