@@ -9,6 +9,7 @@ Tolerances (max |HIP - oracle| / max |oracle| per field):
 """
 
 import os
+from dataclasses import replace
 from pathlib import Path
 
 import numpy as np
@@ -276,3 +277,28 @@ def test_odd_row_count_matches_oracle(cap, dtype, tol):
         ref = O.step(ref, p)
         s = step.solver.step(s)
     check(s, ref, tol if dtype == "float64" else {f: 3 * t for f, t in tol.items()}, "3 steps")
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("local", [False, True], ids=["column", "localized"])
+@pytest.mark.parametrize("dtype,tol", [("float64", TOL64), ("float32", dict(u=1e-4, v=1e-4, T=2e-6, S=2e-6, eta=1e-4))])
+def test_convective_subcycle_counts(nc, local, dtype, tol):
+    """Convective sub-cycles of 1-4 substeps run in registers in one pass down
+    the column, longer ones in LDS; kappa_conv is chosen so that
+    ceil(kappa_conv dt / (0.4 min dz^2)) gives each count."""
+    grid, physics, dt, kwargs, T, S = land_case()
+    min_dz = float(np.min(-np.diff(grid.z)))
+    physics = replace(physics, kappa_conv=(nc - 0.5) * 0.4 * min_dz ** 2 / dt)
+    kw = dict(kwargs, localize_conv=True) if local else kwargs
+    print(f"case land conv_nsub={nc} localize_conv={local} {dtype}", flush=True)
+    p = oracle_params(grid, physics, dt, kw)
+    assert p.conv_nsub == nc
+    step, _, _ = make_solver_global(grid, physics, dt, dtype=dtype, mode_split=True, **kw)
+    s0 = perturbed_state(p, T, S)
+    assert O.convective_mask(s0.T, s0.S, p).any()
+    ref = O.step(s0, p)
+    s = step.solver.step(JaxStateG(*s0))
+    check(s, ref, tol, "step")
+    ref = O.step(ref, p)
+    s = step.solver.step(s)
+    check(s, ref, tol if dtype == "float64" else {f: 2 * t for f, t in tol.items()}, "2 steps")

@@ -535,13 +535,18 @@ struct Solver : SolverBase {
     while (n < n_nu) {
       real* du = flip ? nBu : nAu;
       real* dv = flip ? nBv : nAv;
-      if ((n == 0 && bih_u) || nu_block <= 1) {
-        zm::launch_nu_substep(g, su, sv, du, dv, (n == 0 && bih_u) ? Lu : (real*)nullptr,
-                              (n == 0 && bih_u) ? Lv : (real*)nullptr, P.nu_h, dts, s);
+      const bool lap = n == 0 && bih_u;  // the first substep also stores lap(u0), lap(v0)
+      // blocks of nu_block substeps; a last single substep would cost nearly
+      // a whole block, so a remainder of nu_block + 1 is split in two
+      int h = std::min(nu_block, n_nu - n);
+      if (n_nu - n - h == 1 && h > 2) h -= 1;
+      if (nu_block <= 1 || h <= 1 || (lap && !zm::nu_block_writes_lap(g, h))) {
+        zm::launch_nu_substep(g, su, sv, du, dv, lap ? Lu : (real*)nullptr, lap ? Lv : (real*)nullptr, P.nu_h,
+                              dts, s);
         n += 1;
       } else {
-        const int h = std::min(nu_block, n_nu - n);
-        zm::launch_nu_block(g, su, sv, du, dv, P.nu_h, dts, h, s);
+        zm::launch_nu_block(g, su, sv, du, dv, P.nu_h, dts, h, s, lap ? Lu : (real*)nullptr,
+                            lap ? Lv : (real*)nullptr);
         n += h;
       }
       su = du;

@@ -24,9 +24,13 @@ void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* u
                        real* lv, real nu_h, real dts, hipStream_t s);
 template <typename real>
 int nu_block_max();
+// nsub substeps in one launch; lu, lv (optional) receive lap(u), lap(v) of the
+// input, which needs nu_block_writes_lap(g, nsub).
+template <typename real>
+bool nu_block_writes_lap(const Geo<real>& g, int nsub);
 template <typename real>
 void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
-                     int nsub, hipStream_t s);
+                     int nsub, hipStream_t s, real* lu = nullptr, real* lv = nullptr);
 template <typename real>
 void launch_d1_unit(const Geo<real>& g, const real* T, const real* S, real* dT, real* dS, hipStream_t s);
 template <typename real>

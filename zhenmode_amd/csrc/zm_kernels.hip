@@ -5,6 +5,8 @@
 // function it follows is named above each kernel.  All face gates of the
 // reference (wet * neighbour-wet products) are evaluated as selects on the
 // wet-level count, which equals the reference's 0/1 products for finite data.
+#include <stdexcept>
+
 #include "zm_common.h"
 #include "zm_kernels.h"
 
@@ -421,19 +423,18 @@ __device__ __forceinline__ void buf_st(double x, __amdgpu_buffer_rsrc_t r, unsig
 
 // Buffer descriptors of the streaming kernel's input and output level planes.
 struct NuBufs {
-  __amdgpu_buffer_rsrc_t u, v, uo, vo;
+  __amdgpu_buffer_rsrc_t u, v, uo, vo, lu, lv;
 };
 
 // One level-row of the streaming kernel: row r of level l + 1 from the window
 // (xs, x, xn) of level l; b = r's bit in the gate words.  cls (uniform): 1 =
 // every interior lane has all four faces open, 2 = every face is closed
 // (only the metric term remains: lap_strip with zero face differences), 0 =
-// mixed.
+// mixed.  Lx, Ly receive the Laplacians.
 template <typename real>
 __device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y, real yn, real ys, int cls,
                                                const NuGates& G, int b, real id2, real tr, real idy2, real hidy,
-                                               real nu_h, real dts, real& ou, real& ov) {
-  real Lx, Ly;
+                                               real nu_h, real dts, real& ou, real& ov, real& Lx, real& Ly) {
   if (cls == 2) {
     Lx = lap_sum(real(0), real(0), xn - xs, id2, idy2, hidy, tr);
     Ly = lap_sum(real(0), real(0), yn - ys, id2, idy2, hidy, tr);
@@ -462,8 +463,9 @@ struct NuRange {
   int c[NS], n[NS];
 };
 
-// One march step t.
-template <typename real, int NS, int SLOT>
+// One march step t.  LU: level 0 also stores the Laplacians of the input
+// rows [jb, je) (the first substep's lap(u), lap(v)).
+template <typename real, int NS, int SLOT, bool LU>
 __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> met,
                                                const NuGates& G, unsigned long long openw, unsigned long long dryw,
                                                int t, const NuRange<NS>& R,
@@ -505,8 +507,16 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
       }
       const int b = r - R.wb;
       const int cls = ((openw >> b) & 1ull) ? 1 : (((dryw >> b) & 1ull) ? 2 : 0);
+      real Lx, Ly;
       nu_stream_node(st.W1u[l], st.W2u[l], st.W0u[l], st.W1v[l], st.W2v[l], st.W0v[l], cls, G, b, id2, tr, idy2,
-                     hidy, nu_h, dts, Ou[l], Ov[l]);
+                     hidy, nu_h, dts, Ou[l], Ov[l], Lx, Ly);
+      if constexpr (LU) {
+        if (l == 0 && r >= R.jb && r < R.je && store_lane) {
+          const int ro = r * g.nx * (int)sizeof(real);
+          buf_st(Lx, B.lu, iob, ro);
+          buf_st(Ly, B.lv, iob, ro);
+        }
+      }
     }
   }
   // the new rows enter the windows of the level above
@@ -527,10 +537,11 @@ __device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo
   }
 }
 
-template <typename real, int NS>
+template <typename real, int NS, bool LU>
 __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __restrict__ u,
                                                    const real* __restrict__ v, real* __restrict__ uo,
-                                                   real* __restrict__ vo, real nu_h, real dts, int chunk) {
+                                                   real* __restrict__ vo, real* __restrict__ lu,
+                                                   real* __restrict__ lv, real nu_h, real dts, int chunk) {
   constexpr int OX = 64 - 2 * NS;
   static_assert(NS <= kNuWetBase && 2 * NS - 1 < kNuMetPad, "wet word and metric padding cover NS substeps");
   const int lane = threadIdx.x & 63;
@@ -553,6 +564,10 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   B.v = buf_rsrc(v + plane);
   B.uo = buf_rsrc(uo + plane);
   B.vo = buf_rsrc(vo + plane);
+  if constexpr (LU) {
+    B.lu = buf_rsrc(lu + plane);
+    B.lv = buf_rsrc(lv + plane);
+  }
   const int io = tx * OX + (lane - NS);
   const bool store_lane = lane >= NS && lane < 64 - NS && io < g.nx;
   // 32-bit byte offsets of this lane's column (row bases stay uniform)
@@ -610,7 +625,7 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const int tend = qe + 2 * NS - 1;
   static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
 #define ZM_NU_STEP(S) \
-  nu_stream_step<real, NS, S>(st, g, met, G, openw, dryw, t0 + S, R, B, ib, iob, store_lane, nu_h, dts)
+  nu_stream_step<real, NS, S, LU>(st, g, met, G, openw, dryw, t0 + S, R, B, ib, iob, store_lane, nu_h, dts)
   for (int t0 = qs; t0 < tend; t0 += kNuAhead) {
     ZM_NU_STEP(0);
     if (t0 + 1 >= tend) break;
@@ -1018,6 +1033,99 @@ __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const re
     const long long o = (long long)k * n2 + c2;
     cT[o] = nc > 1 ? aT[k * 64 + lane] / real(nc) : aT[k * 64 + lane];
     cS[o] = nc > 1 ? aS[k * 64 + lane] / real(nc) : aS[k * 64 + lane];
+  }
+}
+
+// Pass 2 for a sub-cycle of NC <= 4 substeps, in registers: substep s updates
+// level L from substep s-1's values at L-1, L, L+1 only, so one pass down the
+// column runs substep s NC-1 ... 0 levels behind substep 0 (at step m, substep
+// s is at level m - s).  Each level receives the arithmetic of k_conv, with
+// its tendencies summed in substep order; no LDS, so the kernel runs at full
+// occupancy.  T and S are loaded kColAhead levels ahead, clamped to the
+// deepest wet level.
+template <typename real, int NC>
+__global__ __launch_bounds__(256) void k_conv_stream(Geo<real> g, Phys<real> P, const real* __restrict__ T,
+                                                     const real* __restrict__ S, real* __restrict__ cT,
+                                                     real* __restrict__ cS, const int* __restrict__ gate,
+                                                     const unsigned long long* __restrict__ ubits) {
+  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  if (c2 >= g.n2 || !gate[c2]) return;
+  const int nw = g.nwet[c2];  // >= 1 on a gated column
+  const long long n2 = g.n2;
+  unsigned long long ub0 = ~0ull, ub1 = ~0ull;  // unstable interfaces, frozen over the sub-cycle
+  if (P.local_conv) {
+    ub0 = ubits[2 * (long long)c2];
+    ub1 = ubits[2 * (long long)c2 + 1];
+  }
+  const real kap = NC > 1 ? P.kappa_conv / real(NC) : P.kappa_conv;
+  const real h = P.dt / real(NC);
+  // per substep s, before step m: cur = t^s[m-s], fp = F^s[m-s-1] (0 above the
+  // surface), acc = tendency sum of level m-s over substeps 0..s
+  real curT[NC], curS[NC], fpT[NC], fpS[NC], acT[NC], acS[NC];
+#pragma unroll
+  for (int s = 0; s < NC; ++s) {
+    curT[s] = curS[s] = fpT[s] = fpS[s] = acT[s] = acS[s] = real(0);
+  }
+  curT[0] = T[c2];
+  curS[0] = S[c2];
+  constexpr int D = kColAhead;
+  real Tq[D], Sq[D];  // t^0 of levels m+1 .. m+D
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const long long o = (long long)(1 + d < nw ? 1 + d : nw - 1) * n2 + c2;
+    Tq[d] = T[o];
+    Sq[d] = S[o];
+  }
+  const int mend = g.nz + NC - 1;
+  for (int m0 = 0; m0 < mend; m0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int m = m0 + d;
+      if (m < mend) {  // uniform
+        real upT = Tq[d], upS = Sq[d];
+        const long long on = (long long)(m + 1 + D < nw ? m + 1 + D : nw - 1) * n2 + c2;
+        Tq[d] = T[on];
+        Sq[d] = S[on];
+        real aoT[NC], aoS[NC];
+#pragma unroll
+        for (int s = 0; s < NC; ++s) {
+          aoT[s] = acT[s];
+          aoS[s] = acS[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NC; ++s) {
+          const int L = m - s;  // uniform
+          const int Lz = L < 0 ? 0 : (L < g.nz - 1 ? L : g.nz - 1);
+          const int Li = Lz < g.nz - 2 ? Lz : (g.nz >= 2 ? g.nz - 2 : 0);
+          const real ot = curT[s], os = curS[s];
+          real fT = 0, fS = 0, et = 0, es = 0;
+          if (L >= 0 && L < nw) {
+            const bool open = (L < 64 ? (ub0 >> L) : (ub1 >> ((L - 64) & 63))) & 1ull;
+            if (L + 1 < nw && open) {
+              fT = (-kap * (upT - ot)) * ldc(g.idzi, Li);
+              fS = (-kap * (upS - os)) * ldc(g.idzi, Li);
+            }
+            et = (fpT[s] - fT) * ldc(g.idzn, Lz);
+            es = (fpS[s] - fS) * ldc(g.idzn, Lz);
+          }
+          const real at = s == 0 ? et : aoT[s - 1] + et;
+          const real as = s == 0 ? es : aoS[s - 1] + es;
+          acT[s] = at;
+          acS[s] = as;
+          fpT[s] = fT;
+          fpS[s] = fS;
+          curT[s] = upT;
+          curS[s] = upS;
+          upT = ot + et * h;  // t^{s+1}[L], the level below substep s+1's
+          upS = os + es * h;
+          if (s == NC - 1 && L >= 0 && L < g.nz) {
+            const long long o = (long long)L * n2 + c2;
+            cT[o] = NC > 1 ? at / real(NC) : at;
+            cS[o] = NC > 1 ? as / real(NC) : as;
+          }
+        }
+      }
+    }
   }
 }
 
@@ -2224,21 +2332,33 @@ static void launch_nu_strip(const Geo<real>& g, const real* u, const real* v, re
 // nsub substeps per launch on the narrowest halo that covers them: a 3-wide
 // halo computes 12 % more useful nodes per tile than a 4-wide one.
 template <typename real>
+bool nu_block_writes_lap(const Geo<real>& g, int nsub) {
+  return nsub >= 2 && nsub <= 4 && g.nu_chunk > 0;
+}
+
+template <typename real>
 void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
-                     int nsub, hipStream_t s) {
+                     int nsub, hipStream_t s, real* lu, real* lv) {
   if (nsub >= 2 && nsub <= 4 && g.nu_chunk > 0) {
     const int chunk = g.nu_chunk;
     auto go = [&](auto kern, int ns) {
       const int OX = 64 - 2 * ns;
       const long long waves = (long long)((g.nx + OX - 1) / OX) * ((g.ny + chunk - 1) / chunk) * g.nz;
       const unsigned blocks = (unsigned)((waves + 3) / 4);
-      hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, g, u, v, uo, vo, nu_h, dts, chunk);
+      hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, g, u, v, uo, vo, lu, lv, nu_h, dts, chunk);
     };
-    if (nsub == 2) go(k_nu_stream<real, 2>, 2);
-    else if (nsub == 3) go(k_nu_stream<real, 3>, 3);
-    else go(k_nu_stream<real, 4>, 4);
+    if (lu) {
+      if (nsub == 2) go(k_nu_stream<real, 2, true>, 2);
+      else if (nsub == 3) go(k_nu_stream<real, 3, true>, 3);
+      else go(k_nu_stream<real, 4, true>, 4);
+    } else {
+      if (nsub == 2) go(k_nu_stream<real, 2, false>, 2);
+      else if (nsub == 3) go(k_nu_stream<real, 3, false>, 3);
+      else go(k_nu_stream<real, 4, false>, 4);
+    }
     return;
   }
+  if (lu) throw std::runtime_error("launch_nu_block: lap output needs the streaming kernel (2-4 substeps)");
   using T = NuTile<real>;
   if (nsub <= 3)
     launch_nu_strip<real, T::H, 3>(g, u, v, uo, vo, nu_h, dts, nsub, s);
@@ -2269,6 +2389,16 @@ template <typename real>
 void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
                  int* gate, unsigned long long* ubits, hipStream_t s) {
   hipLaunchKernelGGL(k_conv_gate<real>, dim3(blocks_for(g.n2, 256)), dim3(256), 0, s, g, P, T, S, gate, ubits);
+  if (P.conv_nsub >= 1 && P.conv_nsub <= 4) {
+    const unsigned b = blocks_for(g.n2, 256);
+    const auto cg = (const int*)gate;
+    const auto cu = (const unsigned long long*)ubits;
+    if (P.conv_nsub == 1) hipLaunchKernelGGL((k_conv_stream<real, 1>), dim3(b), dim3(256), 0, s, g, P, T, S, cT, cS, cg, cu);
+    else if (P.conv_nsub == 2) hipLaunchKernelGGL((k_conv_stream<real, 2>), dim3(b), dim3(256), 0, s, g, P, T, S, cT, cS, cg, cu);
+    else if (P.conv_nsub == 3) hipLaunchKernelGGL((k_conv_stream<real, 3>), dim3(b), dim3(256), 0, s, g, P, T, S, cT, cS, cg, cu);
+    else hipLaunchKernelGGL((k_conv_stream<real, 4>), dim3(b), dim3(256), 0, s, g, P, T, S, cT, cS, cg, cu);
+    return;
+  }
   const size_t lds = (size_t)4 * g.nz * 64 * sizeof(real);
   if (lds > 65536) {
     static bool raised = false;  // once per instantiation (one per real type)
@@ -2430,8 +2560,9 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                         real*, real, real, hipStream_t);                                      \
   template void launch_d1_unit<real>(const Geo<real>&, const real*, const real*, real*, real*, hipStream_t); \
   template int nu_block_max<real>();                                                                         \
+  template bool nu_block_writes_lap<real>(const Geo<real>&, int);                                            \
   template void launch_nu_block<real>(const Geo<real>&, const real*, const real*, real*, real*, real, real,  \
-                                      int, hipStream_t);                                                     \
+                                      int, hipStream_t, real*, real*);                                       \
   template void launch_linear_finish<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,   \
                                            const real*, const real*, const real*, const real*, const real*, \
                                            const real*, const real*, const real*, real*, real*, real*,      \
