@@ -915,53 +915,83 @@ __device__ __forceinline__ real rho_anom(real T, real S, const Phys<real>& P) {
   return real(kRho0) * (real(-kAlpha) * (T - P.T_ref) + real(kBeta) * (S - P.S_ref));
 }
 
+// Levels of the column kernels split over the four waves of a block: wave q
+// of a 256-thread block serves levels [q Ls, min((q + 1) Ls, nz)) of the
+// block's 64 columns, Ls = ceil(nz / 4), so each column's chain of dependent
+// loads is a quarter as long and four times as many waves are in flight.
+constexpr int kColSegs = 4;
+
 template <typename real>
 __global__ __launch_bounds__(256) void k_conv_gate(Geo<real> g, Phys<real> P, const real* __restrict__ T,
                                                    const real* __restrict__ S, int* __restrict__ gate,
                                                    unsigned long long* __restrict__ ubits) {
   // pass 1: the column gate and the unstable interfaces k+1/2 (k < 128), one
-  // column per thread with no LDS, so every column is tested at full occupancy
-  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
-  if (c2 >= g.n2) return;
-  const int nw = g.nwet[c2];
-  const long long n2 = g.n2;
+  // column per lane and no LDS but the combine: wave q tests the interfaces
+  // above levels [max(a, 1), b) of its segment and the four waves' flags are
+  // OR-ed (exact) in LDS
+  __shared__ unsigned long long sb[kColSegs][2][64];
+  __shared__ int sun[kColSegs][64];
+  const int lane = (int)threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 64 + lane);
+  const bool live = c2 < g.n2;
+  const int Ls = (g.nz + kColSegs - 1) / kColSegs;
+  const int a = q * Ls > 1 ? q * Ls : 1, b = (q + 1) * Ls < g.nz ? (q + 1) * Ls : g.nz;  // interfaces above [a, b)
   bool unstable = false;
   unsigned long long ub0 = 0, ub1 = 0;
-  real rprev = rho_anom(T[c2], S[c2], P);
-  // full-depth loop with a wet test; T, S are loaded kColAhead levels ahead
-  // (clamped to the last level) so later levels are in flight while this one
-  // is tested; levels at or below nw never set a bit
-  constexpr int D = kColAhead;
-  real Tq[D], Sq[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const long long o = (long long)(1 + d < g.nz ? 1 + d : g.nz - 1) * n2 + c2;
-    Tq[d] = T[o];
-    Sq[d] = S[o];
-  }
-  for (int k0 = 1; k0 < g.nz; k0 += D) {
+  if (live && a < b) {  // a, b uniform
+    const int nw = g.nwet[c2];
+    const long long n2 = g.n2;
+    real rprev = rho_anom(T[(long long)(a - 1) * n2 + c2], S[(long long)(a - 1) * n2 + c2], P);
+    // T, S loaded kColAhead levels ahead (clamped to the segment's last
+    // level) so later levels are in flight while this one is tested; levels
+    // at or below nw never set a bit
+    constexpr int D = kColAhead;
+    real Tq[D], Sq[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int k = k0 + d;
-      if (k < g.nz) {  // uniform
-        const real Tk = Tq[d], Sk = Sq[d];
-        const long long on = (long long)(k + D < g.nz ? k + D : g.nz - 1) * n2 + c2;
-        Tq[d] = T[on];
-        Sq[d] = S[on];
-        const real r = rho_anom(Tk, Sk, P);
-        const bool un = k < nw && rprev > r;
-        unstable |= un;
-        if (k - 1 < 64) ub0 |= (unsigned long long)un << (k - 1);
-        else ub1 |= (unsigned long long)un << ((k - 65) & 63);
-        rprev = r;
+      const long long o = (long long)(a + d < b ? a + d : b - 1) * n2 + c2;
+      Tq[d] = T[o];
+      Sq[d] = S[o];
+    }
+    for (int k0 = a; k0 < b; k0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int k = k0 + d;
+        if (k < b) {  // uniform
+          const real Tk = Tq[d], Sk = Sq[d];
+          const long long on = (long long)(k + D < b ? k + D : b - 1) * n2 + c2;
+          Tq[d] = T[on];
+          Sq[d] = S[on];
+          const real r = rho_anom(Tk, Sk, P);
+          const bool un = k < nw && rprev > r;
+          unstable |= un;
+          if (k - 1 < 64) ub0 |= (unsigned long long)un << (k - 1);
+          else ub1 |= (unsigned long long)un << ((k - 65) & 63);
+          rprev = r;
+        }
       }
     }
   }
-  const bool on = unstable && P.kappa_conv > real(0);
-  gate[c2] = on ? 1 : 0;
-  if (on && P.local_conv) {
-    ubits[2 * (long long)c2] = ub0;
-    ubits[2 * (long long)c2 + 1] = ub1;
+  sun[q][lane] = unstable ? 1 : 0;
+  sb[q][0][lane] = ub0;
+  sb[q][1][lane] = ub1;
+  __syncthreads();
+  if (q == 0 && live) {
+    int any = 0;
+    unsigned long long w0 = 0, w1 = 0;
+#pragma unroll
+    for (int r = 0; r < kColSegs; ++r) {
+      any |= sun[r][lane];
+      w0 |= sb[r][0][lane];
+      w1 |= sb[r][1][lane];
+    }
+    const bool on = any && P.kappa_conv > real(0);
+    gate[c2] = on ? 1 : 0;
+    if (on && P.local_conv) {
+      ubits[2 * (long long)c2] = w0;
+      ubits[2 * (long long)c2 + 1] = w1;
+    }
   }
 }
 
@@ -1041,24 +1071,44 @@ __global__ __launch_bounds__(64) void k_conv(Geo<real> g, Phys<real> P, const re
 // column runs substep s NC-1 ... 0 levels behind substep 0 (at step m, substep
 // s is at level m - s).  Each level receives the arithmetic of k_conv, with
 // its tendencies summed in substep order; no LDS, so the kernel runs at full
-// occupancy.  T and S are loaded kColAhead levels ahead, clamped to the
-// deepest wet level.
+// occupancy.  The levels are split over the block's four waves (kColSegs):
+// wave q outputs levels [a, b) and starts its pass NC - 1 levels higher, at
+// s0 = a - (NC - 1) with the exact substep-0 flux above s0; substep s is then
+// exact from level s0 + s down, so every output level is (the values above
+// are finite and never stored).  T and S are loaded kColAhead levels ahead,
+// clamped to the deepest wet level.  Blocks map to columns in launch order,
+// which the hardware deals round-robin over the 8 XCDs: the gated columns
+// cluster in latitude, and XCD-contiguous tiles would leave most of them to
+// one or two XCDs.
 template <typename real, int NC>
 __global__ __launch_bounds__(256) void k_conv_stream(Geo<real> g, Phys<real> P, const real* __restrict__ T,
                                                      const real* __restrict__ S, real* __restrict__ cT,
                                                      real* __restrict__ cS, const int* __restrict__ gate,
                                                      const unsigned long long* __restrict__ ubits) {
-  const int c2 = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
-  if (c2 >= g.n2 || !gate[c2]) return;
-  const int nw = g.nwet[c2];  // >= 1 on a gated column
+  const int lane = (int)threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int c2 = (int)blockIdx.x * 64 + lane;
+  const int Ls = (g.nz + kColSegs - 1) / kColSegs;
+  const int a = q * Ls, b = (q + 1) * Ls < g.nz ? (q + 1) * Ls : g.nz;  // uniform
+  if (c2 >= g.n2 || a >= b || !gate[c2]) return;
+  const int nw = g.nwet[c2];  // >= 2 on a gated column
   const long long n2 = g.n2;
+  if (a >= nw) {  // a dry segment: every tendency is 0
+    for (int L = a; L < b; ++L) {
+      cT[(long long)L * n2 + c2] = real(0);
+      cS[(long long)L * n2 + c2] = real(0);
+    }
+    return;
+  }
   unsigned long long ub0 = ~0ull, ub1 = ~0ull;  // unstable interfaces, frozen over the sub-cycle
   if (P.local_conv) {
     ub0 = ubits[2 * (long long)c2];
     ub1 = ubits[2 * (long long)c2 + 1];
   }
+  auto open_at = [&](int L) { return ((L < 64 ? (ub0 >> L) : (ub1 >> ((L - 64) & 63))) & 1ull) != 0; };
   const real kap = NC > 1 ? P.kappa_conv / real(NC) : P.kappa_conv;
   const real h = P.dt / real(NC);
+  const int s0 = a - (NC - 1) > 0 ? a - (NC - 1) : 0;  // uniform; s0 < nw
   // per substep s, before step m: cur = t^s[m-s], fp = F^s[m-s-1] (0 above the
   // surface), acc = tendency sum of level m-s over substeps 0..s
   real curT[NC], curS[NC], fpT[NC], fpS[NC], acT[NC], acS[NC];
@@ -1066,18 +1116,23 @@ __global__ __launch_bounds__(256) void k_conv_stream(Geo<real> g, Phys<real> P, 
   for (int s = 0; s < NC; ++s) {
     curT[s] = curS[s] = fpT[s] = fpS[s] = acT[s] = acS[s] = real(0);
   }
-  curT[0] = T[c2];
-  curS[0] = S[c2];
+  curT[0] = T[(long long)s0 * n2 + c2];
+  curS[0] = S[(long long)s0 * n2 + c2];
+  if (s0 > 0 && open_at(s0 - 1)) {  // substep 0's flux above s0 (s0 < nw)
+    const real tu = T[(long long)(s0 - 1) * n2 + c2], su = S[(long long)(s0 - 1) * n2 + c2];
+    fpT[0] = (-kap * (curT[0] - tu)) * ldc(g.idzi, s0 - 1);
+    fpS[0] = (-kap * (curS[0] - su)) * ldc(g.idzi, s0 - 1);
+  }
   constexpr int D = kColAhead;
   real Tq[D], Sq[D];  // t^0 of levels m+1 .. m+D
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const long long o = (long long)(1 + d < nw ? 1 + d : nw - 1) * n2 + c2;
+    const long long o = (long long)(s0 + 1 + d < nw ? s0 + 1 + d : nw - 1) * n2 + c2;
     Tq[d] = T[o];
     Sq[d] = S[o];
   }
-  const int mend = g.nz + NC - 1;
-  for (int m0 = 0; m0 < mend; m0 += D) {
+  const int mend = b + NC - 1;
+  for (int m0 = s0; m0 < mend; m0 += D) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int m = m0 + d;
@@ -1100,8 +1155,7 @@ __global__ __launch_bounds__(256) void k_conv_stream(Geo<real> g, Phys<real> P, 
           const real ot = curT[s], os = curS[s];
           real fT = 0, fS = 0, et = 0, es = 0;
           if (L >= 0 && L < nw) {
-            const bool open = (L < 64 ? (ub0 >> L) : (ub1 >> ((L - 64) & 63))) & 1ull;
-            if (L + 1 < nw && open) {
+            if (L + 1 < nw && open_at(L)) {
               fT = (-kap * (upT - ot)) * ldc(g.idzi, Li);
               fS = (-kap * (upS - os)) * ldc(g.idzi, Li);
             }
@@ -1118,7 +1172,7 @@ __global__ __launch_bounds__(256) void k_conv_stream(Geo<real> g, Phys<real> P, 
           curS[s] = upS;
           upT = ot + et * h;  // t^{s+1}[L], the level below substep s+1's
           upS = os + es * h;
-          if (s == NC - 1 && L >= 0 && L < g.nz) {
+          if (s == NC - 1 && L >= a && L < b) {
             const long long o = (long long)L * n2 + c2;
             cT[o] = NC > 1 ? at / real(NC) : at;
             cS[o] = NC > 1 ? as / real(NC) : as;
@@ -2388,9 +2442,9 @@ void launch_fz(const Geo<real>& g, const real* u, const real* v, real* Fz, hipSt
 template <typename real>
 void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
                  int* gate, unsigned long long* ubits, hipStream_t s) {
-  hipLaunchKernelGGL(k_conv_gate<real>, dim3(blocks_for(g.n2, 256)), dim3(256), 0, s, g, P, T, S, gate, ubits);
+  hipLaunchKernelGGL(k_conv_gate<real>, dim3(blocks_for(g.n2, 64)), dim3(256), 0, s, g, P, T, S, gate, ubits);
   if (P.conv_nsub >= 1 && P.conv_nsub <= 4) {
-    const unsigned b = blocks_for(g.n2, 256);
+    const unsigned b = blocks_for(g.n2, 64);  // 64 columns x kColSegs level segments per block
     const auto cg = (const int*)gate;
     const auto cu = (const unsigned long long*)ubits;
     if (P.conv_nsub == 1) hipLaunchKernelGGL((k_conv_stream<real, 1>), dim3(b), dim3(256), 0, s, g, P, T, S, cT, cS, cg, cu);
