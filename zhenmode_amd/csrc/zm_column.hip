@@ -34,14 +34,15 @@ __device__ __forceinline__ double mix_kappa(double Tc, double Sc, double Tn, dou
 // float64 scratch of 3 nz n2 values laid out [k][column] (coalesced across
 // lanes), the back substitution reads them in reverse; one reciprocal of the
 // pivot per level.  Dry levels are held.  `only` (optional) restricts the
-// solve to flagged columns.
+// solve to flagged columns.  Blocks map to columns in launch order (dealt
+// round-robin over the XCDs): flagged columns cluster in latitude.
 // ---------------------------------------------------------------------------
 template <typename real>
 __global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const real* __restrict__ T,
                                               const real* __restrict__ S, real* __restrict__ To,
                                               real* __restrict__ So, const int* __restrict__ only,
                                               double* __restrict__ scratch) {
-  const long long c2 = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const long long c2 = (long long)blockIdx.x * 256 + threadIdx.x;
   if (c2 >= g.n2) return;
   if (only && only[c2] == 0) return;
   const int nz = g.nz, nw = g.nwet[c2];
