@@ -90,19 +90,29 @@ class ColumnMLP:
             wt.copy_(w.t())
 
     # ------------------------------------------------------------------ forward
-    def forward(self, X, keep=False):
+    def inference_buffers(self, M):
+        """Preallocated activations and output of forward() over M rows."""
+        dev = self.device
+        acts = [torch.empty((M, n), dtype=torch.bfloat16, device=dev) for n in self.sizes[1:-1]]
+        return acts, torch.empty((M, N_OUT_PAD), dtype=torch.float32, device=dev)
+
+    def forward(self, X, keep=False, bufs=None):
         """X: (M, 256) bf16, M a multiple of 128.  Returns Y (M, 128) float32
-        and, with keep=True, the cache the backward pass needs."""
+        and, with keep=True, the cache the backward pass needs.  bufs (from
+        inference_buffers(M), keep=False only) receives the activations and Y,
+        so that a forward pass allocates nothing."""
         M = X.shape[0]
         if M % ROW_BLOCK or X.dtype != torch.bfloat16 or tuple(X.shape[1:]) != (N_IN,) or not X.is_contiguous():
             raise ValueError("X must be a contiguous (M, 256) bfloat16 tensor with M a multiple of 128")
+        if bufs is not None and (keep or bufs[1].shape[0] != M):
+            raise ValueError("inference buffers are for keep=False over the same row count")
         dev = self.device
         acts, actsT, pre = [X], [], []
         h = X
         nl = len(self.w)
         for l in range(nl - 1):
             n = self.sizes[l + 1]
-            a = torch.empty((M, n), dtype=torch.bfloat16, device=dev)
+            a = bufs[0][l] if bufs is not None else torch.empty((M, n), dtype=torch.bfloat16, device=dev)
             z = torch.empty((M, n), dtype=torch.float32, device=dev) if keep else None
             aT = torch.empty((n, M), dtype=torch.bfloat16, device=dev) if keep else None
             _check(self.lib.zm_gemm_bf16_nt(M, n, self.sizes[l], _p(h), _p(self.w_bf[l]), _lib.EPI_BIAS_GELU,
@@ -111,7 +121,7 @@ class ColumnMLP:
             actsT.append(aT)
             pre.append(z)
             h = a
-        Y = torch.empty((M, N_OUT_PAD), dtype=torch.float32, device=dev)
+        Y = bufs[1] if bufs is not None else torch.empty((M, N_OUT_PAD), dtype=torch.float32, device=dev)
         _check(self.lib.zm_gemm_bf16_nt(M, N_OUT_PAD, self.sizes[-2], _p(h), _p(self.w_bf[-1]), _lib.EPI_BIAS_F32,
                                         _p(self.b[-1]), None, _p(Y), None, None, _stream()))
         return Y, ((acts, actsT, pre) if keep else None)
@@ -210,6 +220,7 @@ class LearnedMixer:
         self.X = torch.empty((self.rows, N_IN), dtype=torch.bfloat16, device=dev)
         self.flags = torch.zeros(self.n2, dtype=torch.int32, device=dev)
         self.count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.bufs = mlp.inference_buffers(self.rows)
 
     def features(self, state, tau_x=None, tau_y=None, q_heat=None, out=None):
         s = self.solver
@@ -227,7 +238,7 @@ class LearnedMixer:
         s = self.solver
         src = s.coerce(state)
         X = self.features(src, tau_x, tau_y, q_heat)
-        Y, _ = self.mlp.forward(X)
+        Y, _ = self.mlp.forward(X, bufs=self.bufs)
         from .solver import empty3
         To = empty3(*s.shape3, s.dtype, s.device) if out is None else out[0]
         So = empty3(*s.shape3, s.dtype, s.device) if out is None else out[1]
