@@ -108,4 +108,17 @@ __device__ __forceinline__ long long xcd_tile(long long L, long long B) {
   return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
+// Map a launch-order block id onto a tile id so that groups of G consecutive
+// tiles are dealt round-robin over the 8 XCDs: work that varies with depth or
+// latitude (tile ranges) spreads evenly over the XCDs while the tiles of a
+// group still share their halos through one L2.  Bijective for any block
+// count (blocks past the last whole round of groups keep their own id).
+__device__ __forceinline__ long long xcd_spread(long long L, long long B, long long G) {
+  const long long nxcd = 8;
+  const long long full = (B / (nxcd * G)) * nxcd * G;
+  if (L >= full) return L;
+  const long long x = L % nxcd, i = L / nxcd;
+  return ((i / G) * nxcd + x) * G + (i % G);
+}
+
 }  // namespace zm

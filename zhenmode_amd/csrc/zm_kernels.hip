@@ -547,7 +547,10 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const int lane = threadIdx.x & 63;
   const int ntx = (g.nx + OX - 1) / OX, nch = (g.ny + chunk - 1) / chunk;
   const long long ntiles = (long long)ntx * nch * g.nz;
-  long long t = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  // groups of 8 blocks (32 waves, about 2.5 chunk rows of one level) dealt
+  // over the XCDs: the open-row share falls with depth, so XCD-contiguous
+  // level ranges would leave the surface levels' work to one XCD
+  long long t = xcd_spread(blockIdx.x, gridDim.x, 8) * 4 + (threadIdx.x >> 6);
   if (t >= ntiles) return;  // uniform per wave
   const int tx = __builtin_amdgcn_readfirstlane((int)(t % ntx));
   t /= ntx;
