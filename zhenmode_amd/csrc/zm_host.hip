@@ -231,7 +231,7 @@ struct Solver : SolverBase {
     ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
     g.nwet = nw;
     g.seg = 10;
-    g.nu_chunk = 36;
+    g.nu_chunk = 20;  // rows per wave chunk of the streaming nu_h kernel (measured best at 0.5 degrees)
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
     g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
     {
