@@ -302,3 +302,19 @@ def test_convective_subcycle_counts(nc, local, dtype, tol):
     ref = O.step(ref, p)
     s = step.solver.step(s)
     check(s, ref, tol if dtype == "float64" else {f: 2 * t for f, t in tol.items()}, "2 steps")
+
+
+@pytest.mark.parametrize("nz,nc", [(3, 2), (3, 4), (5, 3)])
+def test_convective_kernels_on_few_levels(nz, nc):
+    """Few levels leave some of the four level segments of the convective
+    gate and sub-cycle kernels one level long or empty."""
+    grid, physics, dt, kwargs, T, S = land_case(nz=nz)
+    min_dz = float(np.min(-np.diff(grid.z)))
+    physics = replace(physics, kappa_conv=(nc - 0.5) * 0.4 * min_dz ** 2 / dt)
+    print(f"case land nz={nz} conv_nsub={nc} float64", flush=True)
+    p = oracle_params(grid, physics, dt, kwargs)
+    assert p.conv_nsub == nc
+    step, _, _ = make_solver_global(grid, physics, dt, dtype="float64", mode_split=True, **kwargs)
+    s0 = perturbed_state(p, T, S)
+    assert O.convective_mask(s0.T, s0.S, p).any()
+    check(step.solver.step(JaxStateG(*s0)), O.step(s0, p), TOL64, "step")
