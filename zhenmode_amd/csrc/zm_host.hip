@@ -230,7 +230,10 @@ struct Solver : SolverBase {
     int* nw = dev_alloc<int>(n2);
     ZM_HIP(hipMemcpy(nw, c.nwet, n2 * sizeof(int), hipMemcpyHostToDevice));
     g.nwet = nw;
-    g.seg = 10;
+    // levels per thread of the column kernels: whole columns when there are
+    // enough of them to fill the GPU (best for every column kernel at 0.5
+    // degrees), else 10 (zm_autotune refines either)
+    g.seg = n2 >= 65536 ? nz : std::min(10, nz);
     g.nu_chunk = 20;  // rows per wave chunk of the streaming nu_h kernel (measured best at 0.5 degrees)
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
     g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
