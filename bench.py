@@ -248,6 +248,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-projection", action="store_true")
     ap.add_argument("--tiles", default="", help="nu_block,seg: fixed launch shapes instead of the autotuner")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="counter runs: only the warm-up and timed steps and the 1 GiB copy, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
@@ -329,6 +331,10 @@ def main():
     copy_ms = ce0.elapsed_time(ce1) / 5
     copy_gbps = 2.0 * copy_src.numel() * 4 / (copy_ms * 1e-3) / 1e9
     del copy_src, copy_dst
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "steps": args.warmup + args.steps, "ms_per_step": ms_per_step,
+                          "copy_1GiB_GBps": copy_gbps, "finite": finite}), flush=True)
+        return
 
     word = 4 if args.dtype == "float32" else 8
     n3 = g.nx * g.ny * g.nz

@@ -30,6 +30,20 @@ How each part is pinned (checked by ``tests/test_oracle_*.py``):
     conservation on closed basins.
   * whole step: a resting uniform ocean is an exact fixed point; volume is
     conserved to round-off.
+  * index loops written from the reference semantics, each with planted-
+    mistake twins that must fail (``tests/test_oracle_pins.py``): the
+    face-gated and layer-gated 3-D gradients, momentum advection with the
+    ghost-filled -w d/dz, the vertical velocity, the centred flux-form tracer
+    advection with its donor-cell vertical flux, the barotropic density
+    forcing and the legacy barotropic velocity.
+  * closed forms (same file): one forward-backward substep rotates (u, v) by
+    -atan(f dt) and scales the speed by 1/sqrt(1 + (f dt)^2) times the drag
+    1/(1 + r dt); the momentum residual of an eta field vanishes and that of
+    a zonal shear flow is the bottom drag alone; the legacy convective
+    sub-cycle's mean rate on two-level columns; the nu_h sub-cycle and the
+    biharmonic on a zonal eigenmode; the barotropic projection identity; the
+    prescribed-heat, bulk-heat and SST/SSS restoring integrals of one RK2
+    step (``tests/fd/test_stage_budgets.py:48-82``).
   * production flags (conservative_kv, localize_conv, fct_adv,
     monotone_adv): the reference's FCT/monotone assertions and the flux-form
     telescoping identity (``tests/test_oracle_flags.py``).
@@ -682,7 +696,7 @@ def bt_rho_pgf(T, S, p):
 # tendencies (reference dynamics/tendencies.py)
 # --------------------------------------------------------------------------
 def tracer_tendency(u, v, T, S, p, rec=None):
-    """dT/dt, dS/dt, legacy defaults (tendencies.py:120-245); ``rec`` receives
+    """dT/dt, dS/dt, legacy defaults (tendencies.py:94-219); ``rec`` receives
     the surface sources, process tendencies and surface advective fluxes
     (tendencies.py:211-214)."""
     wm = p.wet_mask_z
@@ -743,7 +757,7 @@ def tracer_tendency(u, v, T, S, p, rec=None):
 
 
 def tracer_residual(u, v, T, S, p, rec=None):
-    """Tendency minus the L-step's linear diffusion, add-then-subtract (tendencies.py:253-271)."""
+    """Tendency minus the L-step's linear diffusion, add-then-subtract (tendencies.py:227-245)."""
     dTdt, dSdt, (dhT, dhS, dvT, dvS) = tracer_tendency(u, v, T, S, p, rec=rec)
     dTdt = dTdt - dhT
     dSdt = dSdt - dhS
@@ -753,7 +767,7 @@ def tracer_residual(u, v, T, S, p, rec=None):
 
 
 def momentum_tendency(u, v, T, S, eta, p):
-    """du/dt, dv/dt and the shared terms (tendencies.py:65-98)."""
+    """du/dt, dv/dt and the shared terms (tendencies.py:39-72)."""
     wm = p.wet_mask_z
     w = vertical_velocity(u, v, p)
     adv_u, adv_v = advection_flux_form(u, v, w, p)
@@ -772,7 +786,7 @@ def momentum_tendency(u, v, T, S, eta, p):
 
 
 def momentum_residual(u, v, T, S, eta, p):
-    """Tendency minus linear parts (tendencies.py:273-345, legacy mode-split branch)."""
+    """Tendency minus linear parts (tendencies.py:247-319, legacy mode-split branch)."""
     dudt, dvdt, (lap_u, lap_v, dvu, dvv) = momentum_tendency(u, v, T, S, eta, p)
     f3 = p.f[:, :, None]
     dudt = dudt - p.nu_h * lap_u

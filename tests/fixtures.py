@@ -110,3 +110,33 @@ def perturbed_state(p, T, S, seed=3, amp=0.05):
     v = amp * rng.standard_normal(wm.shape) * wm
     eta = 0.1 * rng.standard_normal(wm.shape[:2]) * p.wet_mask
     return O.State(u, v, s.T, s.S, eta, s.ice)
+
+
+def surface_fixture(heat=100.0, **kwargs):
+    """8x8x4 all-wet column set of ``tests/support/fd/surface_energy.py:14-27``
+    (z = 0, -5, -20, -50 m; every diffusivity and viscosity zero; dt 3600 s;
+    no polar cap) with T = -1.8, S = 35 and the state at rest.  Extra keyword
+    arguments go to ``make_params`` (bulk heat, restoring)."""
+    grid = all_wet_grid(nx=8, ny=8, nz=4)
+    grid = replace(grid, z=np.array([0.0, -5.0, -20.0, -50.0]), dz=np.array([5.0, 15.0, 30.0]))
+    forcing = (np.zeros((8, 8)), np.zeros((8, 8)), np.full((8, 8), float(heat)))
+    p = O.make_params(grid, quiet_physics(), 3600.0, forcing=forcing, polar_cap_rows=0, polar_cap_taper=0,
+                      **kwargs)
+    shape = (8, 8, 4)
+    state = O.State(np.zeros(shape), np.zeros(shape), np.full(shape, -1.8), np.full(shape, 35.0),
+                    np.zeros((8, 8)), np.zeros((8, 8)))
+    return grid, p, state
+
+
+def planted(fn, old, new):
+    """A copy of oracle function ``fn`` with the source text ``old`` replaced
+    by ``new``: a planted mistake, used to show that a check fails on it.
+    The copy resolves names in a snapshot of ``fn``'s module namespace."""
+    import inspect
+    import textwrap
+
+    src = textwrap.dedent(inspect.getsource(fn))
+    assert src.count(old) == 1, f"{old!r} must occur once in {fn.__name__}"
+    namespace = dict(vars(inspect.getmodule(fn)))
+    exec(compile(src.replace(old, new), f"<planted {fn.__name__}>", "exec"), namespace)
+    return namespace[fn.__name__]

@@ -909,7 +909,7 @@ __global__ __launch_bounds__(256) void k_fz(Geo<real> g, const real* __restrict_
 
 // ---------------------------------------------------------------------------
 // Convective mixing: column gate (_convective_mask, physics/vertical.py:63-82)
-// and the legacy sub-cycle of _conv_flux_tendency (tendencies.py:175-191,
+// and the legacy sub-cycle of _conv_flux_tendency (tendencies.py:149-165,
 // physics/vertical.py:13-37).  One column per lane; the working column sits in
 // LDS as [k][lane].  conv buffers are written only where the gate is set.
 // ---------------------------------------------------------------------------
@@ -1187,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_conv_stream(Geo<real> g, Phys<real> P, 
 }
 
 // ---------------------------------------------------------------------------
-// Tracer residual (tendencies.py:120-271, legacy branches) at one node, fused
+// Tracer residual (tendencies.py:94-245, legacy branches) at one node, fused
 // with the RK2 update of _explicit_full_step (step.py:258-307).
 //   mode 0: dT1 -> dT1buf, T_pred = T + dT1 dt
 //   mode 1: T_new = Tbase + 0.5 (dT1 + dT2) dt
@@ -1846,7 +1846,7 @@ __global__ __launch_bounds__(256) void k_dealias_fft(int N, FftPlan plan, const 
 }
 
 // ---------------------------------------------------------------------------
-// Momentum residual (tendencies.py:65-98 and 273-345, legacy mode-split
+// Momentum residual (tendencies.py:39-72 and 247-319, legacy mode-split
 // branch) fused with the RK2 velocity update (step.py:263-319).
 //   mode 0: du1 -> buffers, u_pred = u + du1 dt
 //   mode 1: u_new = (u + 0.5 (du1 + du2) dt) * exp(-r_bot dt bm)
