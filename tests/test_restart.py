@@ -80,3 +80,26 @@ def test_tampered_archive_is_refused(tmp_path):
     np.savez_compressed(path, **payload)
     with pytest.raises(ValueError, match="unversioned"):
         R.load_restart(path, contract)
+
+
+def test_extra_arrays_unsafe_outputs_and_bad_counters_are_refused(tmp_path):
+    contract, state, _ = _setup(tmp_path)
+    path = tmp_path / "r.npz"
+    R.save_restart(path, state, contract, step=2, counters=dict(accepted=2), cumulative={}, history={},
+                   outputs={"snap/day1.npz": "0" * 64})
+    assert R.load_restart(path, contract).outputs == {"snap/day1.npz": "0" * 64}
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["code.py", "r.npz"]   # no temporary left
+    with np.load(path, allow_pickle=False) as saved:
+        payload = {k: saved[k] for k in saved.files}
+    payload["history__extra"] = np.zeros(2)
+    np.savez_compressed(path, **payload)
+    with pytest.raises(ValueError, match="does not list"):
+        R.load_restart(path, contract)
+    for outputs in ({"../escape.npz": "0" * 64}, {"/abs.npz": "0" * 64}, {"ok.npz": "xyz"}):
+        with pytest.raises(ValueError, match="output"):
+            R.save_restart(path, state, contract, step=1, counters={}, cumulative={}, history={}, outputs=outputs)
+    for counters in ({"n": -1}, {"n": 1.5}):
+        with pytest.raises(ValueError, match="nonnegative integer"):
+            R.save_restart(path, state, contract, step=1, counters=counters, cumulative={}, history={})
+    with pytest.raises(ValueError, match="'__'"):
+        R.save_restart(path, state, contract, step=1, counters={}, cumulative={"a__b": np.zeros(1)}, history={})

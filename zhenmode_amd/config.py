@@ -45,19 +45,30 @@ class PhysicsConfig:
     thermodynamics: str = "linear"
 
 
-def finite_number(name, value, *, positive=False, nonnegative=False):
-    """Reference config.py:158-165."""
-    if isinstance(value, (bool, np.bool_)) or not isinstance(value, Real) or not np.isfinite(value):
-        raise ValueError(f"{name} must be a finite number")
-    if positive and value <= 0.0:
-        raise ValueError(f"{name} must be positive")
-    if nonnegative and value < 0.0:
-        raise ValueError(f"{name} must be nonnegative")
+def _is_bool(value):
+    return isinstance(value, (bool, np.bool_))
+
+
+def checked_scalar(name, value, *, integer=False, above=None, at_least=None):
+    """Return ``value`` if it is a real (or, with ``integer``, an integral)
+    non-boolean finite number within the stated bounds; otherwise raise
+    ``ValueError`` naming the argument, as the reference factory does for
+    invalid configuration (config.py:157-170)."""
+    kind = Integral if integer else Real
+    if _is_bool(value) or not isinstance(value, kind) or not np.isfinite(value):
+        raise ValueError(f"{name} must be {'an integer' if integer else 'a finite number'}, got {value!r}")
+    if above is not None and not value > above:
+        raise ValueError(f"{name} must be greater than {above}, got {value!r}")
+    if at_least is not None and not value >= at_least:
+        raise ValueError(f"{name} must be at least {at_least}, got {value!r}")
     return value
+
+
+def finite_number(name, value, *, positive=False, nonnegative=False):
+    """A finite real: > 0 with ``positive``, >= 0 with ``nonnegative``."""
+    return checked_scalar(name, value, above=0.0 if positive else None, at_least=0.0 if nonnegative else None)
 
 
 def integer_count(name, value, *, minimum=0):
-    """Reference config.py:167-170."""
-    if isinstance(value, (bool, np.bool_)) or not isinstance(value, Integral) or value < minimum:
-        raise ValueError(f"{name} must be an integer >= {minimum}")
-    return value
+    """An integer >= ``minimum``."""
+    return checked_scalar(name, value, integer=True, at_least=minimum)

@@ -952,7 +952,7 @@ void validate(const zm_config& c) {
   if (c.n_subcyc < 1 || c.n_nu < 1 || c.conv_nsub < 1) bad("substep counts must be >= 1");
   if (c.adv_nsub != 1) bad("adv_nsub > 1 (vertical advection sub-cycling) is not supported");
   if ((long long)4 * c.nz * 64 * c.dtype_bytes > 160 * 1024)
-    bad("too many levels for the convective column kernel (nz <= 156 in float32, 78 in float64)");
+    bad("too many levels for the convective column kernel (nz <= 160 in float32, 80 in float64)");
   if (c.localize_conv && c.nz > 129) bad("localize_conv supports at most 129 levels");
   if (c.project_adv_vel) {
     if (c.projection_niter < 1) bad("projection_niter must be a positive integer");

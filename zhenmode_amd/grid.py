@@ -63,20 +63,27 @@ def build_global_grid(z_levels, depth, lat_max, *, dy=None):
 
 
 def validate_grid(grid):
-    """Reference grid.py:384-398."""
+    """Reject a grid record whose sizes or metrics the step cannot use (the
+    reference checks the same properties, grid.py:384-398): sizes are integers
+    of at least 2, dy is positive, dx_2d has the (nx, ny) shape and positive
+    finite entries, z has nz finite strictly descending nodes, and dz has the
+    nz - 1 positive gaps between them."""
     for name in ("nx", "ny", "nz"):
         integer_count(name, getattr(grid, name), minimum=2)
     finite_number("dy", grid.dy, positive=True)
-    widths = np.asarray(grid.dx_2d)
-    depths = np.asarray(grid.z)
-    spacing = np.asarray(grid.dz)
-    if widths.shape != (grid.nx, grid.ny) or not np.all(np.isfinite(widths) & (widths > 0.0)):
-        raise ValueError("dx_2d must have grid shape and positive finite spacing")
-    if depths.shape != (grid.nz,) or not np.all(np.isfinite(depths)) or not np.all(np.diff(depths) < 0.0):
-        raise ValueError("z must contain finite, strictly descending nodes")
-    if (spacing.shape != (grid.nz - 1,) or not np.all(np.isfinite(spacing) & (spacing > 0.0))
-            or not np.allclose(spacing, -np.diff(depths), rtol=1e-12, atol=0.0)):
-        raise ValueError("dz must match the positive distances between z nodes")
+    dx, z, dz = (np.asarray(getattr(grid, name), dtype=np.float64) for name in ("dx_2d", "z", "dz"))
+    checks = (
+        ("dx_2d", dx.shape == (grid.nx, grid.ny) and bool(np.all(np.isfinite(dx))) and bool(np.all(dx > 0.0)),
+         "an (nx, ny) array of positive finite spacings"),
+        ("z", z.shape == (grid.nz,) and bool(np.all(np.isfinite(z))) and bool(np.all(z[1:] < z[:-1])),
+         "nz finite nodes in strictly descending order"),
+        ("dz", dz.shape == (grid.nz - 1,) and z.shape == (grid.nz,)
+         and bool(np.all(np.abs(dz - (z[:-1] - z[1:])) <= 1e-12 * np.abs(z[:-1] - z[1:]))) and bool(np.all(dz > 0.0)),
+         "the nz - 1 positive distances between consecutive z nodes"),
+    )
+    for name, ok, expected in checks:
+        if not ok:
+            raise ValueError(f"grid.{name} must be {expected}")
 
 
 def all_wet_grid(nx=32, ny=32, nz=8):

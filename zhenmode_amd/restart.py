@@ -1,22 +1,32 @@
-"""Strict, atomic restart files with the reference's npz schema.
+"""Restart files for the mode-split step.
 
-Mirrors ``src/zhenmode/model/io/restart.py:17-227``: the same contract record
-(``make_restart_contract``), the same archive keys (``state__{u,v,T,S,eta,ice}``,
-``cumulative__*``, ``history__*``, ``metadata_json``, ``metadata_sha256``), the
-same finite-only/no-pickle rule and the same rejection of any contract or byte
-mismatch with ``ValueError``.  Device tensors are copied to host numpy arrays
-in the reference layout (nx, ny, nz) before writing.  A contract embeds source
-hashes, so files are exchangeable only between runs of the same code; the
-schema and semantics are shared with the reference.
+Interface kept from the reference (``src/zhenmode/model/io/restart.py:17-227``)
+so that the same tooling reads both: the contract record's keys, the archive
+keys (``state__{u,v,T,S,eta,ice}``, ``cumulative__<name>``, ``history__<name>``,
+``metadata_json``, ``metadata_sha256``), the metadata fields (schema version,
+contract, step, elapsed seconds, counters, outputs and a shape/dtype/SHA-256
+descriptor per array), the canonical JSON text that the checksum covers, and
+the rejection rules (any contract, checksum, descriptor or array-set
+difference, and any non-finite or non-numeric array, raises ``ValueError``;
+nothing is unpickled).  The contract embeds the SHA-256 of the code files, so
+an archive is accepted only by a run of the same code.
+
+Design: every value that enters the contract is reduced to a canonical JSON
+tree by a small type-dispatch table; an archive is assembled in memory,
+written to a fresh exclusive file beside the target, flushed to disk and
+renamed over the target (the directory is synced too); loading verifies the
+metadata checksum first, then the contract, then each array against its
+descriptor, before any array is returned.  Device tensors are copied to host
+arrays in the reference layout (nx, ny, nz).
 """
 
 from __future__ import annotations
 
 import hashlib
+import io
 import json
 import os
-import tempfile
-from contextlib import contextmanager
+import secrets
 from dataclasses import dataclass, fields, is_dataclass
 from pathlib import Path, PurePosixPath
 
@@ -24,127 +34,104 @@ import numpy as np
 
 SCHEMA_VERSION = 1
 STATE_FIELDS = ("u", "v", "T", "S", "eta", "ice")
+GROUPS = ("state", "cumulative", "history")
+STATE_FAMILY = "FD_point_samples_linear_free_surface"
 
 
-def _json(value):
-    return json.dumps(value, sort_keys=True, separators=(",", ":"), allow_nan=False)
+def canonical_json(tree):
+    """The exact text the checksum covers: sorted keys, no spaces, no NaN."""
+    return json.dumps(tree, sort_keys=True, separators=(",", ":"), allow_nan=False)
 
 
-def _host(value):
+def host_array(value):
+    """A C-contiguous host copy of a numpy array or a (device) torch tensor."""
     if hasattr(value, "detach"):
-        return value.detach().cpu().numpy()
-    return np.asarray(value)
+        value = value.detach().cpu().numpy()
+    return np.ascontiguousarray(value)
 
 
-def _array_descriptor(value):
-    array = np.ascontiguousarray(_host(value))
-    if array.dtype.kind not in "biuf" or not np.isfinite(array).all():
-        raise ValueError("restart arrays must be finite real numeric arrays without pickle")
+def describe(value):
+    """Shape, dtype string and SHA-256 of the C-order bytes of a finite real array."""
+    array = host_array(value)
+    if array.dtype.kind not in "biuf":
+        raise ValueError(f"restart arrays must be real numeric, not dtype {array.dtype}")
+    if array.dtype.kind == "f" and not np.all(np.isfinite(array)):
+        raise ValueError("restart arrays must be finite")
     return {"shape": list(array.shape), "dtype": array.dtype.str,
-            "sha256": hashlib.sha256(array.tobytes(order="C")).hexdigest()}
+            "sha256": hashlib.sha256(array.tobytes()).hexdigest()}
+
+
+def _scalar_float(value):
+    value = float(value)
+    if not np.isfinite(value):
+        raise ValueError("restart contract values must be finite")
+    return value
+
+
+def _mapping(value):
+    bad = [key for key in value if not isinstance(key, str)]
+    if bad:
+        raise ValueError(f"restart contract mappings need string keys, got {bad[0]!r}")
+    return {key: fingerprint(item) for key, item in value.items()}
+
+
+# (test, conversion) pairs, tried in order; anything else is described as an array
+_CANONICAL = (
+    (lambda v: v is None or isinstance(v, (str, bool)), lambda v: v),
+    (lambda v: isinstance(v, (int, np.integer)) and not isinstance(v, (bool, np.bool_)), int),
+    (lambda v: isinstance(v, (float, np.floating)), _scalar_float),
+    (lambda v: isinstance(v, dict), _mapping),
+    (lambda v: is_dataclass(v) and not isinstance(v, type),
+     lambda v: {f.name: fingerprint(getattr(v, f.name)) for f in fields(v)}),
+    (lambda v: hasattr(v, "_asdict"), lambda v: _mapping(v._asdict())),
+    (lambda v: isinstance(v, (list, tuple)), lambda v: [fingerprint(item) for item in v]),
+)
 
 
 def fingerprint(value):
-    """Fingerprint effective values (reference restart.py:35-53)."""
-    if is_dataclass(value):
-        return {f.name: fingerprint(getattr(value, f.name)) for f in fields(value)}
-    if hasattr(value, "_asdict"):
-        return fingerprint(value._asdict())
-    if isinstance(value, dict):
-        if any(not isinstance(k, str) for k in value):
-            raise ValueError("restart mappings require string keys")
-        return {k: fingerprint(v) for k, v in value.items()}
-    if isinstance(value, (tuple, list)):
-        return [fingerprint(v) for v in value]
-    if value is None or isinstance(value, (str, bool, int)):
-        return value
-    if isinstance(value, float):
-        if not np.isfinite(value):
-            raise ValueError("restart contract contains a nonfinite scalar")
-        return value
-    if isinstance(value, (np.integer,)):
-        return int(value)
-    if isinstance(value, (np.floating,)):
-        return fingerprint(float(value))
-    return _array_descriptor(value)
+    """Canonical JSON tree of a contract value: scalars as themselves,
+    records and mappings by field, sequences by item, arrays by descriptor."""
+    for test, convert in _CANONICAL:
+        if test(value):
+            return convert(value)
+    return describe(value)
 
 
 def sha256_file(path):
     digest = hashlib.sha256()
     with open(path, "rb") as stream:
-        for block in iter(lambda: stream.read(1 << 20), b""):
+        while block := stream.read(1 << 20):
             digest.update(block)
     return digest.hexdigest()
 
 
 def effective_params(params):
-    """The frozen, array- and scalar-valued part of a parameter namespace."""
-    out = {}
-    for name, value in sorted(vars(params).items()):
-        if name == "solver" or callable(value):
-            continue
-        if isinstance(value, np.ndarray) or isinstance(value, (int, float, str, bool)) or value is None:
-            out[name] = value
-    return out
+    """The value-carrying part of a frozen parameter namespace: arrays, scalars
+    and strings, by name (handles and callables left out)."""
+    keep = (np.ndarray, int, float, str, bool, type(None))
+    return {name: value for name, value in sorted(vars(params).items())
+            if name != "solver" and isinstance(value, keep)}
 
 
 def make_restart_contract(grid, params, *, dtype, forcing, controls, code_paths, execution):
-    """Reference restart.py:56-73."""
-    dims = [int(grid.nx), int(grid.ny), int(grid.nz)]
+    """The record a restart must match exactly (reference restart.py:56-73):
+    state shapes and dtype, dt, the grid, the effective parameters, forcing,
+    run controls, the code files' SHA-256 and the execution settings."""
     compute = np.dtype(dtype)
-    if compute.name not in {"float32", "float64"}:
-        raise ValueError("restart compute dtype must be float32 or float64")
+    if compute not in (np.dtype(np.float32), np.dtype(np.float64)):
+        raise ValueError(f"restart compute dtype must be float32 or float64, not {compute}")
     dt = float(params.dt)
-    if not np.isfinite(dt) or dt <= 0.0:
+    if not (np.isfinite(dt) and dt > 0.0):
         raise ValueError("restart dt must be finite and positive")
-    eff = effective_params(params) if not hasattr(params, "_asdict") else params
-    return {"schema_version": SCHEMA_VERSION,
-            "state_family": "FD_point_samples_linear_free_surface",
-            "state_shapes": {name: dims[:2] if name in {"eta", "ice"} else dims for name in STATE_FIELDS},
-            "state_dtype": compute.str, "dt_s": dt,
-            "grid": fingerprint(grid), "effective_params": fingerprint(eff),
-            "forcing": fingerprint(forcing), "controls": fingerprint(controls),
+    volume = [int(grid.nx), int(grid.ny), int(grid.nz)]
+    shapes = {name: volume[:2] if name in ("eta", "ice") else volume for name in STATE_FIELDS}
+    values = params if hasattr(params, "_asdict") else effective_params(params)
+    return {"schema_version": SCHEMA_VERSION, "state_family": STATE_FAMILY, "state_shapes": shapes,
+            "state_dtype": compute.str, "dt_s": dt, "grid": fingerprint(grid),
+            "effective_params": fingerprint(values), "forcing": fingerprint(forcing),
+            "controls": fingerprint(controls),
             "sources": {name: sha256_file(path) for name, path in code_paths.items()},
             "execution": fingerprint(execution)}
-
-
-def _integer(value, name):
-    array = np.asarray(value)
-    if array.shape != () or array.dtype.kind not in "iu" or int(array) < 0:
-        raise ValueError(f"restart {name} must be a nonnegative integer scalar")
-    return int(array)
-
-
-def _numeric_mapping(values, name):
-    if not isinstance(values, dict) or any(not isinstance(k, str) or not k or "__" in k for k in values):
-        raise ValueError(f"restart {name} requires nonempty string array names without '__'")
-    arrays = {k: np.array(_host(v), copy=True) for k, v in values.items()}
-    for array in arrays.values():
-        _array_descriptor(array)
-    return arrays
-
-
-def _validate_state(state, contract):
-    if set(state) != set(STATE_FIELDS):
-        raise ValueError("restart must contain all six state fields including ice")
-    arrays = _numeric_mapping(state, "state")
-    for name, array in arrays.items():
-        if list(array.shape) != contract["state_shapes"][name]:
-            raise ValueError(f"restart {name} shape differs from the frozen grid")
-        if array.dtype.str != contract["state_dtype"]:
-            raise ValueError(f"restart {name} dtype differs from the frozen compute dtype")
-    return arrays
-
-
-def _validate_outputs(outputs):
-    if not isinstance(outputs, dict):
-        raise ValueError("restart outputs must be a mapping")
-    for name, digest in outputs.items():
-        if (not isinstance(name, str) or "\\" in name or ":" in name
-                or PurePosixPath(name).is_absolute() or ".." in PurePosixPath(name).parts
-                or not name or not isinstance(digest, str) or len(digest) != 64
-                or any(ch not in "0123456789abcdef" for ch in digest)):
-            raise ValueError("restart output paths/hashes must be safe relative paths and SHA256")
 
 
 @dataclass(frozen=True)
@@ -158,93 +145,146 @@ class RestartRecord:
     outputs: dict
 
 
-@contextmanager
-def atomic_archive(path, *, replace=False):
-    """Same-directory temporary file, fsync, then atomic publication."""
+def _count(value, what):
+    array = np.asarray(value)
+    if array.ndim or array.dtype.kind not in "iu" or array < 0:
+        raise ValueError(f"restart {what} must be one nonnegative integer")
+    return int(array)
+
+
+def _group(values, what):
+    """Host copies of a named array group; names are non-empty and contain no
+    '__' (the archive key separator)."""
+    if not isinstance(values, dict):
+        raise ValueError(f"restart {what} must be a mapping of named arrays")
+    out = {}
+    for name, value in values.items():
+        if not isinstance(name, str) or not name or "__" in name:
+            raise ValueError(f"restart {what} array names must be non-empty strings without '__'")
+        out[name] = np.array(host_array(value), copy=True)
+        describe(out[name])
+    return out
+
+
+def _state_group(state, contract):
+    if hasattr(state, "_asdict"):
+        state = state._asdict()
+    if not isinstance(state, dict) or set(state) != set(STATE_FIELDS):
+        raise ValueError("restart state must hold the six state fields u, v, T, S, eta and ice")
+    arrays = _group(state, "state")
+    for name, array in arrays.items():
+        if list(array.shape) != contract["state_shapes"][name]:
+            raise ValueError(f"restart state {name} has shape {array.shape}, the contract's grid differs")
+        if array.dtype.str != contract["state_dtype"]:
+            raise ValueError(f"restart state {name} has dtype {array.dtype}, the contract's compute dtype differs")
+    return arrays
+
+
+def _safe_outputs(outputs):
+    for name, digest in outputs.items():
+        parts = PurePosixPath(name).parts if isinstance(name, str) else ()
+        relative = (bool(name) and isinstance(name, str) and "\\" not in name and ":" not in name
+                    and not PurePosixPath(name).is_absolute() and ".." not in parts)
+        hexdigest = isinstance(digest, str) and len(digest) == 64 and set(digest) <= set("0123456789abcdef")
+        if not (relative and hexdigest):
+            raise ValueError(f"restart output {name!r} needs a relative path and a SHA-256 hex digest")
+    return outputs
+
+
+def _publish(path, data):
+    """Write ``data`` to a new exclusive file beside ``path``, sync it, rename it
+    over ``path`` and sync the directory; a crash leaves at most the hidden
+    temporary file, never a partial archive under the final name."""
     path = Path(path)
-    tmp = None
+    directory = path.parent if str(path.parent) else Path(".")
+    temporary = directory / f".{path.name}.{secrets.token_hex(6)}.part"
+    fd = os.open(temporary, os.O_WRONLY | os.O_CREAT | os.O_EXCL, 0o644)
     try:
-        with tempfile.NamedTemporaryFile(dir=path.parent, prefix=f".{path.name}.", suffix=".tmp",
-                                         delete=False) as stream:
-            tmp = Path(stream.name)
-            yield stream
+        with os.fdopen(fd, "wb") as stream:
+            stream.write(data)
             stream.flush()
             os.fsync(stream.fileno())
-        if replace:
-            os.replace(tmp, path)
-        else:
-            os.link(tmp, path)
+        os.replace(temporary, path)
     finally:
-        if tmp is not None and tmp.exists():
-            tmp.unlink()
+        if temporary.exists():
+            temporary.unlink()
+    dir_fd = os.open(directory, os.O_RDONLY)
+    try:
+        os.fsync(dir_fd)
+    finally:
+        os.close(dir_fd)
 
 
 def save_restart(path, state, contract, *, step, counters, cumulative, history, outputs=None):
-    """Reference restart.py:153-179."""
+    """Check everything, then publish the archive atomically."""
     if contract.get("schema_version") != SCHEMA_VERSION:
-        raise ValueError("unsupported restart contract version")
-    step = _integer(step, "step")
-    counters = {k: _integer(v, k) for k, v in counters.items()}
-    state = state._asdict() if hasattr(state, "_asdict") else state
-    arrays = {"state": _validate_state(state, contract),
-              "cumulative": _numeric_mapping(cumulative, "cumulative"),
-              "history": _numeric_mapping(history, "history")}
-    outputs = dict(outputs or {})
-    _validate_outputs(outputs)
+        raise ValueError(f"restart contract schema {contract.get('schema_version')!r} is not {SCHEMA_VERSION}")
+    step = _count(step, "step")
+    groups = {"state": _state_group(state, contract), "cumulative": _group(cumulative, "cumulative"),
+              "history": _group(history, "history")}
     elapsed = step * contract["dt_s"]
     if not np.isfinite(elapsed):
-        raise ValueError("restart elapsed time is nonfinite")
+        raise ValueError("restart elapsed time is not finite")
     metadata = {"schema_version": SCHEMA_VERSION, "contract": contract, "step": step,
-                "elapsed_seconds": elapsed, "counters": counters, "outputs": outputs,
-                "arrays": {g: {k: _array_descriptor(v) for k, v in vals.items()} for g, vals in arrays.items()}}
-    metadata_json = _json(metadata)
-    payload = {f"{g}__{k}": v for g, vals in arrays.items() for k, v in vals.items()}
-    payload.update(metadata_json=np.asarray(metadata_json),
-                   metadata_sha256=np.asarray(hashlib.sha256(metadata_json.encode()).hexdigest()))
-    with atomic_archive(path, replace=True) as stream:
-        np.savez_compressed(stream, **payload)
+                "elapsed_seconds": elapsed,
+                "counters": {name: _count(value, name) for name, value in counters.items()},
+                "outputs": _safe_outputs(dict(outputs or {})),
+                "arrays": {g: {name: describe(a) for name, a in arrays.items()} for g, arrays in groups.items()}}
+    text = canonical_json(metadata)
+    members = {f"{g}__{name}": a for g, arrays in groups.items() for name, a in arrays.items()}
+    members["metadata_json"] = np.asarray(text)
+    members["metadata_sha256"] = np.asarray(hashlib.sha256(text.encode()).hexdigest())
+    buffer = io.BytesIO()
+    np.savez_compressed(buffer, **members)
+    _publish(path, buffer.getvalue())
+
+
+def _verified_metadata(archive):
+    if "metadata_json" not in archive.files or "metadata_sha256" not in archive.files:
+        raise ValueError("unversioned checkpoint (no checksummed metadata): not a verified restart")
+    text, checksum = archive["metadata_json"], archive["metadata_sha256"]
+    if text.ndim or checksum.ndim or text.dtype.kind != "U":
+        raise ValueError("restart metadata must be one JSON string and one checksum")
+    text = str(text)
+    if hashlib.sha256(text.encode()).hexdigest() != str(checksum):
+        raise ValueError("restart metadata checksum mismatch")
+    metadata = json.loads(text)
+    if metadata.get("schema_version") != SCHEMA_VERSION:
+        raise ValueError(f"restart schema {metadata.get('schema_version')!r} is not {SCHEMA_VERSION}")
+    return metadata
 
 
 def load_restart(path, expected_contract):
-    """Reference restart.py:182-227: reject before building any device state."""
-    with np.load(path, allow_pickle=False) as saved:
-        if "metadata_json" not in saved or "metadata_sha256" not in saved:
-            raise ValueError("unversioned checkpoint is not a verified restart; migration required")
-        encoded, checksum = saved["metadata_json"], saved["metadata_sha256"]
-        if encoded.shape != () or encoded.dtype.kind != "U" or checksum.shape != ():
-            raise ValueError("restart metadata must be scalar JSON and checksum")
-        metadata_json = str(encoded)
-        if hashlib.sha256(metadata_json.encode()).hexdigest() != str(checksum):
-            raise ValueError("restart metadata checksum mismatch")
-        metadata = json.loads(metadata_json)
-        if metadata.get("schema_version") != SCHEMA_VERSION:
-            raise ValueError("unsupported restart schema version")
-        if _json(metadata["contract"]) != _json(expected_contract):
-            bad = [k for k in expected_contract if metadata["contract"].get(k) != expected_contract[k]]
-            raise ValueError(f"restart contract mismatch: {', '.join(bad)}")
-        step = _integer(metadata["step"], "step")
+    """Read and verify an archive; nothing is returned unless every check holds."""
+    with np.load(path, allow_pickle=False) as archive:
+        metadata = _verified_metadata(archive)
+        saved = metadata["contract"]
+        if canonical_json(saved) != canonical_json(expected_contract):
+            keys = sorted(set(saved) | set(expected_contract))
+            differing = [k for k in keys if canonical_json(saved.get(k)) != canonical_json(expected_contract.get(k))]
+            raise ValueError("restart contract mismatch in: " + ", ".join(differing))
+        step = _count(metadata["step"], "step")
         elapsed = step * expected_contract["dt_s"]
-        if not np.isfinite(elapsed) or metadata["elapsed_seconds"] != elapsed:
-            raise ValueError("restart absolute time differs from step * frozen dt")
-        counters = {k: _integer(v, k) for k, v in metadata["counters"].items()}
-        _validate_outputs(metadata["outputs"])
-        arrays, required = {}, {"metadata_json", "metadata_sha256"}
-        if set(metadata["arrays"]) != {"state", "cumulative", "history"}:
+        if metadata["elapsed_seconds"] != elapsed:
+            raise ValueError("restart elapsed time is not step x the contract's dt")
+        counters = {name: _count(value, name) for name, value in metadata["counters"].items()}
+        outputs = _safe_outputs(metadata["outputs"])
+        if sorted(metadata["arrays"]) != sorted(GROUPS):
             raise ValueError("restart array groups mismatch")
-        for group, descriptors in metadata["arrays"].items():
-            arrays[group] = {}
-            for name, descriptor in descriptors.items():
-                key = f"{group}__{name}"
-                required.add(key)
-                if key not in saved:
-                    raise ValueError(f"restart missing array {key}")
-                value = np.array(saved[key], copy=True)
-                if _array_descriptor(value) != descriptor:
-                    raise ValueError(f"restart array content/shape/dtype mismatch: {key}")
-                arrays[group][name] = value
-            _numeric_mapping(arrays[group], group)
-        if set(saved.files) != required:
-            raise ValueError("restart contains unexpected arrays")
-    state = _validate_state(arrays["state"], expected_contract)
-    return RestartRecord(state, step, elapsed, counters, arrays["cumulative"], arrays["history"],
-                         metadata["outputs"])
+        expected_keys = {"metadata_json", "metadata_sha256"}
+        groups = {}
+        for g in GROUPS:
+            groups[g] = {}
+            for name, descriptor in metadata["arrays"][g].items():
+                key = f"{g}__{name}"
+                expected_keys.add(key)
+                if key not in archive.files:
+                    raise ValueError(f"restart array {key} is missing")
+                array = np.array(archive[key], copy=True)
+                if describe(array) != descriptor:
+                    raise ValueError(f"restart array {key} mismatch (content, shape or dtype)")
+                groups[g][name] = array
+        if set(archive.files) != expected_keys:
+            raise ValueError("restart archive holds arrays its metadata does not list")
+    state = _state_group(groups["state"], expected_contract)
+    return RestartRecord(state, step, elapsed, counters, groups["cumulative"], groups["history"], outputs)

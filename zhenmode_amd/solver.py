@@ -153,10 +153,12 @@ class Solver:
         return rho, pres, w
 
     def stage(self, which, state):
+        """One stage of the step; the fields the stage does not write (a
+        residual stage writes only u, v or T, S) keep the input's values."""
         src = self.coerce(state)
         dst = self.new_state()
-        dst.eta.copy_(src.eta)
-        dst.ice.copy_(src.ice)
+        for a, b in zip(dst, src):
+            a.copy_(b)
         rc = self.lib.zm_stage(self.handle, int(which), ctypes.byref(self.zm_state(src)),
                                ctypes.byref(self.zm_state(dst)), self._stream())
         _lib.check(rc, self.handle)
