@@ -301,17 +301,28 @@ def sin_lat_rows(p):
     return p.f[0, :] / (2.0 * OMEGA)
 
 
+def _laplacian_tables(p):
+    """Face gates (products of 0/1 wet masks, so applying them as one product is
+    exact) and the metric row of the legacy Laplacian, computed once per mask."""
+    tables = getattr(p, "_laplacian_tables", None)
+    if tables is None or tables[0] is not p.wet_mask_z:
+        wm = p.wet_mask_z
+        tables = (wm, wm * east(wm), wm * west(wm), north_edge(wm) * wm, wm * south_edge(wm),
+                  _row(-((sin_lat_rows(p) / p.cos_lat) / R_EARTH), 3), 0.5 * p.inv_dy)
+        p._laplacian_tables = tables
+    return tables
+
+
 def laplacian_h(u, p):
-    """Legacy spherical Laplacian (horizontal.py:181-215)."""
-    wm = p.wet_mask_z
+    """Legacy spherical Laplacian (horizontal.py:181-215): gated second
+    differences in x and y (edge-replicated y ghost) minus (tan(lat)/R) du/dy."""
+    _, gate_e, gate_w, gate_n, gate_s, minus_tan_over_r, half_inv_dy = _laplacian_tables(p)
     ue, uw = east(u), west(u)
-    d2x = ((ue - u) * wm * east(wm) - (u - uw) * wm * west(wm)) * p.inv_dx2
+    d2x = ((ue - u) * gate_e - (u - uw) * gate_w) * p.inv_dx2
     un, us = north_edge(u), south_edge(u)
-    wn, ws = north_edge(wm), south_edge(wm)
-    d2y = ((un - u) * wn * wm - (u - us) * wm * ws) * p.inv_dy2
-    du_dy = (un - us) * (0.5 * p.inv_dy)
-    tan_over_r = (sin_lat_rows(p) / p.cos_lat) / R_EARTH
-    return d2x + d2y + (-du_dy * _row(tan_over_r, u.ndim))
+    d2y = ((un - u) * gate_n - (u - us) * gate_s) * p.inv_dy2
+    du_dy = (un - us) * half_inv_dy
+    return d2x + d2y + du_dy * minus_tan_over_r
 
 
 def cos_face_north(p):
