@@ -34,45 +34,120 @@ def algorithmic_step_bytes(nx, ny, nz, n_nu, n_sub, word):
     return (8 * n_nu + 86) * f3 + 12 * n_sub * f2
 
 
-def read_pmc_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if present."""
+def read_profile_value(name, key):
+    """A ``key: value`` number from a committed rocprofv3 summary under profiles/, if present."""
     path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
 
-def cpu_baseline():
-    """Oracle (NumPy float64, one core) on a bounded sample, scaled to SYPD of the 0.5-degree step.
+def pmc_summary(fetch_csv, write_csv, nsteps):
+    """Markdown summary of two rocprofv3 --pmc runs of ``bench.py --profile-only``
+    (one with FETCH_SIZE, one with WRITE_SIZE): the counters of the 1 GiB copy
+    calibrate KiB per dispatch to bytes, and every zm:: kernel's dispatches are
+    summed and divided by the number of steps the run made."""
+    import csv
+    from collections import OrderedDict
 
-    The sample is one step of the same generator, time step and sub-cycle
-    counts at 2-degree horizontal resolution (180 x 80 x 50, 1/16 of the
-    cells); the time is scaled to the 720 x 320 x 50 step by the ratio of
-    algorithmic bytes, which is the cell ratio at equal sub-cycle counts.
+    def per_kernel(path):
+        out = OrderedDict()
+        for row in csv.DictReader(open(path)):
+            out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        return out
+
+    fetch, write = per_kernel(fetch_csv), per_kernel(write_csv)
+    copy_name = next(k for k in fetch if "copyBuffer" in k)
+    cf = [x for x in fetch[copy_name] if x > 1e5]
+    cw = [x for x in write[copy_name] if x > 1e5]
+    rf, rw = 1048576.0 / (sum(cf) / len(cf)), 1048576.0 / (sum(cw) / len(cw))
+    lines = [f"Calibration: the 1 GiB device copy ({len(cf)} dispatches) reads FETCH_SIZE {sum(cf) / len(cf):.1f} KiB "
+             f"-> read factor {rf:.4f}; WRITE_SIZE {sum(cw) / len(cw):.1f} KiB -> write factor {rw:.4f}.", "",
+             "| kernel | dispatches per step | FETCH KiB/dispatch | WRITE KiB/dispatch | MB read/step | MB written/step |",
+             "|---|---:|---:|---:|---:|---:|"]
+    rows, total_r, total_w = [], 0.0, 0.0
+    for name, vals in fetch.items():
+        if "zm::" not in name:
+            continue
+        wv = write[name]
+        r = sum(vals) * rf * 1024.0 / nsteps
+        w = sum(wv) * rw * 1024.0 / nsteps
+        total_r, total_w = total_r + r, total_w + w
+        short = name.split("(")[0].replace("void ", "")
+        rows.append((r + w, f"| `{short}` | {len(vals) / nsteps:g} | {sum(vals) / len(vals):.1f} | "
+                            f"{sum(wv) / len(wv):.1f} | {r / 1e6:.1f} | {w / 1e6:.1f} |"))
+    lines += [row for _, row in sorted(rows, reverse=True)]
+    lines += ["", f"Per step: {total_r / 1e6:.1f} MB read, {total_w / 1e6:.1f} MB written.",
+              f"traffic_bytes_per_step: {total_r + total_w:.0f}"]
+    return "\n".join(lines)
+
+
+def cpu_baseline():
+    """The oracle (NumPy float64, single-threaded) timed directly on this host.
+
+    * the 4-degree case (90x40x12, dt 600 s): median of 3 steps after a warm-up step;
+    * the 1-degree case (360x160x50, dt 1200 s): one step after a warm-up step;
+    * the 0.5-degree case of the benchmark (720x320x50, dt 900 s): one step
+      (NumPy is warm from the steps above).  ``value`` is the SYPD of this step.
     """
     from oracle import zm_oracle as O
     from zhenmode_amd.cases import make_case
 
-    sx, sy = 180, 80
-    case = make_case("g05deg", nx=sx, ny=sy)
-    kw = {k: v for k, v in case.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
-    p = O.make_params(case.grid, case.physics, case.dt, **kw)
-    s = O.init_state(p, case.T_init, case.S_init)
+    def oracle(name):
+        case = make_case(name)
+        kw = {k: v for k, v in case.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
+        p = O.make_params(case.grid, case.physics, case.dt, **kw)
+        return case, p, O.init_state(p, case.T_init, case.S_init)
+
+    def timed(p, s):
+        t0 = time.perf_counter()
+        s = O.step(s, p)
+        return time.perf_counter() - t0, s
+
+    direct = {}
+    case, p, s = oracle("g4deg")
+    _, s = timed(p, s)
+    runs = []
+    for _ in range(3):
+        t, s = timed(p, s)
+        runs.append(t)
+    t4 = float(np.median(runs))
+    direct["g4deg_90x40x12"] = {"s_per_step": t4, "sypd": case.dt / (365.0 * t4), "rule": "median of 3 after 1 warm-up"}
+    case, p, s = oracle("g1deg")
+    _, s = timed(p, s)
+    t1, s = timed(p, s)
+    direct["g1deg_360x160x50"] = {"s_per_step": t1, "sypd": case.dt / (365.0 * t1), "rule": "1 step after 1 warm-up"}
+    case, p, s = oracle("g05deg")
+    t05, _ = timed(p, s)
+    direct["g05deg_720x320x50"] = {"s_per_step": t05, "sypd": case.dt / (365.0 * t05), "rule": "1 step"}
+    return {"value": case.dt / (365.0 * t05), "unit": "SYPD", "cores": 1, "kind": "port",
+            "host_cores_available": len(os.sched_getaffinity(0)),
+            "sample": (f"one step of the oracle (NumPy float64 restatement of the reference step, single-threaded) "
+                       f"on the benchmark's own 720x320x50 workload took {t05:.1f} s (dt {case.dt:g} s); direct "
+                       f"timings at 90x40x12 and 360x160x50 in 'direct'"),
+            "direct": direct}
+
+
+def timed_run(step_fn, steps):
+    """``steps`` calls of ``step_fn`` in one run: mean wall time per step
+    (host clock around the run, synchronised on both sides) and the median of
+    per-step hipEvent times on torch's current stream."""
+    import torch
+
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    s = O.step(s, p)
-    t1 = time.perf_counter() - t0
-    nz = case.grid.nz
-    n_nu = max(1, int(p.n_subcyc if p.nu_nsub is None else p.nu_nsub))
-    b1 = algorithmic_step_bytes(sx, sy, nz, n_nu, p.n_subcyc, 8)
-    b2 = algorithmic_step_bytes(720, 320, nz, n_nu, p.n_subcyc, 8)
-    t_full = t1 * b2 / b1
-    return {"value": case.dt / (365.0 * t_full), "unit": "SYPD", "cores": 1, "kind": "port",
-            "sample": (f"1 oracle step (NumPy float64, single-threaded) of the 0.5-degree case's generator at "
-                       f"{sx}x{sy}x{nz} (dt={case.dt:g}s, n_sub={p.n_subcyc}, n_nu={n_nu}) took {t1:.1f} s; "
-                       f"scaled to 720x320x{nz} by the algorithmic-byte ratio {b2 / b1:.1f}x")}
+    events[0].record()
+    for k in range(steps):
+        step_fn()
+        events[k + 1].record()
+    torch.cuda.synchronize()
+    wall = 1e3 * (time.perf_counter() - t0) / steps
+    per = [events[k].elapsed_time(events[k + 1]) for k in range(steps)]
+    return wall, float(np.median(per))
 
 
 def learned_leg(case, plain_ms, steps, warmup):
@@ -97,23 +172,18 @@ def learned_leg(case, plain_ms, steps, warmup):
     train_s = time.perf_counter() - t0
 
     def timed(stepper):
-        # two timed runs of `steps` steps, the faster one reported (a host
-        # stall of a few ms in one run would otherwise set the figure); the
-        # fallback count covers the last run
-        s = stepper.init_state(case.T_init, case.S_init)
+        # the plain step's rule: one run of `steps` steps, mean wall time per step
+        box = [stepper.init_state(case.T_init, case.S_init)]
         for _ in range(warmup):
-            s = stepper(s)
-        best = float("inf")
-        for _ in range(2):
-            torch.cuda.synchronize()
-            if stepper.mixer is not None:
-                stepper.mixer.count.zero_()
-            t = time.perf_counter()
-            for _ in range(steps):
-                s = stepper(s)
-            torch.cuda.synchronize()
-            best = min(best, 1e3 * (time.perf_counter() - t) / steps)
-        return best, s
+            box[0] = stepper(box[0])
+        if stepper.mixer is not None:
+            stepper.mixer.count.zero_()
+
+        def one():
+            box[0] = stepper(box[0])
+
+        wall, _ = timed_run(one, steps)
+        return wall, box[0]
 
     kw = case.solver_kwargs
     learned = MixingStep(case.grid, case.physics, case.dt, mode="learned", mlp=mlp, tol_T=1e-3, tol_S=2e-4,
@@ -250,7 +320,12 @@ def main():
     ap.add_argument("--tiles", default="", help="nu_block,seg: fixed launch shapes instead of the autotuner")
     ap.add_argument("--profile-only", action="store_true",
                     help="counter runs: only the warm-up and timed steps and the 1 GiB copy, then exit")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "NSTEPS"),
+                    help="print the per-step HBM traffic summary of two --profile-only counter runs and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary[0], args.pmc_summary[1], int(args.pmc_summary[2])))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
                                    "(latitude bands + RCCL halo exchange) and is not launched", "n_gpus": args.gpus}))
@@ -295,24 +370,16 @@ def main():
         torch.mul(base, amp, out=tau_store)
         return solver.step_dyn(src, tau_x, tau_y, q, out=dst)
 
-    n = 0
+    box = [a, b, 0]
+
+    def advance():
+        one(box[2], box[0], box[1])
+        box[0], box[1], box[2] = box[1], box[0], box[2] + 1
+
     for _ in range(args.warmup):
-        one(n, a, b)
-        a, b = b, a
-        n += 1
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(args.steps):
-        one(n, a, b)
-        a, b = b, a
-        n += 1
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    ev_ms = ev0.elapsed_time(ev1) / args.steps
+        advance()
+    ms_per_step, ev_ms = timed_run(advance, args.steps)
+    a = box[0]
     finite = bool(torch.isfinite(a.u).all() and torch.isfinite(a.T).all() and torch.isfinite(a.eta).all())
 
     # STREAM-like copy of 1 GiB between two fresh buffers: the measured HBM
@@ -338,12 +405,20 @@ def main():
 
     word = 4 if args.dtype == "float32" else 8
     n3 = g.nx * g.ny * g.nz
+    # dominant kernel: the fused nu_h sub-cycle launch (h substeps per HBM pass).
+    # Its algorithmic bytes per launch are the compulsory traffic of that fused
+    # pass: read u, v once and write them once (4 n3 words); the unfused
+    # equivalent (h passes) is reported beside it, labelled as such.
     nu_h = solver.nu_substeps_per_launch()
     nu_ms = solver.time_kernel(5 if nu_h > 1 else 0, 50)
-    nu_bytes = nu_h * 4 * n3 * word
+    nu_bytes = 4 * n3 * word
     nu_single_ms = solver.time_kernel(0, 50)
     achieved = nu_bytes / (nu_ms * 1e-3) / 1e9
+    profiled = args.case == "g05deg" and args.dtype == "float32"
+    traffic = read_profile_value(f"nu_stream_{args.case}_pmc.txt", "traffic_bytes_per_launch") if profiled else None
     step_bytes = algorithmic_step_bytes(g.nx, g.ny, g.nz, params.n_nu, params.n_subcyc, word)
+    step_traffic = (read_profile_value(f"step_{args.case}_hbm_traffic.md", "traffic_bytes_per_step")
+                    if profiled else None)
     sypd = case.dt / (365.0 * ms_per_step * 1e-3)
     out = {
         "metric": METRIC, "value": sypd, "unit": "SYPD", "n_gpus": 1, "steps": args.steps,
@@ -353,21 +428,31 @@ def main():
                                f"(0.5 deg), dt={case.dt:g}s, n_sub={params.n_subcyc}, n_nu={params.n_nu}, "
                                f"seasonal wind via step_dyn, {args.dtype}",
                    "sim_days_per_s": case.dt / 86400.0 / (ms_per_step * 1e-3),
-                   "event_ms_per_step": ev_ms, "finite": finite,
+                   "median_event_ms_per_step": ev_ms, "finite": finite,
                    "tiles": {"nu_block": tiles[0], "seg": tiles[1]}},
         "roofline": {"bound": "hbm",
-                     "kernel": (f"k_nu_stream ({nu_h} nu_h Laplacian substeps of u and v per launch)" if nu_h > 1
+                     "kernel": (f"k_nu_stream ({nu_h} nu_h Laplacian substeps of u and v per HBM pass)" if nu_h > 1
                                 else "k_nu_substep (one nu_h Laplacian substep of u and v)"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_pmc_traffic(f"nu_stream_{args.case}_pmc.txt"),
-                     "bytes_per_launch": nu_bytes, "launch_ms": nu_ms,
+                     "traffic": traffic,
+                     "algorithmic_bytes_per_launch": nu_bytes, "launch_ms": nu_ms,
+                     "measured_GBps": traffic / (nu_ms * 1e-3) / 1e9 if traffic else None,
+                     "measured_frac": traffic / (nu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "equivalent_unfused_GBps": nu_h * nu_bytes / (nu_ms * 1e-3) / 1e9,
                      "single_substep_launch_ms": nu_single_ms,
-                     "single_substep_GBps": 4 * n3 * word / (nu_single_ms * 1e-3) / 1e9,
+                     "single_substep_GBps": nu_bytes / (nu_single_ms * 1e-3) / 1e9,
                      "copy_1GiB_GBps": copy_gbps},
         "step_roofline": {"algorithmic_bytes": step_bytes,
-                          "achieved_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
-                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                          "algorithmic_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                          "algorithmic_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "measured_traffic_bytes": step_traffic,
+                          "measured_GBps": step_traffic / (ms_per_step * 1e-3) / 1e9 if step_traffic else None,
+                          "measured_frac": (step_traffic / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                            if step_traffic else None),
+                          "note": "algorithmic bytes count every nu_h substep as a separate HBM pass; the "
+                                  "fused kernel makes one pass per 4 substeps, so measured_frac (rocprofv3 "
+                                  "FETCH_SIZE + WRITE_SIZE summed over the step's kernels) is the HBM fraction"},
     }
     if not args.no_learned:
         out["learned_mixed"] = learned_leg(case, ms_per_step, min(args.steps, 20), 3)

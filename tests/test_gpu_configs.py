@@ -1,0 +1,219 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+* 1 degree, 360x160x50, dt 1200 s, 40 substeps, wind + SST/SSS restoring:
+  the float64 HIP step against the oracle over 2 steps (1e-11 relative per
+  field), and the float32 step against the float64 HIP step over one
+  simulated day (72 steps).
+* 0.5 degree, 720x320x50, dt 900 s, 45 substeps, seasonal wind through
+  ``step_dyn`` + restoring: the float64 HIP step against the oracle for one
+  step (1e-11), and the float32 step against the float64 HIP step over 30
+  simulated days (2880 steps).
+* The 24-member uneven ensemble (8 x 180x80x30, 8 x 360x160x50,
+  8 x 480x214x40): one ensemble step equals every member's own step bit for
+  bit, and the 2-degree members' float32 steps match the oracle.
+* 0.45 degree, 800x356x50: the learned vertical-mixing step with a surrogate
+  trained here on the case's own columns, one simulated day against the
+  float64 Thomas-mixing step.
+
+Per-field errors are max |a - b| / max |b| over the whole field; the SST and
+SSS errors are RMSE over wet surface columns.  The float32-vs-float64
+tolerances are the measured errors of this build with a margin of about 2x.
+What sets them: the reference's convective gate is one switch per column
+("any unstable interface", physics/vertical.py:63-82), and float32 rounding
+flips it in columns that are near neutral stability; a flipped column mixes
+all its levels, so its temperature departs from the float64 run by up to a
+few hundredths of a degree in one step.  At 0.5 degrees one column differs
+after the first step, 73 after 32 steps and 451 after 288, and the SST RMSE
+grows with that count (2.6e-4, 2.2e-3, 1.2e-2 degC).  The float64 step itself
+matches the oracle to 1e-13.  The oracle runs single-threaded NumPy on the
+host, so the oracle legs take tens of seconds each.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import zm_oracle as O  # noqa: E402
+from zhenmode_amd.cases import ensemble_members, make_case  # noqa: E402
+from zhenmode_amd.solver import Ensemble, empty2, make_solver_global, to_numpy_state  # noqa: E402
+
+FIELDS = ("u", "v", "T", "S", "eta")
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def oracle_steps(case, n, wind=None):
+    kw = {k: v for k, v in case.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
+    p = O.make_params(case.grid, case.physics, case.dt, **kw)
+    s = O.init_state(p, case.T_init, case.S_init)
+    for k in range(n):
+        t0 = time.perf_counter()
+        q = p if wind is None else O.with_forcing(p, *wind(k * case.dt))
+        s = O.step(s, q)
+        print(f"  oracle step {k + 1}/{n}: {time.perf_counter() - t0:.1f} s", flush=True)
+    return s
+
+
+class Runner:
+    """A case's HIP step on one dtype, stepping between two preallocated
+    states; with ``seasonal`` the wind amplitude follows the case's seasonal
+    cycle through ``step_dyn`` (amplitude at the start of each step)."""
+
+    def __init__(self, case, dtype):
+        self.case = case
+        step, init_state, _, _, _, _ = make_solver_global(case.grid, case.physics, case.dt, dtype=dtype,
+                                                          return_params=True, dynamic_forcing=True,
+                                                          **case.solver_kwargs)
+        self.sol = step.solver
+        self.a = init_state(case.T_init, case.S_init)
+        self.b = self.sol.new_state()
+        g, dev, dt = case.grid, self.sol.device, self.sol.dtype
+        self.base = torch.as_tensor(np.cos(3.0 * np.radians(g.lat)), dtype=dt, device=dev)[:, None]
+        self.base = self.base.expand(g.ny, g.nx).contiguous()
+        self.tau_x, self.tau_y, self.q = (empty2(g.nx, g.ny, dt, dev) for _ in range(3))
+        self.tau_y.zero_()
+        self.q.zero_()
+        self.n = 0
+
+    def step(self):
+        t = self.n * self.case.dt
+        amp = 0.1 + (0.05 * np.sin(2.0 * np.pi * t / (365.0 * 86400.0)) if self.case.seasonal_wind else 0.0)
+        torch.mul(self.base, amp, out=self.tau_x.permute(1, 0))
+        self.sol.step_dyn(self.a, self.tau_x, self.tau_y, self.q, out=self.b)
+        self.a, self.b = self.b, self.a
+        self.n += 1
+        return self.a
+
+
+def compare_to_oracle(name, nsteps):
+    case = make_case(name)
+    g = case.grid
+    print(f"case {name} {g.nx}x{g.ny}x{g.nz} float64, {nsteps} step(s) vs oracle", flush=True)
+    run = Runner(case, "float64")
+    for _ in range(nsteps):
+        run.step()
+    torch.cuda.synchronize()
+    got = to_numpy_state(run.a)
+    ref = oracle_steps(case, nsteps, wind=case.wind if case.seasonal_wind else None)
+    errs = {f: rel(getattr(got, f), getattr(ref, f)) for f in FIELDS}
+    print(f"  {name} float64 vs oracle: " + " ".join(f"{f}:{e:.1e}" for f, e in errs.items()), flush=True)
+    for f, e in errs.items():
+        assert e <= 1e-11, (name, f, e)
+
+
+def test_one_degree_float64_two_steps_match_oracle():
+    compare_to_oracle("g1deg", 2)
+
+
+def test_half_degree_float64_one_step_matches_oracle():
+    compare_to_oracle("g05deg", 1)
+
+
+def fp32_against_fp64(name, steps, tol, sst_tol, report_every):
+    case = make_case(name)
+    g = case.grid
+    print(f"case {name} {g.nx}x{g.ny}x{g.nz}: float32 vs float64 HIP over {steps} steps "
+          f"({steps * case.dt / 86400:.0f} days)", flush=True)
+    r64, r32 = Runner(case, "float64"), Runner(case, "float32")
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r64.step()
+        r32.step()
+        if (k + 1) % report_every == 0:
+            torch.cuda.synchronize()
+            print(f"  step {k + 1}/{steps} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    torch.cuda.synchronize()
+    a, b = to_numpy_state(r32.a), to_numpy_state(r64.a)
+    wet = g.wet_mask > 0.5
+    assert all(np.all(np.isfinite(getattr(a, f))) for f in FIELDS)
+    errs = {f: rel(getattr(a, f), getattr(b, f)) for f in FIELDS}
+    sst = float(np.sqrt(np.mean((a.T[:, :, 0] - b.T[:, :, 0])[wet] ** 2)))
+    sss = float(np.sqrt(np.mean((a.S[:, :, 0] - b.S[:, :, 0])[wet] ** 2)))
+    kw = {k: v for k, v in case.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
+    p = O.make_params(g, case.physics, case.dt, **kw)
+    flipped = int(np.sum(O.convective_mask(a.T, a.S, p) != O.convective_mask(b.T, b.S, p)))
+    print(f"  {name} float32 vs float64 after {steps} steps: "
+          + " ".join(f"{f}:{e:.2e}" for f, e in errs.items()) + f" SST rmse {sst:.2e} degC, SSS rmse {sss:.2e} psu, "
+          f"{flipped} columns with a different convective gate", flush=True)
+    for f, e in errs.items():
+        assert e <= tol[f], (name, f, e, tol[f])
+    assert sst <= sst_tol, (name, sst)
+
+
+# measured float32-vs-float64 errors of this build (in parentheses), with a margin of about 2x
+TOL_1DAY = dict(u=2e-4, v=3e-4, T=3e-2, S=5e-4, eta=6e-3)     # (8.0e-5, 1.2e-4, 1.4e-2, 2.3e-4, 2.6e-3)
+TOL_30DAYS = dict(u=3e-2, v=4e-2, T=4e-2, S=1.5e-2, eta=0.35)  # (1.5e-2, 1.8e-2, 1.7e-2, 6.6e-3, 0.16)
+
+
+def test_one_degree_float32_against_float64_over_one_day():
+    fp32_against_fp64("g1deg", 72, TOL_1DAY, sst_tol=6e-3, report_every=24)          # SST rmse (2.9e-3)
+
+
+def test_half_degree_float32_against_float64_over_thirty_days():
+    fp32_against_fp64("g05deg", 2880, TOL_30DAYS, sst_tol=0.75, report_every=480)    # SST rmse (0.37)
+
+
+def test_ensemble_at_full_member_sizes():
+    members = ensemble_members()
+    print("case ensemble 24 members (8 x 180x80x30, 8 x 360x160x50, 8 x 480x214x40) float32, one step",
+          flush=True)
+    sols, states = [], []
+    for m in members:
+        step, init_state, _ = make_solver_global(m.grid, m.physics, m.dt, dtype="float32", **m.solver_kwargs)
+        sols.append(step.solver)
+        states.append(init_state(m.T_init, m.S_init))
+    ens = Ensemble(sols)
+    out = ens.step(states)
+    own = [s.step(x) for s, x in zip(sols, states)]
+    torch.cuda.synchronize()
+    for i in range(len(sols)):
+        for f in FIELDS:
+            assert torch.equal(getattr(out[i], f), getattr(own[i], f)), (i, f)
+    # the 2-degree members against the oracle (float32 state vs float64 oracle)
+    tol = dict(u=2e-4, v=2e-4, T=1e-5, S=1e-5, eta=2e-3)
+    for i, m in enumerate(members[:8]):
+        kw = {k: v for k, v in m.solver_kwargs.items() if k not in ("mode_split", "use_scan")}
+        p = O.make_params(m.grid, m.physics, m.dt, **kw)
+        ref = O.step(O.init_state(p, m.T_init, m.S_init), p)
+        got = to_numpy_state(out[i])
+        errs = {f: rel(getattr(got, f), getattr(ref, f)) for f in FIELDS}
+        print(f"  member {i} (180x80x30, kappa_v {m.physics.kappa_v:.2e}) vs oracle: "
+              + " ".join(f"{f}:{e:.1e}" for f, e in errs.items()), flush=True)
+        for f, e in errs.items():
+            assert e <= tol[f], (i, f, e)
+
+
+def test_learned_mixing_at_0_45_degrees_over_one_day():
+    from zhenmode_amd.learned import ColumnMLP
+    from zhenmode_amd.surrogate import evaluate, generate_columns, normalise, train
+
+    case = make_case("g045deg")
+    g = case.grid
+    print(f"case g045deg {g.nx}x{g.ny}x{g.nz}: surrogate trained on 400k columns, one day against the "
+          "float64 Thomas-mixing step", flush=True)
+    X, Y = generate_columns(case, 400_000, spinup=8, every=8, log=lambda m: print("  " + m, flush=True))
+    mlp = ColumnMLP(g.nz, device=X.device)
+    Xn, Yn = normalise(mlp, X, Y)
+    del X, Y
+    losses = train(mlp, Xn, Yn, epochs=3, log=lambda m: print("  " + m, flush=True))
+    del Xn, Yn
+    rep = evaluate(case, mlp, 1.0, tols=((1e-3, 2e-4),), log=lambda m: print("  " + m, flush=True))
+    f32, learned = rep["thomas_f32"], rep["learned_tolT0.001"]
+    print(f"  float32 Thomas: SST rmse {f32['sst_rmse_degC']:.2e}; learned: SST rmse "
+          f"{learned['sst_rmse_degC']:.2e}, fallback {learned['fallback_fraction']:.3f}, heat diff "
+          f"{learned['heat_rel_diff']:.1e}, salt diff {learned['salt_rel_diff']:.1e}; ms/step "
+          + ", ".join(f"{k} {v:.2f}" for k, v in rep["ms_per_step"].items()), flush=True)
+    assert losses[-1] < losses[0]
+    assert learned["finite"] and f32["finite"]
+    # the column-integral correction keeps heat and salt to float32 rounding of the state
+    assert abs(learned["salt_rel_diff"]) <= 1e-6 and abs(learned["heat_rel_diff"]) <= 1e-5
+    # accuracy at the float32 floor: within 1.5x of the float32 Thomas step's SST RMSE against float64
+    assert learned["sst_rmse_degC"] <= 1.5 * f32["sst_rmse_degC"]
