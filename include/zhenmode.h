@@ -171,6 +171,20 @@ int32_t zm_nu_substeps_per_launch(const zm_handle* h);
 int zm_vertical_mix(zm_handle* h, const void* T, const void* S, void* T_out, void* S_out, double dt,
                     double kappa_v, double kappa_conv, const int32_t* only_columns, void* stream);
 
+/* Allocation-free variant of zm_vertical_mix (same arguments, same results bit
+ * for bit).  No counterpart in the reference: it exists for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The elimination values go to a static
+ * 32 MiB float64 device array held by every GPU the library runs on until the
+ * process ends, and the grid is solved in ranges of
+ * floor(4 Mi / (3 nz) / 256) * 256 columns, one launch per range, in stream
+ * order (27,904 columns per range at nz = 50).  Levels beyond what one
+ * 256-column range fits (nz > 5461) are rejected with ZM_EINVAL.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+int zm_vertical_mix_noalloc(zm_handle* h, const void* T, const void* S, void* T_out, void* S_out, double dt,
+                            double kappa_v, double kappa_conv, const int32_t* only_columns, void* stream);
+
 /* Surrogate input rows X[c][0..255] (bfloat16, c = j*nx + i, rows >= ny*nx,
  * rows beyond ny*nx zero): T, S, u, v on nz levels (0 below the seafloor),
  * N^2 at each level's lower interface, q_heat, the SSS-restoring salt flux,

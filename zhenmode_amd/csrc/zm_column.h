@@ -19,10 +19,20 @@ struct MixParams {
   const double* dzi;  // dz_iface [nz-1], float64
 };
 
-// scratch: 3 nz n2 float64 values
+// work: 3 nz n2 float64 values
 template <typename real>
 void launch_vmix(const Geo<real>& g, const MixParams& m, const real* T, const real* S, real* To, real* So,
-                 const int* only, double* scratch, hipStream_t s);
+                 const int* only, double* work, hipStream_t s);
+
+// The same solve through the library's static device scratch (no allocation):
+// kStaticScratchDoubles float64 per GPU, the grid solved in column ranges of
+// static_vmix_columns(nz) columns; throws std::runtime_error when nz is too
+// large for one 256-column range or the scratch's address cannot be found.
+constexpr long long kStaticScratchDoubles = 4ll << 20;  // 32 MiB
+long long static_vmix_columns(int nz);
+template <typename real>
+void launch_vmix_static(const Geo<real>& g, const MixParams& m, const real* T, const real* S, real* To, real* So,
+                        const int* only, hipStream_t s);
 
 // Feature rows X[c][0..255] (bf16, row c = j*nx + i) of every column:
 // T, S, u, v, N^2 at nz levels, q_heat, SSS-restoring salt flux, |tau|, zero pad;

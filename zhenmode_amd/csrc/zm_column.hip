@@ -12,6 +12,10 @@
 // of physics/eos.py:13-18.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "zm_column.h"
 
 namespace zm {
@@ -37,19 +41,24 @@ __device__ __forceinline__ double mix_kappa(double Tc, double Sc, double Tn, dou
 // solve to flagged columns.  Blocks map to columns in launch order (dealt
 // round-robin over the XCDs): flagged columns cluster in latitude.
 // ---------------------------------------------------------------------------
+//
+// A launch solves the columns [c_begin, c_begin + c_count); its elimination
+// values go to work[3][nz][c_count], so a bounded buffer can serve any grid in
+// column ranges (the arithmetic per column does not depend on the range).
 template <typename real>
 __global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const real* __restrict__ T,
                                               const real* __restrict__ S, real* __restrict__ To,
                                               real* __restrict__ So, const int* __restrict__ only,
-                                              double* __restrict__ scratch) {
-  const long long c2 = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (c2 >= g.n2) return;
+                                              double* __restrict__ work, long long c_begin, long long c_count) {
+  const long long off = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long c2 = c_begin + off;
+  if (off >= c_count || c2 >= g.n2) return;
   if (only && only[c2] == 0) return;
   const int nz = g.nz, nw = g.nwet[c2];
-  const long long n2 = g.n2, n3 = n2 * nz;
-  double* cpl = scratch + c2;
-  double* dTl = cpl + n3;
-  double* dSl = dTl + n3;
+  const long long n2 = g.n2, plane = c_count * nz;
+  double* cpl = work + off;
+  double* dTl = cpl + plane;
+  double* dSl = dTl + plane;
   for (int k = nw; k < nz; ++k) {
     To[k * n2 + c2] = T[k * n2 + c2];
     So[k * n2 + c2] = S[k * n2 + c2];
@@ -71,9 +80,9 @@ __global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const re
     cprev = c * inv;
     dT = (Tc - a * dT) * inv;
     dS = (Sc - a * dS) * inv;
-    cpl[k * n2] = cprev;
-    dTl[k * n2] = dT;
-    dSl[k * n2] = dS;
+    cpl[k * c_count] = cprev;
+    dTl[k * c_count] = dT;
+    dSl[k * c_count] = dS;
     Tc = Tn;
     Sc = Sn;
     km = kp;
@@ -82,9 +91,9 @@ __global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const re
   To[(nw - 1) * n2 + c2] = (real)xT;
   So[(nw - 1) * n2 + c2] = (real)xS;
   for (int k = nw - 2; k >= 0; --k) {
-    const double cp = cpl[k * n2];
-    xT = dTl[k * n2] - cp * xT;
-    xS = dSl[k * n2] - cp * xS;
+    const double cp = cpl[k * c_count];
+    xT = dTl[k * c_count] - cp * xT;
+    xS = dSl[k * c_count] - cp * xS;
     To[k * n2 + c2] = (real)xT;
     So[k * n2 + c2] = (real)xS;
   }
@@ -438,9 +447,61 @@ __global__ __launch_bounds__(256) void k_adam(int n_out, int n_in, float* __rest
 // ---------------------------------------------------------------------------
 template <typename real>
 void launch_vmix(const Geo<real>& g, const MixParams& m, const real* T, const real* S, real* To, real* So,
-                 const int* only, double* scratch, hipStream_t s) {
+                 const int* only, double* work, hipStream_t s) {
   const unsigned blocks = (unsigned)((g.n2 + 255) / 256);
-  hipLaunchKernelGGL(k_vmix<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, To, So, only, scratch);
+  hipLaunchKernelGGL(k_vmix<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, To, So, only, work, 0ll, g.n2);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free Thomas solve: the elimination values live in a file-scope
+// device array of kStaticScratchDoubles float64 (32 MiB, held by every GPU
+// the library runs on until the process ends) and the grid is solved in
+// column ranges that fit it, one launch per range, in stream order.  In host
+// code `scratch` only names the symbol; its device address on the current
+// GPU comes from hipGetSymbolAddress, looked up once per device.  Every call
+// on one GPU shares that GPU's array: calls that overlap on one GPU (different
+// streams, or threads) race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_lock;
+double* scratch_address[kMaxDevices] = {};
+}  // namespace
+
+double* static_scratch() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal beyond the static scratch table");
+  std::lock_guard<std::mutex> guard(scratch_lock);
+  if (!scratch_address[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_address[dev] = static_cast<double*>(p);
+  }
+  return scratch_address[dev];
+}
+
+long long static_vmix_columns(int nz) {
+  const long long width = kStaticScratchDoubles / (3ll * nz) / 256 * 256;
+  return width;
+}
+
+template <typename real>
+void launch_vmix_static(const Geo<real>& g, const MixParams& m, const real* T, const real* S, real* To, real* So,
+                        const int* only, hipStream_t s) {
+  const long long width = static_vmix_columns(g.nz);
+  if (width < 256) throw std::runtime_error("too many levels for the static Thomas scratch");
+  double* work = static_scratch();
+  for (long long c0 = 0; c0 < g.n2; c0 += width) {
+    const long long count = g.n2 - c0 < width ? g.n2 - c0 : width;
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    hipLaunchKernelGGL(k_vmix<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, To, So, only, work, c0, count);
+  }
 }
 
 template <typename real>
@@ -497,6 +558,8 @@ void launch_adam(int n_out, int n_in, float* w, const float* gw, float* m, float
 #define ZM_COL_INSTANTIATE(real)                                                                              \
   template void launch_vmix<real>(const Geo<real>&, const MixParams&, const real*, const real*, real*, real*, \
                                   const int*, double*, hipStream_t);                                          \
+  template void launch_vmix_static<real>(const Geo<real>&, const MixParams&, const real*, const real*, real*,  \
+                                         real*, const int*, hipStream_t);                                     \
   template void launch_col_features<real>(const Geo<real>&, const MixParams&, const real*, const real*,       \
                                           const real*, const real*, const real*, const real*, real,           \
                                           const real*, const real*, const float*, const float*, void*,        \

@@ -55,6 +55,8 @@ struct SolverBase {
   virtual int nu_substeps_per_launch() const = 0;
   virtual void vertical_mix(const void* T, const void* S, void* To, void* So, double dt, double kv, double kc,
                             const int* only, hipStream_t s) = 0;
+  virtual void vertical_mix_static(const void* T, const void* S, void* To, void* So, double dt, double kv,
+                                   double kc, const int* only, hipStream_t s) = 0;
   virtual void column_features(const zm_state& in, const void* tau_x, const void* tau_y, const void* q_heat,
                                const float* mean, const float* inv_std, void* X, long long rows,
                                hipStream_t s) = 0;
@@ -856,6 +858,12 @@ struct Solver : SolverBase {
     ZM_HIP(hipGetLastError());
   }
 
+  void vertical_mix_static(const void* T, const void* S, void* To, void* So, double dt, double kv, double kc,
+                           const int* only, hipStream_t s) override {
+    zm::launch_vmix_static(g, mix(dt, kv, kc), (const real*)T, (const real*)S, (real*)To, (real*)So, only, s);
+    ZM_HIP(hipGetLastError());
+  }
+
   void column_features(const zm_state& in, const void* tau_x, const void* tau_y, const void* q_heat,
                        const float* mean, const float* inv_std, void* X, long long rows, hipStream_t s) override {
     zm::launch_col_features(g, mix(P.dt, 0.0, 0.0), (const real*)in.T, (const real*)in.S, (const real*)in.u,
@@ -1163,6 +1171,19 @@ int zm_vertical_mix(zm_handle* h, const void* T, const void* S, void* T_out, voi
   if (T == T_out || S == S_out) return fail(h, "zm_vertical_mix output must not alias its input", ZM_EINVAL);
   return guarded(h, [&] {
     h->solver->vertical_mix(T, S, T_out, S_out, dt, kappa_v, kappa_conv, only_columns, (hipStream_t)stream);
+  });
+}
+
+int zm_vertical_mix_noalloc(zm_handle* h, const void* T, const void* S, void* T_out, void* S_out, double dt,
+                            double kappa_v, double kappa_conv, const int32_t* only_columns, void* stream) {
+  if (!h || !T || !S || !T_out || !S_out || !(dt >= 0.0) || !(kappa_v >= 0.0) || !(kappa_conv >= 0.0))
+    return fail(h, "invalid zm_vertical_mix_noalloc arguments", ZM_EINVAL);
+  if (T == T_out || S == S_out) return fail(h, "zm_vertical_mix_noalloc output must not alias its input", ZM_EINVAL);
+  if (zm::static_vmix_columns(h->solver->nz_) < 256)
+    return fail(h, "zm_vertical_mix_noalloc: nz too large for the 32 MiB static scratch (3 nz x 256 float64 "
+                   "per column range)", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->vertical_mix_static(T, S, T_out, S_out, dt, kappa_v, kappa_conv, only_columns, (hipStream_t)stream);
   });
 }
 
