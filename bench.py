@@ -150,6 +150,36 @@ def timed_run(step_fn, steps):
     return wall, float(np.median(per))
 
 
+def run_loop_leg(solver, state, steps, check_every):
+    """runtime.run (device monitor after every step, host read every
+    check_every steps) against the same number of bare steps between two
+    preallocated states (frozen forcing for both)."""
+    import torch
+
+    from zhenmode_amd.runtime import run
+
+    a, b = solver.new_state(), solver.new_state()
+    for x, y in zip(a, solver.coerce(state)):
+        x.copy_(y)
+    box = [a, b]
+
+    def bare():
+        solver.step(box[0], out=box[1])
+        box[0], box[1] = box[1], box[0]
+
+    for _ in range(3):
+        bare()
+    bare_ms, _ = timed_run(bare, steps)
+    run(lambda s, out: solver.step(s, out=out), solver, box[0], check_every, check_every=check_every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run(lambda s, out: solver.step(s, out=out), solver, box[0], steps, check_every=check_every)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return {"ms_per_step": ms, "bare_ms_per_step": bare_ms, "overhead_ms_per_step": ms - bare_ms,
+            "check_every": check_every, "accepted_steps": res.accepted_steps, "failure": res.status["failure"]}
+
+
 def learned_leg(case, plain_ms, steps, warmup):
     """The learned+mixed step: the plain FP32 step without explicit tracer
     vertical mixing, followed by the BF16 MFMA column surrogate (FP64
@@ -454,6 +484,20 @@ def main():
                                   "fused kernel makes one pass per 4 substeps, so measured_frac (rocprofv3 "
                                   "FETCH_SIZE + WRITE_SIZE summed over the step's kernels) is the HBM fraction"},
     }
+    # barotropic sub-cycle: one persistent launch of n_sub substeps against n_sub launches of one substep
+    n_sub = int(params.n_subcyc)
+    sub_ms = solver.time_kernel(4, 50)
+    bt = {"n_sub": n_sub, "per_substep_launch_ms": sub_ms, "per_substep_launches_ms": n_sub * sub_ms,
+          "algorithmic_bytes_per_substep": 12 * g.nx * g.ny * word}
+    try:
+        bt["persistent_ms"] = solver.time_kernel(15, 20)
+        bt["speedup"] = n_sub * sub_ms / bt["persistent_ms"]
+        bt["timeouts"] = solver.barotropic_timeouts()
+    except ValueError as e:   # the grid's blocks would not all be resident on this device
+        bt["persistent_ms"] = None
+        bt["note"] = str(e)
+    out["barotropic_subcycle"] = bt
+    out["run_loop"] = run_loop_leg(solver, a, min(args.steps, 40), check_every=10)
     if not args.no_learned:
         out["learned_mixed"] = learned_leg(case, ms_per_step, min(args.steps, 20), 3)
     if not args.no_projection:

@@ -342,6 +342,21 @@ int zm_projection_iterations(zm_handle* h, int32_t* iterations, void* stream);
 /* Launch shapes of a handle: nu_block = nu_h substeps per strip launch
  * (1..4 in float32, 1..4 in float64), seg = levels per thread of the
  * column-marching kernels (1..nz).  Results do not depend on either. */
+/* Barotropic sub-cycle as one persistent launch (enabled = 1; needs its
+ * (ny + 1) / 2 blocks of 1024 threads to fit one per CU) or as one launch per
+ * substep (enabled = 0, the default); results are bit-identical.  No counterpart in the reference:
+ * it selects how the sub-cycle of _free_surface_step_fd
+ * (dynamics/barotropic.py:86-254, driven by timestepping/step.py:410-418) is
+ * launched.  Clears the handle's captured graphs.  ZM_EINVAL when enabling a
+ * grid whose blocks would not all be resident. */
+int zm_set_barotropic_persistent(zm_handle* h, int32_t enabled);
+
+/* Number of bounded waits of the persistent sub-cycle that gave up since the
+ * handle was created (a block waited ~1e6 polls for a neighbour); nonzero
+ * means the steps since then are invalid.  Synchronises the handle's stream.
+ * No counterpart in the reference. */
+int zm_barotropic_timeouts(zm_handle* h, int32_t* count);
+
 int zm_set_tiles(zm_handle* h, int32_t nu_block, int32_t seg);
 int zm_get_tiles(const zm_handle* h, int32_t* nu_block, int32_t* seg);
 

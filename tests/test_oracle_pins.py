@@ -798,3 +798,98 @@ def test_init_state_holds_land_and_ghost_at_the_reference_values():
     assert _init_state_masks(O.init_state)
     assert not _init_state_masks(planted(O.init_state, "S = S * wm + (1.0 - wm) * p.S_ref",
                                          "S = S * wm + (1.0 - wm) * p.T_ref"))
+
+
+# ---------------------------------------------------------------------------
+# 2-D conservative divergence and gradient (horizontal.py:69-124)
+# ---------------------------------------------------------------------------
+def loop_divergence_2d(u, v, p):
+    """horizontal.py:69-97: face-averaged transports, open only between two wet
+    columns, the north wall face closed; the y flux carries the face cosine and
+    the difference is divided by dy cos(lat)."""
+    wm = p.wet_mask
+    nx, ny = wm.shape
+    cos = p.cos_lat
+    out = np.zeros((nx, ny))
+
+    def fx(i, j):
+        ie = (i + 1) % nx
+        return 0.5 * (u[i, j] + u[ie, j]) if wm[i, j] > 0.5 and wm[ie, j] > 0.5 else 0.0
+
+    def fy(i, j):
+        if j < 0 or j == ny - 1 or not (wm[i, j] > 0.5 and wm[i, j + 1] > 0.5):
+            return 0.0
+        return 0.5 * (v[i, j] + v[i, j + 1]) * 0.5 * (cos[j] + cos[j + 1])
+
+    for i in range(nx):
+        for j in range(ny):
+            out[i, j] = (fx(i, j) - fx((i - 1) % nx, j)) / p.dx_2d[i, j] + (fy(i, j) - fy(i, j - 1)) / (p.dy * cos[j])
+    return out
+
+
+def test_2d_divergence_and_gradient_match_index_loops(land):
+    p = land[0]
+    rng = np.random.default_rng(28)
+    shape = p.wet_mask.shape
+    u, v, eta = rng.standard_normal(shape), rng.standard_normal(shape), rng.standard_normal(shape)
+    assert _close(O.divergence_conservative(u, v, p), loop_divergence_2d(u, v, p))
+    gx, gy = O.gradient_conservative(eta, p)
+    # the 2-D gradient is the layer-gated 3-D one of a single level whose mask is the surface mask
+    lx, ly = loop_conservative_gradient(eta[:, :, None], p)
+    assert _close(gx, lx[:, :, 0]) and _close(gy, ly[:, :, 0])
+    bad = planted(O.divergence_conservative, "vopen[:, -1] = 0.0", "pass")   # an open north wall
+    assert not _close(bad(u, v, p), loop_divergence_2d(u, v, p))
+
+
+# ---------------------------------------------------------------------------
+# tracer residual: the linear diffusion is removed (tendencies.py:227-245)
+# ---------------------------------------------------------------------------
+def _tracer_residual_of_diffusion_only(residual):
+    """At rest, with no surface sources and a stably stratified column (no
+    convection), the full tracer tendency is the horizontal and vertical
+    diffusion alone, which the residual removes: it vanishes to round-off,
+    while the tendency itself does not."""
+    p, T, S = _land()
+    p.Q_heat_2d = np.zeros_like(p.Q_heat_2d)
+    p.lambda_bulk = 0.0
+    p.coastal_bulk_lambda_2d = np.zeros_like(p.coastal_bulk_lambda_2d)
+    p.restore_coef_S = 0.0
+    p.coastal_restore_coef_2d = np.zeros_like(p.coastal_restore_coef_2d)
+    shape = p.wet_mask_z.shape
+    T = np.broadcast_to(10.0 + 8.0 * np.exp(p.z / 80.0), shape) + 0.3 * np.random.default_rng(29).standard_normal(shape[:2])[:, :, None]
+    S = np.full(shape, 35.0)
+    zero = np.zeros(shape)
+    assert not O.convective_mask(T, S, p).any()
+    full = O.tracer_tendency(zero, zero, T, S, p)[0]
+    dT, dS = residual(zero, zero, T, S, p)
+    scale = float(np.max(np.abs(full)))
+    return scale > 0.0 and float(np.max(np.abs(dT))) <= 1e-12 * scale and not np.any(dS)
+
+
+def test_tracer_residual_removes_the_linear_diffusion():
+    assert _tracer_residual_of_diffusion_only(O.tracer_residual)
+
+
+@pytest.mark.parametrize("old,new", [("dTdt = dTdt - dvT", "dTdt = dTdt"), ("dTdt = dTdt - dhT", "dTdt = dTdt + dhT")],
+                         ids=["kept-vertical-diffusion", "added-horizontal-diffusion"])
+def test_tracer_residual_planted_mistakes_fail(old, new):
+    assert not _tracer_residual_of_diffusion_only(planted(O.tracer_residual, old, new))
+
+
+# ---------------------------------------------------------------------------
+# dynamic forcing (factory.py:735-750) and the throughput unit
+# ---------------------------------------------------------------------------
+def test_step_with_the_frozen_forcing_passed_at_run_time_is_the_step():
+    p, T, S = _land()
+    s = O.init_state(p, T, S)
+    a = O.step(s, p)
+    b = O.step(s, O.with_forcing(p, p.tau_x_2d.copy(), p.tau_y_2d.copy(), p.Q_heat_2d.copy()))
+    for f in ("u", "v", "T", "S", "eta"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+    c = O.step(s, O.with_forcing(p, 2.0 * p.tau_x_2d, p.tau_y_2d, p.Q_heat_2d))
+    assert not np.array_equal(a.u, c.u)
+
+
+def test_sypd_is_simulated_years_per_wall_day():
+    # dt seconds simulated in t seconds of wall time: (dt / (365 d)) per (t / 1 d)
+    assert O.sypd(900.0, 3.7e-3) == pytest.approx(900.0 / 86400.0 / 365.0 / (3.7e-3 / 86400.0), rel=1e-14)

@@ -112,6 +112,17 @@ struct BtMember {
 
 // One whole barotropic substep in one launch, from (eta_in, u_in, v_in) into
 // separate output buffers (same arithmetic as launch_bt_eta + launch_bt_vel).
+// All n_sub substeps in one persistent launch (the arithmetic of n_sub
+// launch_bt_sub calls, alternating sets so that the last lands in (ea, ua, va);
+// the first reads eta0, u0, v0).  progress: int[(ny + 1) / 2] device scratch,
+// cleared by the launcher; timeouts: int device counter, incremented when a
+// bounded wait gives up (results are then invalid).
+template <typename real>
+void launch_bt_persist(const Geo<real>& g, const Phys<real>& P, const real* eta0, const real* u0, const real* v0,
+                       real* ea, real* ua, real* va, real* eb, real* ub, real* vb, const real* Frx, const real* Fry,
+                       const real* fdt, const real* fden, real dtH, real dt_bt, real drag, int nsub, int* progress,
+                       int* timeouts, hipStream_t s);
+
 template <typename real>
 void launch_bt_sub(const Geo<real>& g, const Phys<real>& P, const real* eta_in, const real* u_in,
                    const real* v_in, real* eta_out, real* u_out, real* v_out, const real* Frx, const real* Fry,

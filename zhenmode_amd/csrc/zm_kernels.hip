@@ -1846,6 +1846,181 @@ __global__ __launch_bounds__(256) void k_dealias_fft(int N, FftPlan plan, const 
 }
 
 // ---------------------------------------------------------------------------
+// The same projection for the production row lengths (720, 800, 480, 360,
+// 180, 90) with the radix plan fixed at compile time: every index division is
+// by a constant, the twiddle exponent r k N / (Ns R) needs no reduction
+// (r < R, k < Ns), the 2-, 3-, 4- and 5-point DFTs are written out (no
+// products with the roots 0 and +-1), complex values move through LDS as one
+// 64- or 128-bit word, and the rows of a workgroup go through each stage
+// together (one barrier per stage for all of them).
+// ---------------------------------------------------------------------------
+template <typename real>
+struct __attribute__((aligned(2 * sizeof(real)))) Cx {
+  real re, im;
+};
+
+template <int N>
+struct FixedFft {
+  // radices in the order used: 4s, then 5s, 3s and 2s
+  static constexpr int radix(int s) {
+    int n = N, c = 0;
+    const int order[4] = {4, 5, 3, 2};
+    for (int q = 0; q < 4; ++q) {
+      while (n % order[q] == 0) {
+        if (c == s) return order[q];
+        n /= order[q];
+        ++c;
+      }
+    }
+    return 0;
+  }
+  static constexpr int stages() {
+    int c = 0;
+    while (radix(c) != 0) ++c;
+    return c;
+  }
+  static constexpr int span(int s) {  // Ns before stage s
+    int p = 1;
+    for (int q = 0; q < s; ++q) p *= radix(q);
+    return p;
+  }
+  static constexpr bool complete() { return span(stages()) == N; }
+};
+
+template <typename real>
+__device__ __forceinline__ Cx<real> cmul(Cx<real> a, Cx<real> w) {
+  return Cx<real>{a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re};
+}
+template <typename real>
+__device__ __forceinline__ Cx<real> cadd(Cx<real> a, Cx<real> b) {
+  return Cx<real>{a.re + b.re, a.im + b.im};
+}
+template <typename real>
+__device__ __forceinline__ Cx<real> csub(Cx<real> a, Cx<real> b) {
+  return Cx<real>{a.re - b.re, a.im - b.im};
+}
+// (-i) z for the forward transform, (+i) z for the inverse
+template <typename real, bool INV>
+__device__ __forceinline__ Cx<real> rot(Cx<real> z) {
+  return INV ? Cx<real>{-z.im, z.re} : Cx<real>{z.im, -z.re};
+}
+
+// y_q = sum_r v_r exp(-+2 pi i r q / R), in place
+template <typename real, int R, bool INV>
+__device__ __forceinline__ void small_dft(Cx<real> (&v)[R]) {
+  if constexpr (R == 2) {
+    const Cx<real> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 3) {
+    const real s = real(0.86602540378443864676);  // sin(2 pi / 3)
+    const Cx<real> a = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    const Cx<real> t{v[0].re - real(0.5) * a.re, v[0].im - real(0.5) * a.im};
+    const Cx<real> u = rot<real, INV>(Cx<real>{s * d.re, s * d.im});
+    v[0] = cadd(v[0], a);
+    v[1] = cadd(t, u);
+    v[2] = csub(t, u);
+  } else if constexpr (R == 4) {
+    const Cx<real> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const Cx<real> t2 = cadd(v[1], v[3]), t3 = rot<real, INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  } else {
+    static_assert(R == 5, "radix");
+    const real c1 = real(0.30901699437494742410), c2 = real(-0.80901699437494742410);  // cos(2 pi/5), cos(4 pi/5)
+    const real s1 = real(0.95105651629515357212), s2 = real(0.58778525229247312917);   // sin(2 pi/5), sin(4 pi/5)
+    const Cx<real> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    const Cx<real> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    const Cx<real> t1{v[0].re + c1 * a1.re + c2 * a2.re, v[0].im + c1 * a1.im + c2 * a2.im};
+    const Cx<real> t2{v[0].re + c2 * a1.re + c1 * a2.re, v[0].im + c2 * a1.im + c1 * a2.im};
+    const Cx<real> u1 = rot<real, INV>(Cx<real>{s1 * b1.re + s2 * b2.re, s1 * b1.im + s2 * b2.im});
+    const Cx<real> u2 = rot<real, INV>(Cx<real>{s2 * b1.re - s1 * b2.re, s2 * b1.im - s1 * b2.im});
+    v[0] = cadd(v[0], cadd(a1, a2));
+    v[1] = cadd(t1, u1);
+    v[4] = csub(t1, u1);
+    v[2] = cadd(t2, u2);
+    v[3] = csub(t2, u2);
+  }
+}
+
+// one Stockham stage over `rows` rows of N points (row-major in src / dst)
+template <typename real, int N, int S, bool INV>
+__device__ __forceinline__ void fixed_stage(const Cx<real>* __restrict__ src, Cx<real>* __restrict__ dst,
+                                            const Cx<real>* __restrict__ tw, int rows) {
+  constexpr int R = FixedFft<N>::radix(S), Ns = FixedFft<N>::span(S), T = N / R, tstep = N / (Ns * R);
+  for (int id = (int)threadIdx.x; id < rows * T; id += (int)blockDim.x) {
+    const int row = id / T, bf = id - row * T;
+    const int k = bf % Ns;
+    const Cx<real>* in = src + row * N;
+    Cx<real> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v[r] = in[bf + r * T];
+      if (Ns > 1 && r > 0) {
+        Cx<real> w = tw[r * k * tstep];  // < N: no reduction
+        if (INV) w.im = -w.im;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    small_dft<real, R, INV>(v);
+    Cx<real>* out = dst + row * N + (bf - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[q * Ns] = v[q];
+  }
+}
+
+// all stages; returns the buffer holding the result
+template <typename real, int N, int S, bool INV>
+__device__ __forceinline__ Cx<real>* fixed_stages(Cx<real>* src, Cx<real>* dst, const Cx<real>* tw, int rows) {
+  if constexpr (S == FixedFft<N>::stages()) {
+    return src;
+  } else {
+    fixed_stage<real, N, S, INV>(src, dst, tw, rows);
+    __syncthreads();
+    return fixed_stages<real, N, S + 1, INV>(dst, src, tw, rows);
+  }
+}
+
+template <typename real>
+constexpr int fixed_fft_rows() {
+  return sizeof(real) == 4 ? 4 : 2;  // keeps the LDS of N = 800 rows within 64 KiB
+}
+
+template <typename real, int N>
+__global__ __launch_bounds__(256) void k_dealias_fixed(const int* __restrict__ kept, int nrows,
+                                                       const real* __restrict__ twr, const real* __restrict__ twi,
+                                                       const real* __restrict__ ina, const real* __restrict__ inb,
+                                                       real* __restrict__ outa, real* __restrict__ outb) {
+  static_assert(FixedFft<N>::complete(), "N must factor into 2, 3, 4 and 5");
+  constexpr int RB = fixed_fft_rows<real>();
+  extern __shared__ unsigned char zm_lds[];
+  Cx<real>* tw = reinterpret_cast<Cx<real>*>(zm_lds);
+  Cx<real>* A = tw + N;
+  Cx<real>* B = A + RB * N;
+  const int row0 = (int)blockIdx.x * RB;
+  const int rows = nrows - row0 < RB ? nrows - row0 : RB;
+  const long long off = (long long)row0 * N;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = Cx<real>{twr[t], twi[t]};
+  for (int t = threadIdx.x; t < rows * N; t += blockDim.x) A[t] = Cx<real>{ina[off + t], inb[off + t]};
+  __syncthreads();
+  Cx<real>* X = fixed_stages<real, N, 0, false>(A, B, tw, rows);
+  for (int t = threadIdx.x; t < rows * N; t += blockDim.x) {
+    const int i = t % N;
+    if (!kept[i]) X[t] = Cx<real>{real(0), real(0)};
+  }
+  __syncthreads();
+  Cx<real>* Y = fixed_stages<real, N, 0, true>(X, X == A ? B : A, tw, rows);
+  const real invN = real(1) / real(N);
+  for (int t = threadIdx.x; t < rows * N; t += blockDim.x) {
+    const Cx<real> y = Y[t];
+    outa[off + t] = y.re * invN;
+    outb[off + t] = y.im * invN;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Momentum residual (tendencies.py:39-72 and 247-319, legacy mode-split
 // branch) fused with the RK2 velocity update (step.py:263-319).
 //   mode 0: du1 -> buffers, u_pred = u + du1 dt
@@ -2010,12 +2185,27 @@ __device__ __forceinline__ real cap_weight(const Phys<real>& P, int j, int ny) {
 
 constexpr int kRowChunk = 4;  // cells per thread per row (nx <= 1024)
 
+// Loads and stores of the barotropic state.  COH = true makes them agent-scope
+// relaxed atomics (cache-coherent across the XCDs without flushing or
+// invalidating the L2: the persistent sub-cycle exchanges halo rows this way);
+// COH = false is a plain access.  The value moved is the same either way.
+template <bool COH, typename real>
+__device__ __forceinline__ real bt_load(const real* p) {
+  if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool COH, typename real>
+__device__ __forceinline__ void bt_store(real* p, real v) {
+  if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
 // eta' = eta - dt H div(ubt) on row j with the polar-cap blend of band rows;
 // reads eta_row[i] (row j of eta), writes out[i] (and out2[i] unless null).
 // out may alias eta_row.  t: thread within the 256-thread row group; ALLRED:
 // reduce on every row so that the block-wide barriers match across groups
 // (the zonal mean is only used on band rows).
-template <typename real, bool ALLRED = false>
+template <typename real, bool ALLRED = false, bool COH = false>
 __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
                                            const real* __restrict__ eta_row, const real* __restrict__ ubt,
                                            const real* __restrict__ vbt, real dtH, real* out, real* red,
@@ -2035,9 +2225,9 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
     const int n2 = hasN ? c2 + g.nx : c2, s2 = hasS ? c2 - g.nx : c2;  // clamped: unused at the walls
     // every index is in bounds: load unconditionally, then select
     const real wC = wet[c2], wE = wet[e2], wW = wet[w2], wN = wet[n2], wS = wet[s2];
-    const real uC = ubt[c2], uE = ubt[e2], uW = ubt[w2];
-    const real vC = vbt[c2], vN = vbt[n2], vS = vbt[s2];
-    const real eC = eta_row[i];
+    const real uC = bt_load<COH>(ubt + c2), uE = bt_load<COH>(ubt + e2), uW = bt_load<COH>(ubt + w2);
+    const real vC = bt_load<COH>(vbt + c2), vN = bt_load<COH>(vbt + n2), vS = bt_load<COH>(vbt + s2);
+    const real eC = bt_load<COH>(eta_row + i);
     const bool wc = wC != real(0);
     const real fe = (wc && wE != real(0)) ? real(0.5) * (uC + uE) : real(0);
     const real fw = (wW != real(0) && wc) ? real(0.5) * (uW + uC) : real(0);
@@ -2065,7 +2255,7 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
     real e = en[q];
     if (wb >= real(0)) e = wet[c2] != real(0) ? wb * zm + (real(1) - wb) * e : real(0);
     out[i] = e;
-    if (out2) out2[i] = e;
+    if (out2) bt_store<COH>(out2 + i, e);
   }
 }
 
@@ -2073,7 +2263,7 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
 // pointers; en / es are only read when that row exists): gradient, forcing,
 // legacy implicit Coriolis, drag, mask, polar-cap blend.  Outputs may alias
 // the inputs.
-template <typename real, bool ALLRED = false>
+template <typename real, bool ALLRED = false, bool COH = false>
 __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
                                            const real* ec_row, const real* en_row, const real* es_row,
                                            const real* __restrict__ ubt, const real* __restrict__ vbt, real* uo,
@@ -2100,7 +2290,7 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
     const real wN = wet[hasN ? c2 + g.nx : c2], wS = wet[hasS ? c2 - g.nx : c2];
     const real ec = ec_row[i], eE = ec_row[wrap_e(i, g.nx)], eW = ec_row[wrap_w(i, g.nx)];
     const real eN = en_safe[i], eS = es_safe[i];
-    const real u0 = ubt[c2], v0 = vbt[c2];
+    const real u0 = bt_load<COH>(ubt + c2), v0 = bt_load<COH>(vbt + c2);
     const bool wc = wC != real(0);
     const real dxp = (wc && wE != real(0)) ? eE - ec : real(0);
     const real dxm = (wc && wW != real(0)) ? ec - eW : real(0);
@@ -2143,8 +2333,8 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
       a = wc ? wb * zu + (real(1) - wb) * a : real(0);
       b = wc ? wb * zv + (real(1) - wb) * b : real(0);
     }
-    uo[c2] = a;
-    vo[c2] = b;
+    bt_store<COH>(uo + c2, a);
+    bt_store<COH>(vo + c2, b);
   }
 }
 
@@ -2214,6 +2404,124 @@ __global__ __launch_bounds__(1024) void k_bt_sub(Geo<real> g, Phys<real> P, cons
   const int sl = jc - (j0 - 1);  // LDS slot of row jc (1 or 2)
   bt_vel_row<real, true>(g, P, jc, t, &erow[sl][0], &erow[sl + 1][0], &erow[sl - 1][0], u_in, v_in, u_out, v_out, Frx,
                          Fry, fdt[jc], fden[jc], dt_bt, drag, &red[grp][0], own);
+}
+
+// The whole sub-cycle of n_sub substeps in one persistent launch.  Block b
+// keeps k_bt_sub's work (rows 2b, 2b + 1, the same row arithmetic, so every
+// substep is bit-identical to k_bt_sub's) and steps through the substeps on
+// its own; the only dependence between substeps is on the two neighbouring
+// blocks (their halo rows), so instead of a grid-wide barrier each block
+// publishes the number of substeps it has finished in progress[b] and, before
+// substep q > 0, waits until both neighbours have finished substep q - 1.
+// That one wait also orders the ping-pong buffers: the set a block writes in
+// substep q was last read by its neighbours in substep q - 1.
+// Coherence without cache flushes: every load and store of eta, ubt, vbt is
+// an agent-scope relaxed atomic (coherent across the XCDs' L2 caches), each
+// wave waits until its stores have completed (s_waitcnt 0) before the block
+// barrier, and thread 0 then stores the progress count, also as an
+// agent-scope atomic; a waiting block polls the neighbours' counts the same
+// way.  (Agent-scope fences would write back and invalidate the whole L2 of
+// the XCD from every wave: ten times slower than one launch per substep.)
+// The waits are bounded: after kBtSpinLimit polls a wait records a timeout in
+// *timeouts and returns, and every later wait returns at once while
+// *timeouts is nonzero, so the kernel always finishes (with wrong values,
+// which the host detects from the counter).  All blocks must be resident at
+// once: the grid is ceil(ny / 2) blocks of 1024 threads, which the host
+// allows only up to the device's CU count, and the kernel runs alone on its
+// stream.  progress[] must be zero at launch.
+constexpr int kBtSpinLimit = 1 << 20;
+
+__device__ __forceinline__ void bt_wait(int* progress, int b, int nblocks, int q, int* timeouts) {
+  const int* ps = progress + (b > 0 ? b - 1 : b);
+  const int* pn = progress + (b + 1 < nblocks ? b + 1 : b);
+  for (int spins = 0;; ++spins) {
+    // both counters in flight together (a missing neighbour reads this block's own count, which is >= q)
+    const int s = __hip_atomic_load(ps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int n = __hip_atomic_load(pn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s >= q && n >= q) return;
+    if ((spins & 63) == 63) {
+      if (__hip_atomic_load(timeouts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+      if (spins >= kBtSpinLimit) {
+        __hip_atomic_fetch_add(timeouts, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+    }
+  }
+}
+
+template <typename real>
+__global__ __launch_bounds__(1024) void k_bt_persist(Geo<real> g, Phys<real> P, const real* __restrict__ eta0,
+                                                     const real* __restrict__ u0, const real* __restrict__ v0,
+                                                     real* ea, real* ua, real* va, real* eb, real* ub, real* vb,
+                                                     const real* __restrict__ Frx, const real* __restrict__ Fry,
+                                                     const real* __restrict__ fdt, const real* __restrict__ fden,
+                                                     real dtH, real dt_bt, real drag, int nsub, int* progress,
+                                                     int* timeouts) {
+  __shared__ real erow[4][kRowChunk * 256];
+  __shared__ real red[4][16];
+  // The rows, wet mask, forcing and wind are the same in every substep; left
+  // to itself the compiler hoists their index arithmetic and loads out of the
+  // substep loop, which keeps them live across it and spills (a 1024-thread
+  // block has 128 VGPRs per thread).  The first row and the base pointers are
+  // therefore read from LDS after a barrier in every substep, which the
+  // compiler cannot move out of the loop.
+  __shared__ const real* invariant[5];
+  __shared__ int first_row;
+  if (threadIdx.x == 0) {
+    invariant[0] = P.wet2;
+    invariant[1] = P.tau_x;
+    invariant[2] = P.tau_y;
+    invariant[3] = Frx;
+    invariant[4] = Fry;
+    first_row = (int)blockIdx.x * 2;
+  }
+  const int grp = (int)threadIdx.x >> 8, t = (int)threadIdx.x & 255;
+  const int b = (int)blockIdx.x, nblocks = (int)gridDim.x;
+  const real *ei = eta0, *ui = u0, *vi = v0;
+  for (int q = 0; q < nsub; ++q) {
+    // the last substep lands in set a, the sets alternate backwards from there
+    const bool last_half = ((nsub - 1 - q) & 1) == 0;
+    real* eo = last_half ? ea : eb;
+    real* uo = last_half ? ua : ub;
+    real* vo = last_half ? va : vb;
+    if (q > 0 && threadIdx.x == 0) bt_wait(progress, b, nblocks, q, timeouts);
+    __syncthreads();
+    Phys<real> Pq = P;
+    Pq.wet2 = invariant[0];
+    Pq.tau_x = invariant[1];
+    Pq.tau_y = invariant[2];
+    const real* Frq = invariant[3];
+    const real* Fyq = invariant[4];
+    const int j0 = first_row;
+    {
+      const int je = j0 - 1 + grp;
+      const bool in = je >= 0 && je < g.ny;
+      const int jc = in ? je : (je < 0 ? 0 : g.ny - 1);
+      const bool own = in && (grp == 1 || grp == 2);
+      bt_eta_row<real, true, true>(g, Pq, jc, t, ei + (long long)jc * g.nx, ui, vi, dtH, &erow[grp][0],
+                                   &red[grp][0], own ? eo + (long long)jc * g.nx : (real*)nullptr);
+    }
+    __syncthreads();
+    {
+      const int jv = j0 + (grp & 1);
+      const bool own = grp < 2 && jv < g.ny;
+      const int jc = jv < g.ny ? jv : g.ny - 1;
+      const int sl = jc - (j0 - 1);
+      bt_vel_row<real, true, true>(g, Pq, jc, t, &erow[sl][0], &erow[sl + 1][0], &erow[sl - 1][0], ui, vi, uo,
+                                   vo, Frq, Fyq, fdt[jc], fden[jc], dt_bt, drag, &red[grp][0], own);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(progress + b, q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ei = eo;
+    ui = uo;
+    vi = vo;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zero_int(int* __restrict__ a, int n) {
+  const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (i < n) a[i] = 0;
 }
 
 // The substep's two kernels for many independent grids in one launch each:
@@ -2539,6 +2847,17 @@ void launch_dealias_x(int N, int N1, int N2, const int* kept, int nrows, const r
 template <typename real>
 void launch_dealias_fft(int N, const FftPlan& plan, const int* kept, int nrows, const real* twr, const real* twi,
                         const real* ina, const real* inb, real* outa, real* outb, hipStream_t s) {
+  constexpr int RB = fixed_fft_rows<real>();
+  const size_t lds_fixed = (size_t)(1 + 2 * RB) * N * sizeof(Cx<real>);
+  const unsigned fixed_blocks = (unsigned)((nrows + RB - 1) / RB);
+#define ZM_FIXED_FFT(LEN)                                                                                    if (N == LEN) {                                                                                              hipLaunchKernelGGL((k_dealias_fixed<real, LEN>), dim3(fixed_blocks), dim3(256), lds_fixed, s, kept, nrows,                        twr, twi, ina, inb, outa, outb);                                                        return;                                                                                                  }
+  ZM_FIXED_FFT(720)
+  ZM_FIXED_FFT(800)
+  ZM_FIXED_FFT(480)
+  ZM_FIXED_FFT(360)
+  ZM_FIXED_FFT(180)
+  ZM_FIXED_FFT(90)
+#undef ZM_FIXED_FFT
   const size_t lds = (size_t)6 * N * sizeof(real);
   const unsigned blocks = (unsigned)((nrows + kFftRowsPerBlock - 1) / kFftRowsPerBlock);
   hipLaunchKernelGGL(k_dealias_fft<real>, dim3(blocks), dim3(256), lds, s, N, plan, kept, nrows, twr, twi, ina, inb,
@@ -2574,6 +2893,17 @@ void launch_bt_sub(const Geo<real>& g, const Phys<real>& P, const real* eta_in, 
                    const real* fdt, const real* fden, real dtH, real dt_bt, real drag, hipStream_t s) {
   hipLaunchKernelGGL(k_bt_sub<real>, dim3((unsigned)((g.ny + 1) / 2)), dim3(1024), 0, s, g, P, eta_in, u_in, v_in,
                      eta_out, u_out, v_out, Frx, Fry, fdt, fden, dtH, dt_bt, drag);
+}
+
+template <typename real>
+void launch_bt_persist(const Geo<real>& g, const Phys<real>& P, const real* eta0, const real* u0, const real* v0,
+                       real* ea, real* ua, real* va, real* eb, real* ub, real* vb, const real* Frx, const real* Fry,
+                       const real* fdt, const real* fden, real dtH, real dt_bt, real drag, int nsub, int* progress,
+                       int* timeouts, hipStream_t s) {
+  const int nblocks = (g.ny + 1) / 2;
+  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, s, progress, nblocks);
+  hipLaunchKernelGGL(k_bt_persist<real>, dim3((unsigned)nblocks), dim3(1024), 0, s, g, P, eta0, u0, v0, ea, ua, va,
+                     eb, ub, vb, Frx, Fry, fdt, fden, dtH, dt_bt, drag, nsub, progress, timeouts);
 }
 
 template <typename real>
@@ -2655,6 +2985,10 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
                                     const real*, const real*, const real*, const real*, real, real,         \
                                     hipStream_t);                                                            \
   template void launch_bt_multi<real>(const BtMember<real>*, int, int, hipStream_t);                        \
+  template void launch_bt_persist<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,        \
+                                        const real*, real*, real*, real*, real*, real*, real*, const real*,     \
+                                        const real*, const real*, const real*, real, real, real, int, int*,     \
+                                        int*, hipStream_t);                                                     \
   template void launch_bt_sub<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, const real*,  \
                                     real*, real*, real*, const real*, const real*, const real*, const real*,   \
                                     real, real, real, hipStream_t);                                             \

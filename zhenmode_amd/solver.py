@@ -212,6 +212,18 @@ class Solver:
         _lib.check(self.lib.zm_projection_iterations(self.handle, ptr(its), self._stream()), self.handle)
         return [int(x) for x in its.cpu()]
 
+    def set_barotropic_persistent(self, enabled):
+        """Run the barotropic sub-cycle as one persistent launch (True) or one
+        launch per substep (False); results are bit-identical."""
+        _lib.check(self.lib.zm_set_barotropic_persistent(self.handle, int(bool(enabled))), self.handle)
+
+    def barotropic_timeouts(self):
+        """Bounded waits of the persistent sub-cycle that gave up since the handle
+        was created (nonzero: the steps since then are invalid)."""
+        n = ctypes.c_int32(0)
+        _lib.check(self.lib.zm_barotropic_timeouts(self.handle, ctypes.byref(n)), self.handle)
+        return int(n.value)
+
     def set_tiles(self, nu_block, seg):
         _lib.check(self.lib.zm_set_tiles(self.handle, int(nu_block), int(seg)), self.handle)
 
