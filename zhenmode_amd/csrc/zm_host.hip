@@ -1050,15 +1050,21 @@ struct zm_handle {
   std::string err;
 };
 
-// Uneven float32 members advanced together: the 3-D stages of each member in
-// turn, the barotropic sub-cycle as one batched launch per substep over a
-// member table, the final masks of each member; the whole ensemble step is one
-// hipGraph launch.
+// Uneven float32 members advanced together: the 3-D stages of the members on
+// kEnsembleLanes side streams (member i on lane i mod kEnsembleLanes, so the
+// kernels of different members can run at once; captured as parallel graph
+// branches), the barotropic sub-cycle as one batched launch per substep over
+// a member table on the main stream, then the final masks of the members on
+// the side streams again; the whole ensemble step is one hipGraph launch.
+constexpr int kEnsembleLanes = 4;
+
 struct zm_ensemble {
   std::vector<Solver<float>*> mem;
   zm::BtMember<float>* table = nullptr;
   int total_rows = 0, n_sub = 0;
   hipStream_t own = nullptr;
+  hipStream_t lane[kEnsembleLanes] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kEnsembleLanes] = {};
   bool use_graph = true;
   std::map<std::vector<const void*>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   std::string err;
@@ -1068,18 +1074,50 @@ struct zm_ensemble {
       hipGraphExecDestroy(kv.second.second);
       hipGraphDestroy(kv.second.first);
     }
+    for (int k = 0; k < kEnsembleLanes; ++k) {
+      if (lane[k]) hipStreamDestroy(lane[k]);
+      if (join_ev[k]) hipEventDestroy(join_ev[k]);
+    }
+    if (fork_ev) hipEventDestroy(fork_ev);
     if (own) hipStreamDestroy(own);
     if (table) hipFree(table);
   }
 
+  void make_lanes() {
+    for (int k = 0; k < kEnsembleLanes; ++k) {
+      ZM_HIP(hipStreamCreateWithFlags(&lane[k], hipStreamNonBlocking));
+      ZM_HIP(hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming));
+    }
+    ZM_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+  }
+
+  // the side streams start after everything issued on s so far
+  void fork(hipStream_t s) {
+    ZM_HIP(hipEventRecord(fork_ev, s));
+    for (int k = 0; k < kEnsembleLanes; ++k) ZM_HIP(hipStreamWaitEvent(lane[k], fork_ev, 0));
+  }
+
+  // s continues after everything issued on the side streams
+  void join(hipStream_t s) {
+    for (int k = 0; k < kEnsembleLanes; ++k) {
+      ZM_HIP(hipEventRecord(join_ev[k], lane[k]));
+      ZM_HIP(hipStreamWaitEvent(s, join_ev[k], 0));
+    }
+  }
+
   void issue(const zm_state* in, const zm_state* out, hipStream_t s) {
     const int n = (int)mem.size();
+    fork(s);
     for (int i = 0; i < n; ++i) {
-      mem[i]->pre_barotropic(in[i], mem[i]->P, s);
-      mem[i]->bt_prepare(mem[i]->P, (const float*)in[i].eta, s);
+      hipStream_t ls = lane[i % kEnsembleLanes];
+      mem[i]->pre_barotropic(in[i], mem[i]->P, ls);
+      mem[i]->bt_prepare(mem[i]->P, (const float*)in[i].eta, ls);
     }
+    join(s);
     for (int q = 0; q < n_sub; ++q) zm::launch_bt_multi(table, n, total_rows, s);
-    for (int i = 0; i < n; ++i) mem[i]->post_barotropic(in[i], out[i], mem[i]->P, s);
+    fork(s);
+    for (int i = 0; i < n; ++i) mem[i]->post_barotropic(in[i], out[i], mem[i]->P, lane[i % kEnsembleLanes]);
+    join(s);
   }
 
   void step(const zm_state* in, const zm_state* out, hipStream_t s) {
@@ -1517,6 +1555,7 @@ int zm_ensemble_create(zm_handle* const* members, int32_t n, zm_ensemble** out) 
     ZM_HIP(hipMalloc(&e->table, host.size() * sizeof(zm::BtMember<float>)));
     ZM_HIP(hipMemcpy(e->table, host.data(), host.size() * sizeof(zm::BtMember<float>), hipMemcpyHostToDevice));
     ZM_HIP(hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking));
+    e->make_lanes();
   });
   if (rc != ZM_OK) {
     delete e;
