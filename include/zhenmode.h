@@ -132,6 +132,15 @@ int zm_step_dyn(zm_handle* h, const zm_state* in, const zm_state* out, const voi
 /* diagnostics(state) -> (rho, p, w) (factory.py:752-760); 3-D outputs. */
 int zm_diagnostics(zm_handle* h, const zm_state* in, void* rho, void* pressure, void* w, void* stream);
 
+/* terms_fn(state) -> _tracer_terms(state, params) (factory.py:763-764,
+ * dynamics/tendencies.py:74-92): terms = device array [6][nz][ny][nx] of the
+ * handle's dtype receiving the temperature's advection, horizontal diffusion,
+ * vertical diffusion, convection (one pass with kappa_conv, no sub-cycle), GM
+ * and Redi tendencies (the last two zero: the closures are not supported), each
+ * masked to wet nodes.  Uses the handle's workspace (calls on one handle are
+ * serialised, as for every entry point). */
+int zm_tracer_terms(zm_handle* h, const zm_state* in, void* terms, void* stream);
+
 /* One stage of the step (ZM_STAGE_*), for stage-level parity checks. */
 int zm_stage(zm_handle* h, int32_t stage, const zm_state* in, const zm_state* out, void* stream);
 

@@ -767,6 +767,22 @@ def tracer_tendency(u, v, T, S, p, rec=None):
     return dTdt, dSdt, (diff_h_T, diff_h_S, diff_v_T, diff_v_S)
 
 
+def tracer_terms(u, v, T, S, p):
+    """_tracer_terms (tendencies.py:74-92): the (6, nx, ny, nz) stack of the
+    temperature's [advection, horizontal diffusion, vertical diffusion,
+    convection (one pass with kappa_conv), GM, Redi] tendencies, wet-masked
+    (GM and Redi are zero in this configuration)."""
+    Fz = vertical_transport_iface(u, v, p)
+    adv = advection_scalar(T, u, v, Fz, p)
+    diff_h = tracer_diffusion(T, p)
+    diff_v = vertical_diffusion(T, p)
+    unstable = unstable_interfaces(T, S, p)
+    gate = np.any(unstable, axis=-1, keepdims=True)
+    conv = conv_flux_tendency(T, gate, p.kappa_conv, p, unstable if getattr(p, "localize_conv", False) else None)
+    zero = np.zeros_like(T)
+    return np.stack([t * p.wet_mask_z for t in (adv, diff_h, diff_v, conv, zero, zero)])
+
+
 def tracer_residual(u, v, T, S, p, rec=None):
     """Tendency minus the L-step's linear diffusion, add-then-subtract (tendencies.py:227-245)."""
     dTdt, dSdt, (dhT, dhS, dvT, dvS) = tracer_tendency(u, v, T, S, p, rec=rec)

@@ -318,3 +318,34 @@ def test_convective_kernels_on_few_levels(nz, nc):
     s0 = perturbed_state(p, T, S)
     assert O.convective_mask(s0.T, s0.S, p).any()
     check(step.solver.step(JaxStateG(*s0)), O.step(s0, p), TOL64, "step")
+
+
+@pytest.mark.parametrize("flags", [{}, {"fct_adv": True}, {"monotone_adv": True},
+                                   {"localize_conv": True, "conservative_kv": True}],
+                         ids=["centred", "fct", "monotone", "localized-conservative"])
+@pytest.mark.parametrize("dtype,tol", [("float64", 1e-11), ("float32", 1e-4)])
+def test_terms_fn_matches_oracle(flags, dtype, tol):
+    """terms_fn (_tracer_terms, tendencies.py:74-92) against the oracle: each of
+    the advection, horizontal and vertical diffusion and convection planes
+    relative to its own maximum; GM and Redi are zero.  float32 is compared
+    with the oracle on the same float32-rounded state."""
+    print(f"case land terms_fn {flags} {dtype}", flush=True)
+    grid, physics, dt, kwargs, T, S = land_case()
+    kw = dict(kwargs, **flags)
+    step, _, _, params, terms_fn = make_solver_global(grid, physics, dt, return_params=True, dtype=dtype,
+                                                      mode_split=True, **kw)
+    p = oracle_params(grid, physics, dt, kw)
+    s0 = perturbed_state(p, T, S)
+    if dtype == "float32":
+        s0 = O.State(*(np.asarray(a, dtype=np.float32).astype(np.float64) for a in s0))
+    got = terms_fn(JaxStateG(*s0))
+    torch.cuda.synchronize()
+    got = got.double().cpu().numpy()
+    ref = O.tracer_terms(s0.u, s0.v, s0.T, s0.S, p)
+    assert got.shape == ref.shape
+    for q, name in enumerate(("adv", "diff_h", "diff_v", "conv")):
+        assert np.max(np.abs(ref[q])) > 0.0, name
+        e = rel(got[q], ref[q])
+        print(f"  {name}: {e:.1e}", flush=True)
+        assert e <= tol, (name, e)
+    assert not got[4:].any()

@@ -29,7 +29,7 @@ import pytest
 
 from oracle import zm_ledger as L
 from oracle import zm_oracle as O
-from tests.fixtures import (diffusion_grid, land_case, oracle_params, planted, quiet_physics,
+from tests.fixtures import (diffusion_grid, land_case, oracle_params, perturbed_state, planted, quiet_physics,
                             surface_fixture)
 
 
@@ -893,3 +893,41 @@ def test_step_with_the_frozen_forcing_passed_at_run_time_is_the_step():
 def test_sypd_is_simulated_years_per_wall_day():
     # dt seconds simulated in t seconds of wall time: (dt / (365 d)) per (t / 1 d)
     assert O.sypd(900.0, 3.7e-3) == pytest.approx(900.0 / 86400.0 / 365.0 / (3.7e-3 / 86400.0), rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# terms_fn: _tracer_terms (tendencies.py:74-92)
+# ---------------------------------------------------------------------------
+def _terms_sum_to_the_tendency(terms_fn, localize=False):
+    """With one convective substep and no temperature sources the temperature
+    tendency of _compute_tracer_tendency is exactly the sum of the four
+    process terms that _tracer_terms reports ("mirrors _compute_tracer_tendency")."""
+    from dataclasses import replace
+
+    grid, physics, dt, kwargs, T, S = land_case()
+    physics = replace(physics, kappa_conv=1e-3)       # conv_nsub = ceil(1e-3 * 600 / (0.4 * 100)) = 1
+    kw = {k: v for k, v in kwargs.items() if k not in ("T_atm", "lambda_bulk", "coastal_restore_mask",
+                                                       "coastal_restore_T", "coastal_bulk_mask")}
+    kw["forcing"] = (kwargs["forcing"][0], kwargs["forcing"][1], np.zeros_like(kwargs["forcing"][2]))
+    kw["localize_conv"] = localize
+    p = oracle_params(grid, physics, dt, kw)
+    assert p.conv_nsub == 1 and p.adv_nsub == 1
+    s = perturbed_state(p, T, S, amp=0.2)
+    terms = terms_fn(s.u, s.v, s.T, s.S, p)
+    dTdt = O.tracer_tendency(s.u, s.v, s.T, s.S, p)[0]
+    scale = float(np.max(np.abs(terms)))
+    return (terms.shape == (6,) + s.T.shape and not terms[4:].any() and bool(terms[3].any())
+            and float(np.max(np.abs(terms[:4].sum(axis=0) - dTdt))) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("localize", [False, True])
+def test_tracer_terms_sum_to_the_tracer_tendency(localize):
+    assert _terms_sum_to_the_tendency(O.tracer_terms, localize)
+
+
+@pytest.mark.parametrize("old,new", [
+    ("diff_v = vertical_diffusion(T, p)", "diff_v = np.zeros_like(T)"),
+    ("adv = advection_scalar(T, u, v, Fz, p)", "adv = advection_scalar(S, u, v, Fz, p)"),
+], ids=["dropped-vertical-diffusion", "salinity-advection"])
+def test_tracer_terms_planted_mistakes_fail(old, new):
+    assert not _terms_sum_to_the_tendency(planted(O.tracer_terms, old, new))

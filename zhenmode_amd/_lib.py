@@ -69,7 +69,8 @@ EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", 
            "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
            "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget",
            "zm_project_velocity", "zm_projection_iterations", "zm_projection_operator", "zm_budget_step",
-           "zm_vertical_mix_noalloc", "zm_set_barotropic_persistent", "zm_barotropic_timeouts")
+           "zm_vertical_mix_noalloc", "zm_set_barotropic_persistent", "zm_barotropic_timeouts",
+           "zm_tracer_terms")
 
 # epilogues of zm_gemm_bf16_nt
 EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
@@ -100,6 +101,8 @@ def load():
     lib.zm_step_dyn.restype = ctypes.c_int
     lib.zm_diagnostics.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp, vp, vp]
     lib.zm_diagnostics.restype = ctypes.c_int
+    lib.zm_tracer_terms.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp]
+    lib.zm_tracer_terms.restype = ctypes.c_int
     lib.zm_stage.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ZmState), ctypes.POINTER(ZmState), vp]
     lib.zm_stage.restype = ctypes.c_int
     lib.zm_set_graph.argtypes = [vp, ctypes.c_int32]

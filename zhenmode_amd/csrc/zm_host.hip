@@ -50,6 +50,7 @@ struct SolverBase {
   virtual void step(const zm_state& in, const zm_state& out, int nsteps, const void* const* forcing,
                     hipStream_t s) = 0;
   virtual void diagnostics(const zm_state& in, void* rho, void* p, void* w, hipStream_t s) = 0;
+  virtual void tracer_terms(const zm_state& in, void* terms, hipStream_t s) = 0;
   virtual void stage(int which, const zm_state& in, const zm_state& out, hipStream_t s) = 0;
   virtual double time_kernel(int which, int reps) = 0;
   virtual int nu_substeps_per_launch() const = 0;
@@ -127,6 +128,21 @@ struct Solver : SolverBase {
   int* bt_timeout = nullptr;
   real *ubt2, *vbt2, *etaw2;  // the other half of the barotropic ping-pong
   hipStream_t own = nullptr;
+  // two side streams for the stages of a step that do not depend on each other
+  // (captured as parallel branches of the step graph), joined by events
+  hipStream_t sideA = nullptr, sideB = nullptr;
+  static constexpr int kEvents = 16;
+  // captured step graphs kept per handle, one per (input, output, forcing)
+  // pointer set: a run loop stepping through a ring of n device states needs n
+  static constexpr size_t kGraphCache = 32;
+  // false while an ensemble issues this member's stages: the members then run
+  // side by side on the ensemble's own streams and each member's stages stay
+  // on the stream it is given
+  bool fork_stages = true;
+  hipStream_t lane_a(hipStream_t s) const { return fork_stages ? sideA : s; }
+  hipStream_t lane_b(hipStream_t s) const { return fork_stages ? sideB : s; }
+  hipEvent_t ev_pool[kEvents] = {};
+  int ev_next = 0;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
   template <typename T>
@@ -159,7 +175,20 @@ struct Solver : SolverBase {
       hipGraphDestroy(kv.second.first);
     }
     if (own) hipStreamDestroy(own);
+    if (sideA) hipStreamDestroy(sideA);
+    if (sideB) hipStreamDestroy(sideB);
+    for (hipEvent_t e : ev_pool)
+      if (e) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
+  }
+
+  // `waiter` continues only after everything issued on `src` so far
+  void after(hipStream_t waiter, hipStream_t src) {
+    if (waiter == src) return;
+    hipEvent_t e = ev_pool[ev_next];
+    ev_next = (ev_next + 1) % kEvents;
+    ZM_HIP(hipEventRecord(e, src));
+    ZM_HIP(hipStreamWaitEvent(waiter, e, 0));
   }
 
   explicit Solver(const zm_config& c) {
@@ -399,6 +428,9 @@ struct Solver : SolverBase {
       bt_persist = false;
     }
     ZM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    ZM_HIP(hipStreamCreateWithFlags(&sideA, hipStreamNonBlocking));
+    ZM_HIP(hipStreamCreateWithFlags(&sideB, hipStreamNonBlocking));
+    for (hipEvent_t& e : ev_pool) ZM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ZM_HIP(hipDeviceSynchronize());
   }
 
@@ -554,7 +586,12 @@ struct Solver : SolverBase {
   void linear(const real* u0, const real* v0, const real* T0, const real* S0, real* uo, real* vo, real* To,
               real* So, hipStream_t s) {
     const bool bih_u = P.nu_bi > real(0), bih_t = P.kappa_bi > real(0);
-    if (bih_t) zm::launch_d1_unit(g, T0, S0, DT, DS, s);
+    // the inner D1 of the tracer biharmonic (memory-bound) beside the nu_h chain (VALU-bound)
+    const hipStream_t A = lane_a(s);
+    if (bih_t) {
+      after(A, s);
+      zm::launch_d1_unit(g, T0, S0, DT, DS, A);
+    }
     const real* su = u0;
     const real* sv = v0;
     int n = 0, flip = 0;
@@ -579,22 +616,34 @@ struct Solver : SolverBase {
       sv = dv;
       flip ^= 1;
     }
+    if (bih_t) after(s, A);
     zm::launch_linear_finish(g, P, u0, v0, T0, S0, su, sv, Lu, Lv, DT, DS, uo, vo, To, So, s);
   }
 
   void nonlinear(const zm::Phys<real>& Pf, const real* eta, real* uo, real* vo, real* To, real* So,
                  hipStream_t s) {
-    // tracer residual #1 at the N-step start (u1, v1, T1, S1)
+    // tracer residual #1 at the N-step start (u1, v1, T1, S1); the convective
+    // gate and sub-cycle (side stream A) need only T1, S1, the momentum
+    // advection and its de-aliasing (side stream B) only u1, v1 and Fz1
+    const hipStream_t A = lane_a(s), B = lane_b(s);
+    after(A, s);
+    zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, ubits, A);
     zm::launch_fz(g, u1, v1, Fz1, s);
-    zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, ubits, s);
+    after(B, s);
+    zm::launch_mom_adv(g, u1, v1, Fz1, au, av, B);
+    dealias(au, av, axu, axv, B);
+    after(s, A);
     zm::launch_tracer_residual(g, Pf, 0, u1, v1, T1, S1, Fz1, cT, cS, gate, dT1, dS1, T1, S1, Tp, Sp, s);
     if (ledger_on) ledger_tracer(Pf, u1, v1, T1, S1, Fz1, s);
+    // the second convective pass needs only the predicted tracers (and the
+    // convective buffers free again): side stream A, beside momentum residual #1
+    after(A, s);
+    zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, ubits, A);
     // momentum residual #1 at (u1, v1, T_pred, S_pred)
     zm::launch_pbc(g, Pf, Tp, Sp, pbc, s);
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, (const real*)nullptr, (const real*)nullptr, (real*)nullptr,
                       (real*)nullptr, s);
-    zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
-    dealias(au, av, axu, axv, s);
+    after(s, B);
     zm::launch_mom_residual(g, Pf, 0, axu, axv, u1, v1, pbc, eta, Frx, Fry, du1, dv1, up, vp, decay_bot, s);
     // tracer residual #2 at (u_adv, v_adv, T_pred, S_pred) -> T_new, with
     // (u_adv, v_adv) the predictor velocity or its column-divergence projection
@@ -611,7 +660,7 @@ struct Solver : SolverBase {
       zm::launch_fz(g, up, vp, Fz1, s);
       zm::launch_ledger_transport(g, area64, (const real*)Fz1, (const real*)Fz2, led_part, led_raw, s);
     }
-    zm::launch_conv(g, Pf, Tp, Sp, cT, cS, gate, ubits, s);
+    after(s, A);
     zm::launch_tracer_residual(g, Pf, 1, uad, vad, Tp, Sp, Fz2, cT, cS, gate, dT1, dS1, T1, S1, To, So, s);
     if (ledger_on) ledger_tracer(Pf, uad, vad, Tp, Sp, Fz2, s);
     // momentum residual #2 at (u1, v1, T_new, S_new); advection of (u1, v1) is reused
@@ -798,7 +847,7 @@ struct Solver : SolverBase {
     GraphKey gk{key, nsteps};
     auto it = graphs.find(gk);
     if (it == graphs.end()) {
-      if (graphs.size() >= 8) {
+      if (graphs.size() >= kGraphCache) {
         auto first = graphs.begin();
         ZM_HIP(hipStreamSynchronize(s));
         hipGraphExecDestroy(first->second.second);
@@ -819,6 +868,21 @@ struct Solver : SolverBase {
       it = graphs.emplace(gk, std::make_pair(graph, exec)).first;
     }
     ZM_HIP(hipGraphLaunch(it->second.second, s));
+  }
+
+  // _tracer_terms (tendencies.py:74-92) into terms[6][nz][ny][nx]: advection,
+  // horizontal diffusion, vertical diffusion and single-pass convection of T
+  // on wet nodes, then zero GM and Redi planes (neither closure is supported)
+  void tracer_terms(const zm_state& in, void* terms, hipStream_t s) override {
+    const real *u = (const real*)in.u, *v = (const real*)in.v, *T = (const real*)in.T, *S = (const real*)in.S;
+    zm::launch_fz(g, u, v, Fz1, s);
+    zm::Phys<real> Pc = P;
+    Pc.conv_nsub = 1;
+    zm::launch_conv(g, Pc, T, S, cT, cS, gate, ubits, s);
+    real* o = (real*)terms;
+    zm::launch_tracer_terms(g, P, u, v, T, S, Fz1, cT, cS, gate, o, o + n3, o + 2 * n3, o + 3 * n3, s);
+    ZM_HIP(hipMemsetAsync(o + 4 * n3, 0, (size_t)2 * n3 * sizeof(real), s));
+    ZM_HIP(hipGetLastError());
   }
 
   void diagnostics(const zm_state& in, void* rho, void* pres, void* w, hipStream_t s) override {
@@ -1107,6 +1171,18 @@ struct zm_ensemble {
 
   void issue(const zm_state* in, const zm_state* out, hipStream_t s) {
     const int n = (int)mem.size();
+    // Each member's stages stay on its lane: forking them further inside the
+    // ensemble graph (stage streams nested in lane streams) ended the first
+    // capture of a 24-member ensemble in a segmentation fault in the runtime.
+    struct InStream {
+      std::vector<Solver<float>*>& m;
+      explicit InStream(std::vector<Solver<float>*>& v) : m(v) {
+        for (auto* x : m) x->fork_stages = false;
+      }
+      ~InStream() {
+        for (auto* x : m) x->fork_stages = true;
+      }
+    } in_stream(mem);
     fork(s);
     for (int i = 0; i < n; ++i) {
       hipStream_t ls = lane[i % kEnsembleLanes];
@@ -1237,6 +1313,11 @@ int zm_step_dyn(zm_handle* h, const zm_state* in, const zm_state* out, const voi
 int zm_diagnostics(zm_handle* h, const zm_state* in, void* rho, void* pressure, void* w, void* stream) {
   if (!h || !in || !rho || !pressure || !w) return fail(h, "invalid zm_diagnostics arguments", ZM_EINVAL);
   return guarded(h, [&] { h->solver->diagnostics(*in, rho, pressure, w, (hipStream_t)stream); });
+}
+
+int zm_tracer_terms(zm_handle* h, const zm_state* in, void* terms, void* stream) {
+  if (!h || !in || !terms) return fail(h, "invalid zm_tracer_terms arguments", ZM_EINVAL);
+  return guarded(h, [&] { h->solver->tracer_terms(*in, terms, (hipStream_t)stream); });
 }
 
 int zm_stage(zm_handle* h, int32_t which, const zm_state* in, const zm_state* out, void* stream) {

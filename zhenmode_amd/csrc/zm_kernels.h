@@ -46,6 +46,13 @@ template <typename real>
 // and the conv sub-cycle of the gated columns into cT, cS.
 void launch_conv(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, real* cT, real* cS,
                  int* gate, unsigned long long* ubits, hipStream_t s);
+// _tracer_terms (tendencies.py:74-92): the temperature's advection,
+// horizontal diffusion, vertical diffusion and convection tendencies on wet
+// nodes (zero elsewhere); cT, cS, gate from launch_conv with conv_nsub = 1.
+template <typename real>
+void launch_tracer_terms(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
+                         const real* S, const real* Fz, const real* cT, const real* cS, const int* gate, real* adv,
+                         real* diff_h, real* diff_v, real* conv, hipStream_t s);
 template <typename real>
 void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, const real* u, const real* v,
                             const real* T, const real* S, const real* Fz, const real* cT, const real* cS,

@@ -1231,7 +1231,12 @@ __device__ __forceinline__ real adv_scalar(const real* __restrict__ X, const rea
 // of the surface sources, the advection and convection tendencies (signed and
 // absolute, times the node volume) and the surface advective flux Fz_top*T_top
 // times the cell area, kLedgerTracerSums values per block in lpart.
-template <typename real, int ADV, bool LED>
+// TERMS = true is the diagnostic instantiation of _tracer_terms
+// (tendencies.py:74-92): it writes the temperature's advection, horizontal
+// diffusion, vertical diffusion and convection tendencies, masked to wet
+// nodes, into dT1, dS1, To and So (in that order) and nothing else; cT must
+// then hold the single-pass convective tendency (conv_nsub = 1, kappa_conv).
+template <typename real, int ADV, bool LED, bool TERMS = false>
 __global__ __launch_bounds__(256) void k_tracer_residual(
     Geo<real> g, Phys<real> P, int mode, const real* __restrict__ u, const real* __restrict__ v,
     const real* __restrict__ T, const real* __restrict__ S, const real* __restrict__ Fz,
@@ -1295,6 +1300,7 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
     const real uE = u[pl + n.e2], uW = u[pl + n.w2], vN = v[pl + n.n2e], vS = v[pl + n.s2e];
     const real cTo = cT[o], cSo = cS[o];
     real dTdt = 0, dSdt = 0;
+    real term_adv = 0, term_conv = 0;
     if (wc) {
       // centred face-gated horizontal fluxes (transport.py:224-290)
       const real uxe = h * (uc + uE), uxw = h * (uW + uc);
@@ -1348,6 +1354,8 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
       const real advS = -(hS + (dnS - upS) * ldc(g.idzn, k));
       const real convT = gated ? cTo : real(0);
       const real convS = gated ? cSo : real(0);
+      term_adv = advT;
+      term_conv = convT;
       dTdt = advT + dhT + dvT;
       dSdt = advS + dhS + dvS;
       if constexpr (LED) {
@@ -1394,7 +1402,12 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
     dSdt = dSdt - dhS;
     dTdt = dTdt - dvT;
     dSdt = dSdt - dvS;
-    if constexpr (!LED) {
+    if constexpr (TERMS) {
+      dT1[o] = term_adv;
+      dS1[o] = wc ? dhT : real(0);
+      To[o] = wc ? dvT : real(0);
+      So[o] = term_conv;
+    } else if constexpr (!LED) {
       if (mode == 0) {
         dT1[o] = dTdt;
         dS1[o] = dSdt;
@@ -2796,6 +2809,24 @@ void launch_tracer_residual(const Geo<real>& g, const Phys<real>& P, int mode, c
 }
 
 template <typename real>
+void launch_tracer_terms(const Geo<real>& g, const Phys<real>& P, const real* u, const real* v, const real* T,
+                         const real* S, const real* Fz, const real* cT, const real* cS, const int* gate, real* adv,
+                         real* diff_h, real* diff_v, real* conv, hipStream_t s) {
+  const double* nd = nullptr;
+  double* np_ = nullptr;
+  const real* nr = nullptr;
+  if (P.adv_scheme == 2)
+    hipLaunchKernelGGL((k_tracer_residual<real, 2, false, true>), dim3(segs3(g)), dim3(256), 0, s, g, P, 0, u, v,
+                       T, S, Fz, cT, cS, gate, adv, diff_h, nr, nr, diff_v, conv, nd, nd, np_);
+  else if (P.adv_scheme == 1)
+    hipLaunchKernelGGL((k_tracer_residual<real, 1, false, true>), dim3(segs3(g)), dim3(256), 0, s, g, P, 0, u, v,
+                       T, S, Fz, cT, cS, gate, adv, diff_h, nr, nr, diff_v, conv, nd, nd, np_);
+  else
+    hipLaunchKernelGGL((k_tracer_residual<real, 0, false, true>), dim3(segs3(g)), dim3(256), 0, s, g, P, 0, u, v,
+                       T, S, Fz, cT, cS, gate, adv, diff_h, nr, nr, diff_v, conv, nd, nd, np_);
+}
+
+template <typename real>
 unsigned tracer_ledger_blocks(const Geo<real>& g) {
   return segs3(g);
 }
@@ -2957,6 +2988,9 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
   template void launch_fz<real>(const Geo<real>&, const real*, const real*, real*, hipStream_t);            \
   template void launch_conv<real>(const Geo<real>&, const Phys<real>&, const real*, const real*, real*,     \
                                   real*, int*, unsigned long long*, hipStream_t);                            \
+  template void launch_tracer_terms<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,     \
+                                          const real*, const real*, const real*, const real*, const real*,   \
+                                          const int*, real*, real*, real*, real*, hipStream_t);              \
   template void launch_tracer_residual<real>(const Geo<real>&, const Phys<real>&, int, const real*,         \
                                              const real*, const real*, const real*, const real*,            \
                                              const real*, const real*, const int*, real*, real*,            \

@@ -152,6 +152,17 @@ class Solver:
         _lib.check(rc, self.handle)
         return rho, pres, w
 
+    def tracer_terms(self, state):
+        """_tracer_terms (tendencies.py:74-92): a (6, nx, ny, nz) device tensor of
+        the temperature's [advection, horizontal diffusion, vertical diffusion,
+        convection, GM, Redi] tendencies on wet nodes (GM and Redi zero)."""
+        src = self.coerce(state)
+        nx, ny, nz = self.shape3
+        out = torch.empty((6, nz, ny, nx), dtype=self.dtype, device=self.device)
+        rc = self.lib.zm_tracer_terms(self.handle, ctypes.byref(self.zm_state(src)), ptr(out), self._stream())
+        _lib.check(rc, self.handle)
+        return out.permute(0, 3, 2, 1)
+
     def stage(self, which, state):
         """One stage of the step; the fields the stage does not write (a
         residual stage writes only u, v or T, S) keep the input's values."""
@@ -286,8 +297,7 @@ def make_solver_global(grid, physics, dt, forcing=None, *, return_params=False, 
         return solver.diagnostics(state, surface_pressure_pa)
 
     def terms_fn(state):
-        raise NotImplementedError("the per-process tracer term diagnosis (_tracer_terms) is not part of "
-                                  "the HIP step")
+        return solver.tracer_terms(state)
 
     def step_dyn(state, tau_x, tau_y, q_heat, T_atm_3d=None, surface_pressure_pa=None):
         return solver.step_dyn(state, tau_x, tau_y, q_heat, T_atm_3d, surface_pressure_pa)
