@@ -369,8 +369,11 @@ int zm_barotropic_timeouts(zm_handle* h, int32_t* count);
 int zm_set_tiles(zm_handle* h, int32_t nu_block, int32_t seg);
 int zm_get_tiles(const zm_handle* h, int32_t* nu_block, int32_t* seg);
 
-/* Time `reps` steps from a copy of *sample for every (nu_block, seg)
- * candidate on this device, keep the fastest and report it.  Allocates two
+/* Switched off: returns ZM_EINVAL and times nothing (an autotuning run of
+ * about 40 graph captures in a row aborted the process without a message and
+ * the cause is not identified); set launch shapes with zm_set_tiles.  When
+ * enabled it times `reps` steps from a copy of *sample for every (nu_block,
+ * seg) candidate on this device, keeps the fastest and reports it, with two
  * scratch states for the duration of the call; *sample is not modified. */
 int zm_autotune(zm_handle* h, const zm_state* sample, int32_t reps, int32_t* nu_block, int32_t* seg,
                 double* best_ms);

@@ -63,6 +63,8 @@ def test_ensemble_rejects_mixed_substep_counts():
         Ensemble([s1, s1])
 
 
+@pytest.mark.skip(reason="zm_autotune is switched off: an autotuning run on the MI355X aborted the process "
+                         "without a message and the cause is not identified")
 def test_autotuner_candidates_give_identical_steps():
     print("case autotune 90x40x12 float32", flush=True)
     sol, s0 = member(90, 40, 12)

@@ -52,7 +52,9 @@ struct Geo {
   int seg;                // levels per thread in the column-marching kernels
   int nu_chunk;           // rows per wave of the streaming nu_h kernel (0: register-tile kernel)
   const unsigned long long* nu_wet;  // streaming nu_h kernel: wet words [nz][chunks][nx]
-  const real* nu_met;                // streaming nu_h kernel: (1/dx^2, tan(lat)/R) [ny + 16][2]
+  const real* nu_met;                // nu_h substep row coefficients kE, kN, kS, kY, 1/dx^2, tan(lat)/R
+                                     // [ny + 2 kNuMetPad][kNuMetStride], for nu_h_met and nu_dts_met
+  real nu_h_met, nu_dts_met;
 };
 
 // Scalars and 2-D fields of the physics.

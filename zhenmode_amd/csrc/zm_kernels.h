@@ -12,12 +12,15 @@ struct FftPlan {
   int rad[16];
 };
 
-// Set-up data of the streaming nu_h kernel (Geo::nu_wet, Geo::nu_met): the
+// Set-up data of the nu_h sub-cycle kernels (Geo::nu_wet, Geo::nu_met): the
 // wet word of (level k, chunk ch, column i) holds bit b = node (i, ch * chunk -
-// kNuWetBase + b, k) is wet; row j's metrics sit at 2 (j + kNuMetPad).
+// kNuWetBase + b, k) is wet; row j's coefficients sit at kNuMetStride (j +
+// kNuMetPad): kE, kN, kS, kY for nu_h dts (the substep's update), then
+// 1/dx^2 and tan(lat)/R (its Laplacian).
 constexpr int kNuWetBase = 4;                     // >= the substeps per launch
 constexpr int kNuChunkMax = 64 - 2 * kNuWetBase;  // a chunk and its margins fit one 64-bit word
 constexpr int kNuMetPad = 8;                      // > 2 * substeps - 1
+constexpr int kNuMetStride = 8;
 
 template <typename real>
 void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,

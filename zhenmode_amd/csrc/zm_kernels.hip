@@ -52,9 +52,35 @@ __device__ __forceinline__ real lap_sum(real dx, real dy, real dyc, real id2, re
   return fma(dx, id2, fma(dy, idy2, -(dyc * hidy) * tr));
 }
 
+// One explicit nu_h substep x + nu_h dts lap(x) in neighbour differences
+// d = neighbour - x and row coefficients that carry nd = nu_h dts:
+//   kE = nd/dx^2, kN = nd (1/dy^2 - a), kS = nd (1/dy^2 + a), kY = nd/dy^2,
+//   a = tan(lat)/(2 R dy);
+// x + nd lap = x + kE (dE + dW) + kN dN + kS dS.  A closed face drops its
+// flux; the metric term keeps the raw north/south values, so a closed north
+// or south face is removed by -kY d.  With every gate open that correction is
+// fma(-kY, +0, inc) == inc bit for bit, so nu_open equals nu_gated with all
+// gates true, and every kernel of the sub-cycle (single substep, register
+// strips, streaming) rounds identically whatever the row's class.  No
+// multiply here feeds an add, so contraction cannot change the rounding.
 template <typename real>
-__device__ __forceinline__ real nu_update(real x, real L, real nu_h, real dts) {
-  return fma(nu_h * L, dts, x);
+struct NuRow {
+  real kE, kN, kS, kY;
+};
+
+template <typename real>
+__device__ __forceinline__ real nu_open(real x, real xe, real xw, real xn, real xs, const NuRow<real>& k) {
+  const real dE = xe - x, dW = xw - x, dN = xn - x, dS = xs - x;
+  return x + fma(k.kE, dE + dW, fma(k.kN, dN, k.kS * dS));
+}
+
+template <typename real>
+__device__ __forceinline__ real nu_gated(real x, real xe, real xw, real xn, real xs, bool ge, bool gw, bool gn,
+                                         bool gs, const NuRow<real>& k) {
+  const real dE = ge ? xe - x : real(0), dW = gw ? xw - x : real(0);
+  const real dN = xn - x, dS = xs - x;
+  const real inc = fma(k.kE, dE + dW, fma(k.kN, dN, k.kS * dS));
+  return x + fma(-k.kY, (gn ? real(0) : dN) + (gs ? real(0) : dS), inc);
 }
 
 // _laplacian_h legacy branch (numerics/horizontal.py:181-215).
@@ -166,21 +192,25 @@ template <typename real>
 __global__ __launch_bounds__(256) void k_nu_substep(Geo<real> g, const real* __restrict__ u,
                                                     const real* __restrict__ v, real* __restrict__ uo,
                                                     real* __restrict__ vo, real* __restrict__ lu,
-                                                    real* __restrict__ lv, real nu_h, real dts) {
+                                                    real* __restrict__ lv) {
   ZM_CELL_PROLOGUE
-  const real Lu = lap_h(u, plane, n, k, g);
-  const real Lv = lap_h(v, plane, n, k, g);
-  uo[c] = nu_update(u[c], Lu, nu_h, dts);
-  vo[c] = nu_update(v[c], Lv, nu_h, dts);
+  const real* __restrict__ row = g.nu_met + kNuMetStride * (n.j + kNuMetPad);
+  const NuRow<real> kr{row[0], row[1], row[2], row[3]};
+  const bool wc = k < n.nwC;
+  const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW, gn = wc && k < n.nwN, gs = wc && k < n.nwS;
+  uo[c] = nu_gated(u[c], u[plane + n.e2], u[plane + n.w2], u[plane + n.n2e], u[plane + n.s2e], ge, gw, gn, gs, kr);
+  vo[c] = nu_gated(v[c], v[plane + n.e2], v[plane + n.w2], v[plane + n.n2e], v[plane + n.s2e], ge, gw, gn, gs, kr);
   if (lu) {
-    lu[c] = Lu;
-    lv[c] = Lv;
+    lu[c] = lap_h(u, plane, n, k, g);
+    lv[c] = lap_h(v, plane, n, k, g);
   }
 }
 
 // Cross-lane moves (DPP wave_shl:1 / wave_shr:1 with bound_ctrl): lane l
 // receives lane l+1 (from_right) or lane l-1 (from_left); lanes shifted in
-// from outside the wave read 0 (only halo lanes, never stored).
+// from outside the wave read 0 (only halo lanes, never stored).  A source lane
+// that is inactive also reads 0, so these are called under wave-uniform
+// conditions only.
 __device__ __forceinline__ int dpp_from_right(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
 __device__ __forceinline__ int dpp_from_left(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
 __device__ __forceinline__ float dpp_from_right(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
@@ -220,9 +250,10 @@ __device__ __forceinline__ double readlane(double x, int l) {
 // adjacent lanes (DPP), north/south from the neighbouring registers.  `nsub`
 // (<= HALO) substeps run per launch; each substep invalidates one more cell
 // at every tile edge, so the (64 - 2 HALO) x (H - 2 HALO) centre is exact.
-// Every node receives exactly the arithmetic of lap_h / k_nu_substep; rows
-// whose 64 nodes have all four faces open skip the face selects (the
-// expression is the same one with every gate true).
+// Every node receives exactly the arithmetic of k_nu_substep (nu_gated); rows
+// whose 64 nodes have all four faces open use nu_open, which rounds like
+// nu_gated with every gate true.  lap_strip and lap_open are lap_h's
+// Laplacian for the same node classes (the stored lap(u), lap(v)).
 template <typename real>
 __device__ __forceinline__ real lap_strip(real x, real xe, real xw, real xn, real xs, bool fe_on, bool fw_on,
                                           bool fn_on, bool fs_on, real id2, real idy2, real hidy, real tr) {
@@ -240,18 +271,19 @@ __device__ __forceinline__ real lap_open(real x, real xe, real xw, real xn, real
 // nsub substeps of the strip; WALL = the tile holds a closed-wall row (its
 // own mirror neighbour), otherwise the north/south selects are compiled out.
 template <typename real, int H, bool WALL>
-__device__ __forceinline__ void nu_strip_substeps(real (&U)[H], real (&V)[H], real lane_idx2, real lane_tan,
+__device__ __forceinline__ void nu_strip_substeps(real (&U)[H], real (&V)[H], NuRow<real> lane_k,
                                                   unsigned int me, unsigned int mw, unsigned int mn, unsigned int ms,
                                                   unsigned int wallN, unsigned int wallS, unsigned int open_rows,
-                                                  real idy2, real hidy, real nu_h, real dts, int nsub) {
+                                                  int nsub) {
   for (int s = 0; s < nsub; ++s) {
-    // keep one copy of the row metrics instead of H hoisted scalars
-    asm volatile("" : "+v"(lane_idx2), "+v"(lane_tan), "+v"(me), "+v"(mw), "+v"(mn), "+v"(ms), "+s"(wallN),
-                 "+s"(wallS), "+s"(open_rows));
+    // keep one copy of the row coefficients instead of 4 H hoisted scalars
+    asm volatile("" : "+v"(lane_k.kE), "+v"(lane_k.kN), "+v"(lane_k.kS), "+v"(lane_k.kY), "+v"(me), "+v"(mw),
+                 "+v"(mn), "+v"(ms), "+s"(wallN), "+s"(wallS), "+s"(open_rows));
     real pu = U[0], pv = V[0];  // old values of row r-1
 #pragma unroll
     for (int r = 0; r < H; ++r) {
-      const real id2 = readlane(lane_idx2, r), tr = readlane(lane_tan, r);
+      const NuRow<real> kr{readlane(lane_k.kE, r), readlane(lane_k.kN, r), readlane(lane_k.kS, r),
+                           readlane(lane_k.kY, r)};
       const bool atN = WALL && ((wallN >> r) & 1u), atS = WALL && ((wallS >> r) & 1u);
       const int rn = r + 1 < H ? r + 1 : r;
       // ordering barrier: keep each row's neighbour moves next to its update
@@ -261,20 +293,17 @@ __device__ __forceinline__ void nu_strip_substeps(real (&U)[H], real (&V)[H], re
       const real yn = atN ? y : V[rn], ys = atS ? y : pv;
       const real xe = dpp_from_right(x), xw = dpp_from_left(x);
       const real ye = dpp_from_right(y), yw = dpp_from_left(y);
-      real Lx, Ly;
+      pu = x;
+      pv = y;
       if ((open_rows >> r) & 1u) {
-        Lx = lap_open(x, xe, xw, xn, xs, id2, idy2, hidy, tr);
-        Ly = lap_open(y, ye, yw, yn, ys, id2, idy2, hidy, tr);
+        U[r] = nu_open(x, xe, xw, xn, xs, kr);
+        V[r] = nu_open(y, ye, yw, yn, ys, kr);
       } else {
         const bool fe_on = (me >> r) & 1u, fw_on = (mw >> r) & 1u;
         const bool fn_on = (mn >> r) & 1u, fs_on = (ms >> r) & 1u;
-        Lx = lap_strip(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
-        Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
+        U[r] = nu_gated(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, kr);
+        V[r] = nu_gated(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, kr);
       }
-      pu = x;
-      pv = y;
-      U[r] = nu_update(x, Lx, nu_h, dts);
-      V[r] = nu_update(y, Ly, nu_h, dts);
     }
   }
 }
@@ -282,7 +311,7 @@ __device__ __forceinline__ void nu_strip_substeps(real (&U)[H], real (&V)[H], re
 template <typename real, int H, int HALO>
 __global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __restrict__ u,
                                                   const real* __restrict__ v, real* __restrict__ uo,
-                                                  real* __restrict__ vo, real nu_h, real dts, int nsub) {
+                                                  real* __restrict__ vo, int nsub) {
   constexpr int OX = 64 - 2 * HALO, OY = H - 2 * HALO;
   static_assert(H <= 32, "wet bits are kept in one 32-bit word");
   const int lane = threadIdx.x & 63;
@@ -334,17 +363,15 @@ __global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __res
   unsigned int open_rows = __builtin_amdgcn_readfirstlane(~notfull) & inside;
   wallN = __builtin_amdgcn_readfirstlane(wallN);
   wallS = __builtin_amdgcn_readfirstlane(wallS);
-  const real idy2 = g.idy2, hidy = g.hidy;
-  // lane r holds the metrics of strip row r (read back with readlane)
-  real lane_idx2, lane_tan;
+  // lane r holds the coefficients of strip row r (read back with readlane)
+  NuRow<real> lane_k;
   {
     int j = j0 + (lane < H ? lane : H - 1);
     j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
-    lane_idx2 = g.idx2[j];
-    lane_tan = g.tanR[j];
+    const real* __restrict__ row = g.nu_met + kNuMetStride * (j + kNuMetPad);
+    lane_k = NuRow<real>{row[0], row[1], row[2], row[3]};
   }
-  nu_strip_substeps<real, H, true>(U, V, lane_idx2, lane_tan, me, mw, mn, ms, wallN, wallS, open_rows, idy2, hidy,
-                                   nu_h, dts, nsub);
+  nu_strip_substeps<real, H, true>(U, V, lane_k, me, mw, mn, ms, wallN, wallS, open_rows, nsub);
   const int io = tx * OX + (lane - HALO);
   real* __restrict__ uop = uo + plane;
   real* __restrict__ vop = vo + plane;
@@ -372,26 +399,42 @@ __global__ __launch_bounds__(256) void k_nu_strip(Geo<real> g, const real* __res
 // at the chunk ends (NS - l extra rows per side at level l) and input rows
 // are prefetched kNuAhead rows ahead.  East/west neighbours come from the
 // adjacent lanes (DPP), so as in k_nu_strip lanes [NS, 64 - NS) are exact.
-// Every node receives the arithmetic of lap_h / k_nu_substep (lap_open where
-// the row's interior lanes have all four faces open, lap_strip otherwise).
+// Every node receives the arithmetic of k_nu_substep (nu_open where the row's
+// interior lanes have all four faces open, nu_gated otherwise).
 //
 // Per-wave data prepared at set-up (never written by a step):
 //  * nu_wet[(k * nch + ch) * nx + i]: bit b = node (i, jb - kNuWetBase + b, k)
 //    is wet (0 outside the domain), jb = ch * chunk; the four face gates of
 //    every row follow from it once per wave, and the rows whose interior
 //    lanes are all open from one wave-wide AND;
-//  * nu_met[2 (j + kNuMetPad) + {0, 1}] = 1/dx^2, tan(lat)/R of row j, padded
-//    so that every march step reads its rows' metrics unconditionally through
-//    scalar loads.
+//  * nu_met[kNuMetStride (j + kNuMetPad) + q] = row j's substep coefficients
+//    kE, kN, kS, kY (q = 0..3, see nu_open) and 1/dx^2, tan(lat)/R (q = 4, 5),
+//    padded so that every march step reads its rows' coefficients
+//    unconditionally through scalar loads.
 constexpr int kNuAhead = 6;  // a multiple of the period of the three-row window
 
 template <typename real>
 using nu_met_ptr = const __attribute__((address_space(4))) real*;  // read through the scalar cache
 
+// One level's window: rows (south, centre, north) of its next computation.
+// Named scalars per level (no per-field arrays over the levels, which the
+// compiler keeps as register tuples and copies whole at every branch merge).
+template <typename real>
+struct NuWin {
+  real su, cu, nu, sv, cv, nv;
+  __device__ __forceinline__ void push(real u, real v) {
+    su = cu;
+    cu = nu;
+    nu = u;
+    sv = cv;
+    cv = nv;
+    nv = v;
+  }
+};
+
 template <typename real, int NS>
 struct NuStream {
-  real W0u[NS], W1u[NS], W2u[NS];  // level l: rows (south, centre, north) of its next computation
-  real W0v[NS], W1v[NS], W2v[NS];
+  NuWin<real> w[NS];
   real qu[kNuAhead], qv[kNuAhead];  // prefetched input rows
 };
 
@@ -428,13 +471,36 @@ struct NuBufs {
 
 // One level-row of the streaming kernel: row r of level l + 1 from the window
 // (xs, x, xn) of level l; b = r's bit in the gate words.  cls (uniform): 1 =
-// every interior lane has all four faces open, 2 = every face is closed
-// (only the metric term remains: lap_strip with zero face differences), 0 =
-// mixed.  Lx, Ly receive the Laplacians.
+// every interior lane has all four faces open (nu_open), 2 = every face is
+// closed (nu_gated with every gate false, which needs no east/west values),
+// 0 = mixed.
 template <typename real>
 __device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y, real yn, real ys, int cls,
-                                               const NuGates& G, int b, real id2, real tr, real idy2, real hidy,
-                                               real nu_h, real dts, real& ou, real& ov, real& Lx, real& Ly) {
+                                               const NuGates& G, int b, const NuRow<real>& kr, real& ou, real& ov) {
+  if (cls == 2) {
+    ou = nu_gated(x, x, x, xn, xs, false, false, false, false, kr);
+    ov = nu_gated(y, y, y, yn, ys, false, false, false, false, kr);
+  } else {
+    const real xe = dpp_from_right(x), xw = dpp_from_left(x);
+    const real ye = dpp_from_right(y), yw = dpp_from_left(y);
+    if (cls == 1) {
+      ou = nu_open(x, xe, xw, xn, xs, kr);
+      ov = nu_open(y, ye, yw, yn, ys, kr);
+    } else {
+      const bool fe_on = (G.e >> b) & 1ull, fw_on = (G.w >> b) & 1ull;
+      const bool fn_on = (G.n >> b) & 1ull, fs_on = (G.s >> b) & 1ull;
+      ou = nu_gated(x, xe, xw, xn, xs, fe_on, fw_on, fn_on, fs_on, kr);
+      ov = nu_gated(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, kr);
+    }
+  }
+}
+
+// The Laplacians of the same node (lap_h's arithmetic per class: lap_open,
+// lap_strip, or lap_strip with zero face differences).
+template <typename real>
+__device__ __forceinline__ void lap_stream_node(real x, real xn, real xs, real y, real yn, real ys, int cls,
+                                                const NuGates& G, int b, real id2, real tr, real idy2, real hidy,
+                                                real& Lx, real& Ly) {
   if (cls == 2) {
     Lx = lap_sum(real(0), real(0), xn - xs, id2, idy2, hidy, tr);
     Ly = lap_sum(real(0), real(0), yn - ys, id2, idy2, hidy, tr);
@@ -451,8 +517,6 @@ __device__ __forceinline__ void nu_stream_node(real x, real xn, real xs, real y,
       Ly = lap_strip(y, ye, yw, yn, ys, fe_on, fw_on, fn_on, fs_on, id2, idy2, hidy, tr);
     }
   }
-  ou = nu_update(x, Lx, nu_h, dts);
-  ov = nu_update(y, Ly, nu_h, dts);
 }
 
 // Uniform per-wave bounds of the streaming kernel: level l computes its row
@@ -463,77 +527,73 @@ struct NuRange {
   int c[NS], n[NS];
 };
 
-// One march step t.  LU: level 0 also stores the Laplacians of the input
-// rows [jb, je) (the first substep's lap(u), lap(v)).
+// One march step t = t0 + SLOT.  mt = the coefficient row of row t0 - 2 NS + 1
+// (every row of the step then sits at a constant offset from it).  LU: level
+// 0 also stores the Laplacians of the input rows [jb, je) (the first
+// substep's lap(u), lap(v)).
 template <typename real, int NS, int SLOT, bool LU>
-__device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> met,
+__device__ __forceinline__ void nu_stream_step(NuStream<real, NS>& st, const Geo<real>& g, nu_met_ptr<real> mt,
                                                const NuGates& G, unsigned long long openw, unsigned long long dryw,
                                                int t, const NuRange<NS>& R,
-                                               const NuBufs& B, unsigned ib, unsigned iob, bool store_lane,
-                                               real nu_h, real dts) {
+                                               const NuBufs& B, unsigned ib, unsigned iob, bool store_lane) {
   // row t enters the level-0 window
-  st.W0u[0] = st.W1u[0];
-  st.W1u[0] = st.W2u[0];
-  st.W2u[0] = st.qu[SLOT];
-  st.W0v[0] = st.W1v[0];
-  st.W1v[0] = st.W2v[0];
-  st.W2v[0] = st.qv[SLOT];
+  st.w[0].push(st.qu[SLOT], st.qv[SLOT]);
   // prefetch row t + kNuAhead into this slot (uniform row bases)
   if (t + kNuAhead < R.qe) {
     const int ro = (t + kNuAhead) * g.nx * (int)sizeof(real);
     st.qu[SLOT] = buf_ld((real*)nullptr, B.u, ib, ro);
     st.qv[SLOT] = buf_ld((real*)nullptr, B.v, ib, ro);
   }
-  const real idy2 = g.idy2, hidy = g.hidy;
-  real Ou[NS], Ov[NS];
+  // levels from the top: level l's new row enters the window of level l + 1,
+  // which has already computed its row of this step
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
+  for (int l = NS - 1; l >= 0; --l) {
     const int r = t - 2 * l - 1;  // row of level l + 1 computed now
-    const real id2 = met[2 * (r + kNuMetPad)], tr = met[2 * (r + kNuMetPad) + 1];
+    const nu_met_ptr<real> row = mt + kNuMetStride * (SLOT + 2 * NS - 2 - 2 * l);  // row r
+    NuWin<real>& w = st.w[l];
     // a row outside the level's range is never read by a computed row of
     // the level above: it keeps the centre value
-    Ou[l] = st.W1u[l];
-    Ov[l] = st.W1v[l];
+    real ou = w.cu, ov = w.cv;
     if ((unsigned)(t - R.c[l]) < (unsigned)R.n[l]) {  // uniform
       // the wall row is its own mirror neighbour; the slot written is not
-      // read by another computed row (W0 leaves the window, W2 holds row ny)
+      // read by another computed row (the south slot leaves the window, the
+      // north one holds row ny).  The empty asm keeps these rare (wall-chunk)
+      // branches as branches instead of two selects per level and step.
       if (__builtin_expect(r == 0, 0)) {
-        st.W0u[l] = st.W1u[l];
-        st.W0v[l] = st.W1v[l];
+        asm volatile("");
+        w.su = w.cu;
+        w.sv = w.cv;
       }
       if (__builtin_expect(r == g.ny - 1, 0)) {
-        st.W2u[l] = st.W1u[l];
-        st.W2v[l] = st.W1v[l];
+        asm volatile("");
+        w.nu = w.cu;
+        w.nv = w.cv;
       }
       const int b = r - R.wb;
       const int cls = ((openw >> b) & 1ull) ? 1 : (((dryw >> b) & 1ull) ? 2 : 0);
-      real Lx, Ly;
-      nu_stream_node(st.W1u[l], st.W2u[l], st.W0u[l], st.W1v[l], st.W2v[l], st.W0v[l], cls, G, b, id2, tr, idy2,
-                     hidy, nu_h, dts, Ou[l], Ov[l], Lx, Ly);
+      const NuRow<real> kr{row[0], row[1], row[2], row[3]};
+      nu_stream_node(w.cu, w.nu, w.su, w.cv, w.nv, w.sv, cls, G, b, kr, ou, ov);
       if constexpr (LU) {
-        if (l == 0 && r >= R.jb && r < R.je && store_lane) {
-          const int ro = r * g.nx * (int)sizeof(real);
-          buf_st(Lx, B.lu, iob, ro);
-          buf_st(Ly, B.lv, iob, ro);
+        // uniform condition: the Laplacian reads the neighbouring lanes (DPP),
+        // which must all be active; only the store is per lane
+        if (l == 0 && r >= R.jb && r < R.je) {
+          real Lx, Ly;
+          lap_stream_node(w.cu, w.nu, w.su, w.cv, w.nv, w.sv, cls, G, b, row[4], row[5], g.idy2, g.hidy, Lx, Ly);
+          if (store_lane) {
+            const int ro = r * g.nx * (int)sizeof(real);
+            buf_st(Lx, B.lu, iob, ro);
+            buf_st(Ly, B.lv, iob, ro);
+          }
         }
       }
     }
-  }
-  // the new rows enter the windows of the level above
-#pragma unroll
-  for (int l = 1; l < NS; ++l) {
-    st.W0u[l] = st.W1u[l];
-    st.W1u[l] = st.W2u[l];
-    st.W2u[l] = Ou[l - 1];
-    st.W0v[l] = st.W1v[l];
-    st.W1v[l] = st.W2v[l];
-    st.W2v[l] = Ov[l - 1];
-  }
-  const int rs = t - 2 * NS + 1;  // level NS row
-  if (rs >= R.jb && rs < R.je && store_lane) {
-    const int ro = rs * g.nx * (int)sizeof(real);
-    buf_st(Ou[NS - 1], B.uo, iob, ro);
-    buf_st(Ov[NS - 1], B.vo, iob, ro);
+    if (l + 1 < NS) {
+      st.w[l + 1].push(ou, ov);
+    } else if (r >= R.jb && r < R.je && store_lane) {  // level NS: the output rows
+      const int ro = r * g.nx * (int)sizeof(real);
+      buf_st(ou, B.uo, iob, ro);
+      buf_st(ov, B.vo, iob, ro);
+    }
   }
 }
 
@@ -541,7 +601,7 @@ template <typename real, int NS, bool LU>
 __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __restrict__ u,
                                                    const real* __restrict__ v, real* __restrict__ uo,
                                                    real* __restrict__ vo, real* __restrict__ lu,
-                                                   real* __restrict__ lv, real nu_h, real dts, int chunk) {
+                                                   real* __restrict__ lv, int chunk) {
   constexpr int OX = 64 - 2 * NS;
   static_assert(NS <= kNuWetBase && 2 * NS - 1 < kNuMetPad, "wet word and metric padding cover NS substeps");
   const int lane = threadIdx.x & 63;
@@ -577,7 +637,7 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const unsigned ib = (unsigned)i * (unsigned)sizeof(real), iob = store_lane ? (unsigned)io * (unsigned)sizeof(real) : 0u;
   NuStream<real, NS> st;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) st.W0u[l] = st.W1u[l] = st.W2u[l] = st.W0v[l] = st.W1v[l] = st.W2v[l] = real(0);
+  for (int l = 0; l < NS; ++l) st.w[l] = NuWin<real>{real(0), real(0), real(0), real(0), real(0), real(0)};
 #pragma unroll
   for (int q = 0; q < kNuAhead; ++q) {
     st.qu[q] = st.qv[q] = real(0);
@@ -628,8 +688,9 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const int tend = qe + 2 * NS - 1;
   static_assert(kNuAhead == 6, "the step below is unrolled over the prefetch ring");
 #define ZM_NU_STEP(S) \
-  nu_stream_step<real, NS, S, LU>(st, g, met, G, openw, dryw, t0 + S, R, B, ib, iob, store_lane, nu_h, dts)
+  nu_stream_step<real, NS, S, LU>(st, g, mt, G, openw, dryw, t0 + S, R, B, ib, iob, store_lane)
   for (int t0 = qs; t0 < tend; t0 += kNuAhead) {
+    const nu_met_ptr<real> mt = met + kNuMetStride * (t0 - 2 * NS + 1 + kNuMetPad);
     ZM_NU_STEP(0);
     if (t0 + 1 >= tend) break;
     ZM_NU_STEP(1);
@@ -2686,11 +2747,22 @@ static inline unsigned tiles3(const Geo<real>& g) {
   return (unsigned)((long long)((g.nx + kTileI - 1) / kTileI) * ((g.ny + kTileJ - 1) / kTileJ) * g.nz);
 }
 
+// The row coefficients in g.nu_met carry nu_h dts: a launch for another
+// pair would silently use the wrong diffusion.
+template <typename real>
+static void check_nu_coefficients(const Geo<real>& g, real nu_h, real dts) {
+  if (nu_h != g.nu_h_met || dts != g.nu_dts_met)
+    throw std::runtime_error("nu_h sub-cycle: the row coefficients were built for nu_h " +
+                             std::to_string((double)g.nu_h_met) + ", substep " + std::to_string((double)g.nu_dts_met) +
+                             " s, not nu_h " + std::to_string((double)nu_h) + ", substep " +
+                             std::to_string((double)dts) + " s");
+}
+
 template <typename real>
 void launch_nu_substep(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real* lu,
                        real* lv, real nu_h, real dts, hipStream_t s) {
-  hipLaunchKernelGGL(k_nu_substep<real>, dim3(tiles3(g)), dim3(256), 0, s, g, u, v, uo, vo, lu,
-                     lv, nu_h, dts);
+  check_nu_coefficients(g, nu_h, dts);
+  hipLaunchKernelGGL(k_nu_substep<real>, dim3(tiles3(g)), dim3(256), 0, s, g, u, v, uo, vo, lu, lv);
 }
 
 template <typename real>
@@ -2699,12 +2771,12 @@ int nu_block_max() {
 }
 
 template <typename real, int H, int HALO>
-static void launch_nu_strip(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h,
-                            real dts, int nsub, hipStream_t s) {
+static void launch_nu_strip(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, int nsub,
+                            hipStream_t s) {
   constexpr int OX = 64 - 2 * HALO, OY = H - 2 * HALO;
   const long long tiles = (long long)((g.nx + OX - 1) / OX) * ((g.ny + OY - 1) / OY) * g.nz;
   hipLaunchKernelGGL((k_nu_strip<real, H, HALO>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, g, u, v, uo,
-                     vo, nu_h, dts, nsub);
+                     vo, nsub);
 }
 
 // nsub substeps per launch on the narrowest halo that covers them: a 3-wide
@@ -2717,13 +2789,14 @@ bool nu_block_writes_lap(const Geo<real>& g, int nsub) {
 template <typename real>
 void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo, real* vo, real nu_h, real dts,
                      int nsub, hipStream_t s, real* lu, real* lv) {
+  check_nu_coefficients(g, nu_h, dts);
   if (nsub >= 2 && nsub <= 4 && g.nu_chunk > 0) {
     const int chunk = g.nu_chunk;
     auto go = [&](auto kern, int ns) {
       const int OX = 64 - 2 * ns;
       const long long waves = (long long)((g.nx + OX - 1) / OX) * ((g.ny + chunk - 1) / chunk) * g.nz;
       const unsigned blocks = (unsigned)((waves + 3) / 4);
-      hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, g, u, v, uo, vo, lu, lv, nu_h, dts, chunk);
+      hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, g, u, v, uo, vo, lu, lv, chunk);
     };
     if (lu) {
       if (nsub == 2) go(k_nu_stream<real, 2, true>, 2);
@@ -2739,9 +2812,9 @@ void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo,
   if (lu) throw std::runtime_error("launch_nu_block: lap output needs the streaming kernel (2-4 substeps)");
   using T = NuTile<real>;
   if (nsub <= 3)
-    launch_nu_strip<real, T::H, 3>(g, u, v, uo, vo, nu_h, dts, nsub, s);
+    launch_nu_strip<real, T::H, 3>(g, u, v, uo, vo, nsub, s);
   else
-    launch_nu_strip<real, T::H, T::HALO>(g, u, v, uo, vo, nu_h, dts, nsub, s);
+    launch_nu_strip<real, T::H, T::HALO>(g, u, v, uo, vo, nsub, s);
 }
 
 template <typename real>
