@@ -97,7 +97,7 @@ struct Solver : SolverBase {
   real dt_bt = 0, dtH = 0, drag = 0, decay_bot = 0, dts = 0;
   int nu_block = 1;
   std::vector<void*> allocs;
-  real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fden = nullptr;
+  real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fiden = nullptr;
   double *dzn64 = nullptr, *dzi64 = nullptr;
   double* vmix_scratch = nullptr;  // Thomas c', d' (3 n3), allocated on first use
   double* area64 = nullptr;        // cell area per latitude row (dx dy), float64
@@ -203,7 +203,7 @@ struct Solver : SolverBase {
     const int nb = ncap > 0 ? ncap + c.polar_cap_taper : 0;
     // geometry rows
     std::vector<double> cosl(ny), idx(ny), idx2(ny), tanR(ny), frow(ny), cfn(ny), cfs(ny), idyc(ny);
-    std::vector<double> rc(ny), rs(ny), fd(ny), fden_h(ny);
+    std::vector<double> rc(ny), rs(ny), fd(ny), fiden_h(ny);
     const double idy = 1.0 / c.dy;
     for (int j = 0; j < ny; ++j) {
       cosl[j] = c.cos_lat[j];
@@ -218,7 +218,7 @@ struct Solver : SolverBase {
       rc[j] = std::cos(c.f_row[j] * (c.dt / 2.0));
       rs[j] = std::sin(c.f_row[j] * (c.dt / 2.0));
       fd[j] = c.f_row[j] * c.dt_bt;
-      fden_h[j] = 1.0 + fd[j] * fd[j];
+      fiden_h[j] = 1.0 / (1.0 + fd[j] * fd[j]);
     }
     g.nx = nx;
     g.ny = ny;
@@ -272,7 +272,7 @@ struct Solver : SolverBase {
     // enough of them to fill the GPU (best for every column kernel at 0.5
     // degrees), else 10 (zm_autotune refines either)
     g.seg = n2 >= 65536 ? nz : std::min(10, nz);
-    g.nu_chunk = 20;  // rows per wave chunk of the streaming nu_h kernel (measured best at 0.5 degrees)
+    g.nu_chunk = 28;  // rows per wave chunk of the streaming nu_h kernel (measured best at 0.5 degrees)
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
     g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
     {
@@ -371,7 +371,7 @@ struct Solver : SolverBase {
     P.rot_c = upload(rc);
     P.rot_s = upload(rs);
     fdt = upload(fd);
-    fden = upload(fden_h);
+    fiden = upload(fiden_h);
 
     dt_bt = (real)c.dt_bt;
     dtH = (real)(c.dt_bt * c.H_sw);
@@ -697,7 +697,7 @@ struct Solver : SolverBase {
     zm::launch_pbc(g, Pf, T3, S3, pbc, s);
     zm::launch_bt_rho(g, Pf, pbc, Frx, Fry, u3, v3, ubt0, vbt0, s);
     if (bt_persist) {
-      zm::launch_bt_persist(g, Pf, eta_in, ubt0, vbt0, etaw, ubt, vbt, etaw2, ubt2, vbt2, Frx, Fry, fdt, fden, dtH,
+      zm::launch_bt_persist(g, Pf, eta_in, ubt0, vbt0, etaw, ubt, vbt, etaw2, ubt2, vbt2, Frx, Fry, fdt, fiden, dtH,
                             dt_bt, drag, n_sub, bt_progress, bt_timeout, s);
       return;
     }
@@ -707,7 +707,7 @@ struct Solver : SolverBase {
       real* eo = last_half ? etaw : etaw2;
       real* uo = last_half ? ubt : ubt2;
       real* vo = last_half ? vbt : vbt2;
-      zm::launch_bt_sub(g, Pf, ei, ui, vi, eo, uo, vo, Frx, Fry, fdt, fden, dtH, dt_bt, drag, s);
+      zm::launch_bt_sub(g, Pf, ei, ui, vi, eo, uo, vo, Frx, Fry, fdt, fiden, dtH, dt_bt, drag, s);
       ei = eo;
       ui = uo;
       vi = vo;
@@ -725,7 +725,7 @@ struct Solver : SolverBase {
     m.Frx = Frx;
     m.Fry = Fry;
     m.fdt = fdt;
-    m.fden = fden;
+    m.fiden = fiden;
     m.dtH = dtH;
     m.dt_bt = dt_bt;
     m.drag = drag;
@@ -1046,13 +1046,13 @@ struct Solver : SolverBase {
           zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, u1, v1, T1, S1, tmpA, tmpB, u2, v2, s);
           break;
         case 4:
-          zm::launch_bt_sub(g, P, etaw, ubt, vbt, etaw2, ubt2, vbt2, Frx, Fry, fdt, fden, dtH, dt_bt, drag, s);
+          zm::launch_bt_sub(g, P, etaw, ubt, vbt, etaw2, ubt2, vbt2, Frx, Fry, fdt, fiden, dtH, dt_bt, drag, s);
           break;
         case 15:  // the whole persistent sub-cycle (n_sub substeps)
           if (!bt_persist_fits) throw ConfigError{"the persistent sub-cycle does not fit this device"};
           zm::launch_bt_persist(g, P, (r % 2) ? etaw2 : etaw, (r % 2) ? ubt2 : ubt, (r % 2) ? vbt2 : vbt,
                                 (r % 2) ? etaw : etaw2, (r % 2) ? ubt : ubt2, (r % 2) ? vbt : vbt2, (real*)tmpA,
-                                (real*)tmpA + n2, (real*)tmpB, Frx, Fry, fdt, fden, dtH, dt_bt, drag, n_sub,
+                                (real*)tmpA + n2, (real*)tmpB, Frx, Fry, fdt, fiden, dtH, dt_bt, drag, n_sub,
                                 bt_progress, bt_timeout, s);
           break;
         case 14:

@@ -101,7 +101,7 @@ void launch_bt_eta(const Geo<real>& g, const Phys<real>& P, real* eta, const rea
                    real dtH, hipStream_t s);
 template <typename real>
 void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, real* ubt, real* vbt,
-                   const real* Frx, const real* Fry, const real* fdt, const real* fden, real dt_bt, real drag,
+                   const real* Frx, const real* Fry, const real* fdt, const real* fiden, real dt_bt, real drag,
                    hipStream_t s);
 // A member of a batched barotropic substep: one grid's constants and state.
 // row0 = first workgroup of this member in the batched launch.
@@ -115,7 +115,7 @@ struct BtMember {
   const real* Frx;
   const real* Fry;
   const real* fdt;
-  const real* fden;
+  const real* fiden;
   real dtH, dt_bt, drag;
   int row0;
 };
@@ -130,13 +130,13 @@ struct BtMember {
 template <typename real>
 void launch_bt_persist(const Geo<real>& g, const Phys<real>& P, const real* eta0, const real* u0, const real* v0,
                        real* ea, real* ua, real* va, real* eb, real* ub, real* vb, const real* Frx, const real* Fry,
-                       const real* fdt, const real* fden, real dtH, real dt_bt, real drag, int nsub, int* progress,
+                       const real* fdt, const real* fiden, real dtH, real dt_bt, real drag, int nsub, int* progress,
                        int* timeouts, hipStream_t s);
 
 template <typename real>
 void launch_bt_sub(const Geo<real>& g, const Phys<real>& P, const real* eta_in, const real* u_in,
                    const real* v_in, real* eta_out, real* u_out, real* v_out, const real* Frx, const real* Fry,
-                   const real* fdt, const real* fden, real dtH, real dt_bt, real drag, hipStream_t s);
+                   const real* fdt, const real* fiden, real dtH, real dt_bt, real drag, hipStream_t s);
 
 // One barotropic substep (eta then velocity kernel) for every member of a
 // device-resident table.

@@ -2276,22 +2276,23 @@ __device__ __forceinline__ void bt_store(real* p, real v) {
 
 // eta' = eta - dt H div(ubt) on row j with the polar-cap blend of band rows;
 // reads eta_row[i] (row j of eta), writes out[i] (and out2[i] unless null).
-// out may alias eta_row.  t: thread within the 256-thread row group; ALLRED:
-// reduce on every row so that the block-wide barriers match across groups
-// (the zonal mean is only used on band rows).
-template <typename real, bool ALLRED = false, bool COH = false>
+// out may alias eta_row.  t: thread within the 256-thread row group; ALLRED
+// (or allred, block-uniform at run time): reduce on every row so that the
+// block-wide barriers match across groups (the zonal mean is only used on
+// band rows).
+template <typename real, bool ALLRED = false, bool COH = false, int NQ = kRowChunk>
 __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
                                            const real* __restrict__ eta_row, const real* __restrict__ ubt,
                                            const real* __restrict__ vbt, real dtH, real* out, real* red,
-                                           real* out2 = nullptr) {
+                                           real* out2 = nullptr, bool allred = false) {
   const real* wet = P.wet2;
-  real en[kRowChunk];
+  real en[NQ];
   real ssum = 0, wsum = 0;
   // row metrics in registers: the selects below then only mix loaded values
   const bool hasN = j + 1 < g.ny, hasS = j > 0;
   const real idx = g.idx[j], idyc = g.idyc[j], cfN = g.cfn[j], cfS = g.cfn[hasS ? j - 1 : 0];
 #pragma unroll
-  for (int q = 0; q < kRowChunk; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     // lanes past the row end repeat its last column and book nothing
     const bool act = t + q * 256 < g.nx;
     const int i = act ? t + q * 256 : g.nx - 1;
@@ -2316,13 +2317,13 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
   }
   const real wb = cap_weight(P, j, g.ny);
   real zm = 0;
-  if (ALLRED || wb >= real(0)) {
+  if (ALLRED || allred || wb >= real(0)) {
     real sw2[2] = {ssum, wsum};
     group_sums<real, 2>(sw2, red, t);
     zm = sw2[0] / (sw2[1] > real(1) ? sw2[1] : real(1));
   }
 #pragma unroll
-  for (int q = 0; q < kRowChunk; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int i = t + q * 256;
     if (i >= g.nx) continue;
     const int c2 = j * g.nx + i;
@@ -2333,19 +2334,54 @@ __device__ __forceinline__ void bt_eta_row(const Geo<real>& g, const Phys<real>&
   }
 }
 
-// ubt, vbt update of row j from the new eta (rows j-1, j, j+1 given as row
-// pointers; en / es are only read when that row exists): gradient, forcing,
-// legacy implicit Coriolis, drag, mask, polar-cap blend.  Outputs may alias
-// the inputs.
-template <typename real, bool ALLRED = false, bool COH = false>
-__device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
-                                           const real* ec_row, const real* en_row, const real* es_row,
-                                           const real* __restrict__ ubt, const real* __restrict__ vbt, real* uo,
-                                           real* vo, const real* __restrict__ Frx, const real* __restrict__ Fry,
-                                           real fd, real den, real dt_bt, real drag, real* red,
-                                           bool store = true) {
+// The inputs of a row's ubt, vbt update that do not depend on the new eta
+// (masks, old velocities, forcing), per column chunk; a kernel may load them
+// before its eta phase so that their latency overlaps it.
+template <typename real, int NQ>
+struct BtVelIn {
+  real wC[NQ], wE[NQ], wW[NQ], wN[NQ], wS[NQ], u0[NQ], v0[NQ], fx[NQ], fy[NQ], tx[NQ], ty[NQ];
+};
+
+template <typename real, bool COH = false, int NQ = kRowChunk>
+__device__ __forceinline__ void bt_vel_load(const Geo<real>& g, const Phys<real>& P, int j, int t,
+                                            const real* __restrict__ ubt, const real* __restrict__ vbt,
+                                            const real* __restrict__ Frx, const real* __restrict__ Fry,
+                                            BtVelIn<real, NQ>& in) {
   const real* wet = P.wet2;
-  real un[kRowChunk], vn[kRowChunk];
+  const bool hasN = j + 1 < g.ny, hasS = j > 0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    // lanes past the row end repeat its last column (their results are not kept)
+    const bool act = t + q * 256 < g.nx;
+    const int i = act ? t + q * 256 : g.nx - 1;
+    const int c2 = j * g.nx + i;
+    // every index is in bounds (rows j +- 1 clamped to row j at the walls)
+    in.wC[q] = wet[c2];
+    in.wE[q] = wet[j * g.nx + wrap_e(i, g.nx)];
+    in.wW[q] = wet[j * g.nx + wrap_w(i, g.nx)];
+    in.wN[q] = wet[hasN ? c2 + g.nx : c2];
+    in.wS[q] = wet[hasS ? c2 - g.nx : c2];
+    in.u0[q] = bt_load<COH>(ubt + c2);
+    in.v0[q] = bt_load<COH>(vbt + c2);
+    in.fx[q] = Frx[c2];
+    in.fy[q] = Fry[c2];
+    in.tx[q] = P.tau_x[c2];
+    in.ty[q] = P.tau_y[c2];
+  }
+}
+
+// ubt, vbt update of row j from the new eta (rows j-1, j, j+1 given as row
+// pointers; en / es are only read when that row exists) and the row's loaded
+// inputs: gradient, forcing, legacy implicit Coriolis (iden = 1 / (1 + (f
+// dt_bt)^2) of the row), drag, mask, polar-cap blend.  Outputs may alias the
+// inputs the row was loaded from.
+template <typename real, bool ALLRED = false, bool COH = false, int NQ = kRowChunk>
+__device__ __forceinline__ void bt_vel_finish(const Geo<real>& g, const Phys<real>& P, int j, int t,
+                                              const BtVelIn<real, NQ>& in, const real* ec_row,
+                                              const real* en_row, const real* es_row, real* uo, real* vo, real fd,
+                                              real iden, real dt_bt, real drag, real* red, bool store = true,
+                                              bool allred = false) {
+  real un[NQ], vn[NQ];
   real su = 0, sv = 0, sw = 0;
   // row data in registers: the selects below then only mix loaded values
   const bool hasN = j + 1 < g.ny, hasS = j > 0;
@@ -2353,18 +2389,14 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
   const real* en_safe = hasN ? en_row : ec_row;  // rows past the walls are never read
   const real* es_safe = hasS ? es_row : ec_row;
 #pragma unroll
-  for (int q = 0; q < kRowChunk; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     // lanes past the row end repeat its last column and book nothing
     const bool act = t + q * 256 < g.nx;
     const int i = act ? t + q * 256 : g.nx - 1;
-    const int c2 = j * g.nx + i, e2 = j * g.nx + wrap_e(i, g.nx), w2 = j * g.nx + wrap_w(i, g.nx);
-    // every index is in bounds (rows j +- 1 clamped to row j at the walls):
-    // load unconditionally, then select
-    const real wC = wet[c2], wE = wet[e2], wW = wet[w2];
-    const real wN = wet[hasN ? c2 + g.nx : c2], wS = wet[hasS ? c2 - g.nx : c2];
+    const real wC = in.wC[q], wE = in.wE[q], wW = in.wW[q], wN = in.wN[q], wS = in.wS[q];
     const real ec = ec_row[i], eE = ec_row[wrap_e(i, g.nx)], eW = ec_row[wrap_w(i, g.nx)];
     const real eN = en_safe[i], eS = es_safe[i];
-    const real u0 = bt_load<COH>(ubt + c2), v0 = bt_load<COH>(vbt + c2);
+    const real u0 = in.u0[q], v0 = in.v0[q];
     const bool wc = wC != real(0);
     const real dxp = (wc && wE != real(0)) ? eE - ec : real(0);
     const real dxm = (wc && wW != real(0)) ? ec - eW : real(0);
@@ -2372,12 +2404,12 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
     const real dyp = (hasN && wc && wN != real(0)) ? (eN - ec) * cfN : real(0);
     const real dym = (hasS && wc && wS != real(0)) ? (ec - eS) * cfS : real(0);
     const real gy = idyc * real(0.5) * (dyp + dym);
-    const real Fx = Frx[c2] + P.tau_x[c2] * P.inv_rho0H;
-    const real Fy = Fry[c2] + P.tau_y[c2] * P.inv_rho0H;
+    const real Fx = in.fx[q] + in.tx[q] * P.inv_rho0H;
+    const real Fy = in.fy[q] + in.ty[q] * P.inv_rho0H;
     const real us = u0 + dt_bt * (real(-kG) * gx + Fx);
     const real vs = v0 + dt_bt * (real(-kG) * gy + Fy);
-    real a = (us + fd * vs) / den;
-    real b = (vs - fd * us) / den;
+    real a = (us + fd * vs) * iden;
+    real b = (vs - fd * us) * iden;
     a = wc ? a * drag : real(0);
     b = wc ? b * drag : real(0);
     un[q] = act ? a : real(0);
@@ -2388,7 +2420,7 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
   }
   const real wb = cap_weight(P, j, g.ny);
   real zu = 0, zv = 0;
-  if (ALLRED || wb >= real(0)) {
+  if (ALLRED || allred || wb >= real(0)) {
     real s3[3] = {su, sv, sw};
     group_sums<real, 3>(s3, red, t);
     const real d = s3[2] > real(1) ? s3[2] : real(1);
@@ -2397,19 +2429,33 @@ __device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>&
   }
   if (!store) return;
 #pragma unroll
-  for (int q = 0; q < kRowChunk; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int i = t + q * 256;
     if (i >= g.nx) continue;
     const int c2 = j * g.nx + i;
     real a = un[q], b = vn[q];
     if (wb >= real(0)) {
-      const bool wc = wet[c2] != real(0);
+      const bool wc = in.wC[q] != real(0);
       a = wc ? wb * zu + (real(1) - wb) * a : real(0);
       b = wc ? wb * zv + (real(1) - wb) * b : real(0);
     }
     bt_store<COH>(uo + c2, a);
     bt_store<COH>(vo + c2, b);
   }
+}
+
+// The whole row update: bt_vel_load, then bt_vel_finish.
+template <typename real, bool ALLRED = false, bool COH = false, int NQ = kRowChunk>
+__device__ __forceinline__ void bt_vel_row(const Geo<real>& g, const Phys<real>& P, int j, int t,
+                                           const real* ec_row, const real* en_row, const real* es_row,
+                                           const real* __restrict__ ubt, const real* __restrict__ vbt, real* uo,
+                                           real* vo, const real* __restrict__ Frx, const real* __restrict__ Fry,
+                                           real fd, real iden, real dt_bt, real drag, real* red,
+                                           bool store = true, bool allred = false) {
+  BtVelIn<real, NQ> in;
+  bt_vel_load<real, COH, NQ>(g, P, j, t, ubt, vbt, Frx, Fry, in);
+  bt_vel_finish<real, ALLRED, COH, NQ>(g, P, j, t, in, ec_row, en_row, es_row, uo, vo, fd, iden, dt_bt, drag, red,
+                                       store, allred);
 }
 
 template <typename real>
@@ -2425,12 +2471,12 @@ template <typename real>
 __global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const real* __restrict__ eta,
                                                 real* __restrict__ ubt, real* __restrict__ vbt,
                                                 const real* __restrict__ Frx, const real* __restrict__ Fry,
-                                                const real* __restrict__ fdt, const real* __restrict__ fden,
+                                                const real* __restrict__ fdt, const real* __restrict__ fiden,
                                                 real dt_bt, real drag) {
   __shared__ real red[16];
   const int j = blockIdx.x;
   const real* ec = eta + (long long)j * g.nx;
-  bt_vel_row(g, P, j, (int)threadIdx.x, ec, ec + g.nx, ec - g.nx, ubt, vbt, ubt, vbt, Frx, Fry, fdt[j], fden[j], dt_bt,
+  bt_vel_row(g, P, j, (int)threadIdx.x, ec, ec + g.nx, ec - g.nx, ubt, vbt, ubt, vbt, Frx, Fry, fdt[j], fiden[j], dt_bt,
              drag, red);
 }
 
@@ -2440,44 +2486,51 @@ __global__ __launch_bounds__(256) void k_bt_vel(Geo<real> g, Phys<real> P, const
 // eta_out), then groups 0 and 1 update ubt, vbt of the owned rows from the
 // LDS rows.  Inputs and outputs are separate buffers (ping-pong across
 // substeps), so a block's halo rows of eta' equal those computed by their
-// owners.  Groups without a row of their own work on a clamped row and store
-// nothing, so every group reaches the same block-wide barriers.
-// One whole forward-backward substep per launch (the arithmetic of k_bt_eta
-// followed by k_bt_vel).  A block of 4 row groups owns rows j0, j0 + 1: group
-// r computes eta' of row j0 - 1 + r into LDS (the two owned rows also to
-// eta_out), then groups 0 and 1 update ubt, vbt of the owned rows from the
-// LDS rows.  Inputs and outputs are separate buffers (ping-pong across
-// substeps), so a block's halo rows of eta' equal those computed by their
-// owners.  Every group reduces on every row (the zonal means are used on the
-// polar-cap band rows only) so that all groups meet the same block-wide
-// barriers; groups without a row of their own work on a clamped row and
-// store nothing.
-template <typename real>
+// owners.  In blocks that touch a polar-cap band row every group reduces on
+// every row (the zonal means are used on the band rows only), so that all
+// groups meet the same block-wide barriers, and groups without a row of their
+// own work on a clamped row and store nothing.
+template <typename real, int NQ>
 __global__ __launch_bounds__(1024) void k_bt_sub(Geo<real> g, Phys<real> P, const real* __restrict__ eta_in,
                                                  const real* __restrict__ u_in, const real* __restrict__ v_in,
                                                  real* __restrict__ eta_out, real* __restrict__ u_out,
                                                  real* __restrict__ v_out, const real* __restrict__ Frx,
                                                  const real* __restrict__ Fry, const real* __restrict__ fdt,
-                                                 const real* __restrict__ fden, real dtH, real dt_bt, real drag) {
+                                                 const real* __restrict__ fiden, real dtH, real dt_bt, real drag) {
   __shared__ real erow[4][kRowChunk * 256];
   __shared__ real red[4][16];
   const int grp = (int)threadIdx.x >> 8, t = (int)threadIdx.x & 255;
   const int j0 = (int)blockIdx.x * 2;
+  // blocks whose rows (j0 - 1 .. j0 + 2) all lie outside the polar-cap bands
+  // need no zonal means: no group reduces, so no group meets a barrier inside
+  // the row functions, and the groups without an owned row skip the
+  // velocity phase (block-uniform test)
+  const bool band = P.nb > 0 && (j0 - 1 < P.nb || j0 + 2 >= g.ny - P.nb);
+  const int jv = j0 + (grp & 1);
+  const bool own = grp < 2 && jv < g.ny;
+  const int jc = jv < g.ny ? jv : g.ny - 1;
+  const bool vel = band || own;  // uniform per row group
+  // float32: the velocity phase's inputs are loaded first, so that their
+  // latency overlaps the eta phase (float64 keeps them after the barrier: the
+  // extra registers would spill)
+  constexpr bool kPre = sizeof(real) == 4;
+  BtVelIn<real, NQ> vin;
+  if (kPre && vel) bt_vel_load<real, false, NQ>(g, P, jc, t, u_in, v_in, Frx, Fry, vin);
   {
     const int je = j0 - 1 + grp;
     const bool in = je >= 0 && je < g.ny;
     const int jc = in ? je : (je < 0 ? 0 : g.ny - 1);
     const bool own = in && (grp == 1 || grp == 2);
-    bt_eta_row<real, true>(g, P, jc, t, eta_in + (long long)jc * g.nx, u_in, v_in, dtH, &erow[grp][0], &red[grp][0],
-                           own ? eta_out + (long long)jc * g.nx : (real*)nullptr);
+    bt_eta_row<real, false, false, NQ>(g, P, jc, t, eta_in + (long long)jc * g.nx, u_in, v_in, dtH, &erow[grp][0],
+                                       &red[grp][0], own ? eta_out + (long long)jc * g.nx : (real*)nullptr, band);
   }
   __syncthreads();
-  const int jv = j0 + (grp & 1);
-  const bool own = grp < 2 && jv < g.ny;
-  const int jc = jv < g.ny ? jv : g.ny - 1;
   const int sl = jc - (j0 - 1);  // LDS slot of row jc (1 or 2)
-  bt_vel_row<real, true>(g, P, jc, t, &erow[sl][0], &erow[sl + 1][0], &erow[sl - 1][0], u_in, v_in, u_out, v_out, Frx,
-                         Fry, fdt[jc], fden[jc], dt_bt, drag, &red[grp][0], own);
+  if (vel) {
+    if (!kPre) bt_vel_load<real, false, NQ>(g, P, jc, t, u_in, v_in, Frx, Fry, vin);
+    bt_vel_finish<real, false, false, NQ>(g, P, jc, t, vin, &erow[sl][0], &erow[sl + 1][0], &erow[sl - 1][0], u_out,
+                                          v_out, fdt[jc], fiden[jc], dt_bt, drag, &red[grp][0], own, band);
+  }
 }
 
 // The whole sub-cycle of n_sub substeps in one persistent launch.  Block b
@@ -2528,7 +2581,7 @@ __global__ __launch_bounds__(1024) void k_bt_persist(Geo<real> g, Phys<real> P, 
                                                      const real* __restrict__ u0, const real* __restrict__ v0,
                                                      real* ea, real* ua, real* va, real* eb, real* ub, real* vb,
                                                      const real* __restrict__ Frx, const real* __restrict__ Fry,
-                                                     const real* __restrict__ fdt, const real* __restrict__ fden,
+                                                     const real* __restrict__ fdt, const real* __restrict__ fiden,
                                                      real dtH, real dt_bt, real drag, int nsub, int* progress,
                                                      int* timeouts) {
   __shared__ real erow[4][kRowChunk * 256];
@@ -2582,7 +2635,7 @@ __global__ __launch_bounds__(1024) void k_bt_persist(Geo<real> g, Phys<real> P, 
       const int jc = jv < g.ny ? jv : g.ny - 1;
       const int sl = jc - (j0 - 1);
       bt_vel_row<real, true, true>(g, Pq, jc, t, &erow[sl][0], &erow[sl + 1][0], &erow[sl - 1][0], ui, vi, uo,
-                                   vo, Frq, Fyq, fdt[jc], fden[jc], dt_bt, drag, &red[grp][0], own);
+                                   vo, Frq, Fyq, fdt[jc], fiden[jc], dt_bt, drag, &red[grp][0], own);
     }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
     __syncthreads();
@@ -2627,7 +2680,7 @@ __global__ __launch_bounds__(256) void k_bt_vel_multi(const BtMember<real>* __re
   const int j = (int)blockIdx.x - m.row0;
   const real* ec = m.eta + (long long)j * m.g.nx;
   bt_vel_row(m.g, m.P, j, (int)threadIdx.x, ec, ec + m.g.nx, ec - m.g.nx, m.ubt, m.vbt, m.ubt, m.vbt, m.Frx, m.Fry,
-             m.fdt[j], m.fden[j], m.dt_bt, m.drag, red);
+             m.fdt[j], m.fiden[j], m.dt_bt, m.drag, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -2985,29 +3038,39 @@ void launch_bt_eta(const Geo<real>& g, const Phys<real>& P, real* eta, const rea
 
 template <typename real>
 void launch_bt_vel(const Geo<real>& g, const Phys<real>& P, const real* eta, real* ubt, real* vbt,
-                   const real* Frx, const real* Fry, const real* fdt, const real* fden, real dt_bt, real drag,
+                   const real* Frx, const real* Fry, const real* fdt, const real* fiden, real dt_bt, real drag,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_bt_vel<real>, dim3(g.ny), dim3(256), 0, s, g, P, eta, ubt, vbt, Frx, Fry, fdt, fden,
+  hipLaunchKernelGGL(k_bt_vel<real>, dim3(g.ny), dim3(256), 0, s, g, P, eta, ubt, vbt, Frx, Fry, fdt, fiden,
                      dt_bt, drag);
 }
 
 template <typename real>
 void launch_bt_sub(const Geo<real>& g, const Phys<real>& P, const real* eta_in, const real* u_in,
                    const real* v_in, real* eta_out, real* u_out, real* v_out, const real* Frx, const real* Fry,
-                   const real* fdt, const real* fden, real dtH, real dt_bt, real drag, hipStream_t s) {
-  hipLaunchKernelGGL(k_bt_sub<real>, dim3((unsigned)((g.ny + 1) / 2)), dim3(1024), 0, s, g, P, eta_in, u_in, v_in,
-                     eta_out, u_out, v_out, Frx, Fry, fdt, fden, dtH, dt_bt, drag);
+                   const real* fdt, const real* fiden, real dtH, real dt_bt, real drag, hipStream_t s) {
+  // column chunks of 256 per row: exactly as many as the row needs, so that
+  // no thread runs a chunk past the row end
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)((g.ny + 1) / 2)), dim3(1024), 0, s, g, P, eta_in, u_in, v_in, eta_out,
+                       u_out, v_out, Frx, Fry, fdt, fiden, dtH, dt_bt, drag);
+  };
+  switch ((g.nx + 255) / 256) {
+    case 1: go(k_bt_sub<real, 1>); break;
+    case 2: go(k_bt_sub<real, 2>); break;
+    case 3: go(k_bt_sub<real, 3>); break;
+    default: go(k_bt_sub<real, kRowChunk>); break;
+  }
 }
 
 template <typename real>
 void launch_bt_persist(const Geo<real>& g, const Phys<real>& P, const real* eta0, const real* u0, const real* v0,
                        real* ea, real* ua, real* va, real* eb, real* ub, real* vb, const real* Frx, const real* Fry,
-                       const real* fdt, const real* fden, real dtH, real dt_bt, real drag, int nsub, int* progress,
+                       const real* fdt, const real* fiden, real dtH, real dt_bt, real drag, int nsub, int* progress,
                        int* timeouts, hipStream_t s) {
   const int nblocks = (g.ny + 1) / 2;
   hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, s, progress, nblocks);
   hipLaunchKernelGGL(k_bt_persist<real>, dim3((unsigned)nblocks), dim3(1024), 0, s, g, P, eta0, u0, v0, ea, ua, va,
-                     eb, ub, vb, Frx, Fry, fdt, fden, dtH, dt_bt, drag, nsub, progress, timeouts);
+                     eb, ub, vb, Frx, Fry, fdt, fiden, dtH, dt_bt, drag, nsub, progress, timeouts);
 }
 
 template <typename real>
