@@ -114,9 +114,12 @@ def test_residual_stages_match_oracle(dtype):
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
 def test_dealiased_momentum_residual_at_production_row_lengths(nx, dtype):
     """The x de-alias inside the momentum residual at the row lengths of the
-    production grids (Stockham FFT over radices 2, 3, 4, 5: a block's rows
-    transformed together, the last block partial) and at a length with other
-    prime factors (four-step DFT), against the oracle's numpy FFT projection."""
+    production grids (float64: Stockham FFT over radices 2, 3, 4, 5, a block's
+    rows transformed together; float32: the two-factor register FFT; the last
+    block partial in both) and at a length with other prime factors (four-step
+    DFT), against the oracle's numpy FFT projection.  Float32 errors are about
+    1e-5 (float32 rounding of the residual); a wrong root or index would give
+    O(1)."""
     print(f"case land {nx}x6x3 momentum residual {dtype}", flush=True)
     grid, physics, dt, kwargs, T, S = land_case(nx=nx, ny=6, nz=3)
     kwargs = dict(kwargs, polar_cap_rows=0, polar_cap_taper=0)  # six rows leave no room for the cap bands
@@ -128,7 +131,7 @@ def test_dealiased_momentum_residual_at_production_row_lengths(nx, dtype):
     du, dv = du + p.r_bot * s0.u * bm, dv + p.r_bot * s0.v * bm
     gm = to_numpy_state(step.solver.stage(_lib.STAGE_MOMENTUM_RES, JaxStateG(*s0)))
     wet = p.wet_mask_z > 0
-    tol = 1e-10 if dtype == "float64" else 2e-3
+    tol = 1e-10 if dtype == "float64" else 1e-4
     for got, ref, name in ((gm.u, du, "du"), (gm.v, dv, "dv")):
         e = rel(got[wet], ref[wet])
         print(name, f"{e:.1e}", flush=True)

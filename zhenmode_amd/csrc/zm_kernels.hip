@@ -5,6 +5,7 @@
 // function it follows is named above each kernel.  All face gates of the
 // reference (wet * neighbour-wet products) are evaluated as selects on the
 // wet-level count, which equals the reference's 0/1 products for finite data.
+#include <type_traits>
 #include <stdexcept>
 
 #include "zm_common.h"
@@ -2095,6 +2096,222 @@ __global__ __launch_bounds__(256) void k_dealias_fixed(const int* __restrict__ k
 }
 
 // ---------------------------------------------------------------------------
+// Two-factor form of the same projection for float32 rows, N = N1 N2 with
+// n = n1 + N1 n2 and k = k2 + N2 k1:
+//   X[k2 + N2 k1] = sum_n1 w_N1^(n1 k1) w_N^(n1 k2) sum_n2 x[n1 + N1 n2] w_N2^(n2 k2)
+// and the inverse with the conjugate roots in the reverse order.  A thread
+// holds one n1 (N2 values) or one k2 (N1 values) of a row in registers: the
+// N2-point DFT, the twiddle w_N^(n1 k2) and an LDS transpose; the N1-point
+// DFT, the mask, the inverse N1-point DFT and the conjugate twiddle; another
+// transpose and the inverse N2-point DFT, stored straight to HBM.  Two
+// transposes and three barriers per row instead of a pass through LDS per
+// radix stage; the in-register DFTs are Stockham stages unrolled at compile
+// time with their roots as constants (quarter turns without products).
+// ---------------------------------------------------------------------------
+constexpr double kTwoPi = 6.28318530717958647692;
+
+// cos and sin by their Taylor series, |x| <= pi (double precision at 30 terms)
+constexpr double ct_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 30; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ct_sin(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 30; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+// 2 pi m / L reduced to (-pi, pi]
+constexpr double ct_angle(int m, int L) {
+  const double a = kTwoPi * (double)m / (double)L;
+  return a > kTwoPi / 2.0 ? a - kTwoPi : a;
+}
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// v exp(-+2 pi i m / L) for a compile-time m (+ for the inverse)
+template <typename real, int M, int L, bool INV>
+__device__ __forceinline__ Cx<real> twiddle_ct(Cx<real> v) {
+  constexpr int m = M % L;
+  if constexpr (m == 0) {
+    return v;
+  } else if constexpr (4 * m == L) {
+    return rot<real, INV>(v);
+  } else if constexpr (2 * m == L) {
+    return Cx<real>{-v.re, -v.im};
+  } else if constexpr (4 * m == 3 * L) {
+    return rot<real, !INV>(v);
+  } else {
+    constexpr double a = ct_angle(m, L);
+    constexpr real c = (real)ct_cos(a);
+    constexpr real sn = (real)(INV ? ct_sin(a) : -ct_sin(a));
+    return cmul(v, Cx<real>{c, sn});
+  }
+}
+
+// one Stockham stage of an L-point DFT held in registers (fixed_stage with
+// compile-time indices and roots)
+template <typename real, int L, int S, bool INV>
+__device__ __forceinline__ void reg_stage(const Cx<real> (&in)[L], Cx<real> (&out)[L]) {
+  constexpr int R = FixedFft<L>::radix(S), Ns = FixedFft<L>::span(S), T = L / R, tstep = L / (Ns * R);
+  static_for<0, T>([&](auto bfc) {
+    constexpr int bf = decltype(bfc)::value, k = bf % Ns;
+    Cx<real> v[R];
+    static_for<0, R>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      v[r] = twiddle_ct<real, r * k * tstep, L, INV>(in[bf + r * T]);
+    });
+    small_dft<real, R, INV>(v);
+    static_for<0, R>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      out[(bf - k) * R + k + q * Ns] = v[q];
+    });
+  });
+}
+
+template <typename real, int L, int S, bool INV>
+__device__ __forceinline__ void reg_stages(Cx<real> (&a)[L], Cx<real> (&b)[L]) {
+  if constexpr (S < FixedFft<L>::stages()) {
+    reg_stage<real, L, S, INV>(a, b);
+    reg_stages<real, L, S + 1, INV>(b, a);
+  }
+}
+
+// y_q = sum_r v_r exp(-+2 pi i r q / L), in place in registers
+template <typename real, int L, bool INV>
+__device__ __forceinline__ void reg_dft(Cx<real> (&v)[L]) {
+  static_assert(FixedFft<L>::complete(), "L must factor into 2, 3, 4 and 5");
+  Cx<real> w[L];
+  reg_stages<real, L, 0, INV>(v, w);
+  if constexpr (FixedFft<L>::stages() % 2 == 1)
+    static_for<0, L>([&](auto ic) { v[decltype(ic)::value] = w[decltype(ic)::value]; });
+}
+
+template <int N>
+struct TwoFactor;
+template <>
+struct TwoFactor<720> {
+  static constexpr int N1 = 24, N2 = 30;
+};
+template <>
+struct TwoFactor<800> {
+  static constexpr int N1 = 25, N2 = 32;
+};
+template <>
+struct TwoFactor<480> {
+  static constexpr int N1 = 20, N2 = 24;
+};
+template <>
+struct TwoFactor<360> {
+  static constexpr int N1 = 18, N2 = 20;
+};
+template <>
+struct TwoFactor<180> {
+  static constexpr int N1 = 12, N2 = 15;
+};
+template <>
+struct TwoFactor<90> {
+  static constexpr int N1 = 9, N2 = 10;
+};
+
+template <int N>
+struct TwoFactorShape {
+  static constexpr int N1 = TwoFactor<N>::N1, N2 = TwoFactor<N>::N2;
+  static_assert(N1 * N2 == N, "factorisation");
+  static constexpr int RB = 256 / (N1 > N2 ? N1 : N2);  // rows per 256-thread block
+  // odd strides of the transposes: 64-bit LDS accesses of consecutive threads
+  // at stride P1 (P2) complex values fall in distinct banks
+  static constexpr int P1 = N1 % 2 ? N1 : N1 + 1, P2 = N2 % 2 ? N2 : N2 + 1;
+  static constexpr int RS = N2 * P1 > N1 * P2 ? N2 * P1 : N1 * P2;  // LDS complex values per row
+  static constexpr size_t lds() { return (size_t)(N + RB * RS) * sizeof(Cx<float>); }
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_dealias_2f(const int* __restrict__ kept, int nrows,
+                                                    const float* __restrict__ twr, const float* __restrict__ twi,
+                                                    const float* __restrict__ ina, const float* __restrict__ inb,
+                                                    float* __restrict__ outa, float* __restrict__ outb) {
+  using Sh = TwoFactorShape<N>;
+  constexpr int N1 = Sh::N1, N2 = Sh::N2, RB = Sh::RB, P1 = Sh::P1, P2 = Sh::P2, RS = Sh::RS;
+  extern __shared__ unsigned char zm_lds[];
+  Cx<float>* tw = reinterpret_cast<Cx<float>*>(zm_lds);  // exp(-2 pi i e / N)
+  Cx<float>* buf = tw + N;
+  const int tid = (int)threadIdx.x, row0 = (int)blockIdx.x * RB;
+  for (int t = tid; t < N; t += 256) tw[t] = Cx<float>{twr[t], twi[t]};
+  // n1 threads (transforms over n2) and k2 threads (transforms over k1)
+  const int ra = tid / N1, n1 = tid - ra * N1;
+  const bool actA = ra < RB && row0 + ra < nrows;
+  const int rb = tid / N2, k2 = tid - rb * N2;
+  const bool actB = rb < RB && row0 + rb < nrows;
+  Cx<float> x[N2];
+  const long long baseA = (long long)(row0 + ra) * N + n1;
+  if (actA)
+    static_for<0, N2>([&](auto c) {
+      constexpr int n2 = decltype(c)::value;
+      x[n2] = Cx<float>{ina[baseA + N1 * n2], inb[baseA + N1 * n2]};
+    });
+  __syncthreads();  // roots in LDS
+  if (actA) {
+    reg_dft<float, N2, false>(x);
+    Cx<float>* y = buf + ra * RS + n1;
+    static_for<0, N2>([&](auto c) {
+      constexpr int q = decltype(c)::value;
+      y[q * P1] = q == 0 ? x[0] : cmul(x[q], tw[n1 * q]);
+    });
+  }
+  __syncthreads();
+  Cx<float> z[N1];
+  if (actB) {
+    const Cx<float>* y = buf + rb * RS + k2 * P1;
+    static_for<0, N1>([&](auto c) { z[decltype(c)::value] = y[decltype(c)::value]; });
+  }
+  __syncthreads();  // every row of Y read before the inverse overwrites it
+  if (actB) {
+    reg_dft<float, N1, false>(z);  // z[k1] = X[k2 + N2 k1]
+    static_for<0, N1>([&](auto c) {
+      constexpr int k1 = decltype(c)::value;
+      if (!kept[k2 + N2 * k1]) z[k1] = Cx<float>{0.0f, 0.0f};
+    });
+    reg_dft<float, N1, true>(z);
+    Cx<float>* zo = buf + rb * RS + k2;
+    static_for<0, N1>([&](auto c) {
+      constexpr int q = decltype(c)::value;
+      if constexpr (q == 0) {
+        zo[0] = z[0];
+      } else {
+        Cx<float> w = tw[q * k2];
+        w.im = -w.im;
+        zo[q * P2] = cmul(z[q], w);
+      }
+    });
+  }
+  __syncthreads();
+  if (actA) {
+    const Cx<float>* zi = buf + ra * RS + n1 * P2;
+    static_for<0, N2>([&](auto c) { x[decltype(c)::value] = zi[decltype(c)::value]; });
+    reg_dft<float, N2, true>(x);
+    const float invN = 1.0f / (float)N;
+    static_for<0, N2>([&](auto c) {
+      constexpr int n2 = decltype(c)::value;
+      outa[baseA + N1 * n2] = x[n2].re * invN;
+      outb[baseA + N1 * n2] = x[n2].im * invN;
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Momentum residual (tendencies.py:39-72 and 247-319, legacy mode-split
 // branch) fused with the RK2 velocity update (step.py:263-319).
 //   mode 0: du1 -> buffers, u_pred = u + du1 dt
@@ -3004,6 +3221,16 @@ void launch_dealias_x(int N, int N1, int N2, const int* kept, int nrows, const r
 template <typename real>
 void launch_dealias_fft(int N, const FftPlan& plan, const int* kept, int nrows, const real* twr, const real* twi,
                         const real* ina, const real* inb, real* outa, real* outb, hipStream_t s) {
+  if constexpr (sizeof(real) == 4) {
+#define ZM_TWO_FACTOR(LEN)                                                                                       if (N == LEN) {                                                                                                 using Sh = TwoFactorShape<LEN>;                                                                               hipLaunchKernelGGL((k_dealias_2f<LEN>), dim3((unsigned)((nrows + Sh::RB - 1) / Sh::RB)), dim3(256),                              Sh::lds(), s, kept, nrows, (const float*)twr, (const float*)twi, (const float*)ina,                           (const float*)inb, (float*)outa, (float*)outb);                                            return;                                                                                                     }
+    ZM_TWO_FACTOR(720)
+    ZM_TWO_FACTOR(800)
+    ZM_TWO_FACTOR(480)
+    ZM_TWO_FACTOR(360)
+    ZM_TWO_FACTOR(180)
+    ZM_TWO_FACTOR(90)
+#undef ZM_TWO_FACTOR
+  }
   constexpr int RB = fixed_fft_rows<real>();
   const size_t lds_fixed = (size_t)(1 + 2 * RB) * N * sizeof(Cx<real>);
   const unsigned fixed_blocks = (unsigned)((nrows + RB - 1) / RB);
