@@ -204,16 +204,42 @@ int zm_column_features(zm_handle* h, const zm_state* in, const void* tau_x, cons
                        void* stream);
 
 /* Apply surrogate outputs Y (float32 rows of ldy >= 2 nz: nz T then nz S
- * values, normalised): dX = dt (y out_std + out_mean) on wet levels, shifted
- * so that the float64 column integral sum dz_node dX is zero (heat and salt
- * conserved exactly), X_new = X + dX into T_out/S_out.  Columns whose implicit
- * residual max|X_new - dt D(X_new) - X| exceeds tol_T / tol_S get flags[c] = 1
- * and are counted into *n_flagged (device int32, accumulated); the caller
- * passes the same flags to zm_vertical_mix to replace them by the Thomas
- * solution.  nz <= 50. */
+ * values, normalised).  Per column and tracer X, with e_k the explicit
+ * increment of the mixing operator over dt (dt D_kappa X, kappa as in
+ * zm_vertical_mix) and s = max_k |e_k| the column's mixing scale:
+ * dX_k = e_k + s (y_k out_std_k + out_mean_k) on wet levels, shifted so that
+ * the float64 column integral sum dz_node dX is zero (heat and salt conserved
+ * exactly), X_new = X + dX into T_out/S_out.  Columns whose implicit residual
+ * max|X_new - dt D(X_new) - X| exceeds tol_T / tol_S get flags[c] = 1 and are
+ * counted into *n_flagged (device int32, accumulated); the caller passes the
+ * same flags to zm_vertical_mix to replace them by the Thomas solution.
+ * nz <= 50.  No reference counterpart (the reference has no learned part). */
 int zm_mlp_apply(zm_handle* h, const zm_state* in, const float* Y, int32_t ldy, const float* out_mean,
                  const float* out_std, double dt, double kappa_v, double kappa_conv, double tol_T,
                  double tol_S, void* T_out, void* S_out, int32_t* flags, int32_t* n_flagged, void* stream);
+
+/* Training targets of the surrogate, as float32 rows Y[c][0..ldy) (row
+ * c = j nx + i; nz T values, nz S values, zero padding; ldy >= 2 nz): per
+ * column and tracer, (X_thomas_k - X_k - e_k) / s with e_k, s as in
+ * zm_mlp_apply, 0 where s = 0 and on dry levels.  T_thomas / S_thomas hold the
+ * zm_vertical_mix solution of the same state with the same dt, kappa_v,
+ * kappa_conv.  No reference counterpart (the reference has no learned part). */
+int zm_mix_targets(zm_handle* h, const zm_state* in, const void* T_thomas, const void* S_thomas, double dt,
+                   double kappa_v, double kappa_conv, float* Y, int32_t ldy, void* stream);
+
+/* The surrogate step in one launch on a float32 handle: features as
+ * zm_column_features (in_mean / in_inv_std), the four layers
+ * 256 -> 256 -> 256 -> 256 -> 128 (weights[l]: bfloat16 [N][256] row-major,
+ * biases[l]: float32 [N]; GELU after the first three) on MFMA, and the apply
+ * step of zm_mlp_apply, with the activations kept on chip.  Results equal
+ * zm_column_features -> zm_gemm_bf16_nt x 4 -> zm_mlp_apply bit for bit.
+ * tau_x / tau_y / q_heat may be NULL for the handle's frozen forcing.  nz <= 50.
+ * No reference counterpart (the reference has no learned part). */
+int zm_mlp_mix(zm_handle* h, const zm_state* in, const void* tau_x, const void* tau_y, const void* q_heat,
+               const float* in_mean, const float* in_inv_std, const void* const* weights, const float* const* biases,
+               const float* out_mean, const float* out_std, double dt, double kappa_v, double kappa_conv,
+               double tol_T, double tol_S, void* T_out, void* S_out, int32_t* flags, int32_t* n_flagged,
+               void* stream);
 
 /* C = A B^T, A (M x K) and B (N x K) bfloat16 row-major device arrays, float32
  * accumulation on MFMA.  M and N multiples of 128, K a multiple of 32.

@@ -27,7 +27,12 @@ How each function is pinned (``tests/test_oracle_column.py``):
   * ``mlp_forward`` / ``mlp_backward``: central finite differences of the loss
     and a closed-form one-layer case;
   * ``integral_correction``: the corrected column integral equals the target
-    by construction, checked in float64.
+    by construction, checked in float64;
+  * ``explicit_increment``: the flux form of ``mix_residual`` (e = -r(X, X)),
+    telescoping column integrals and a two-level closed form;
+  * ``surrogate_targets`` / ``learned_apply``: the exact targets fed back
+    through the apply step reproduce the Thomas solution with zero residual,
+    and a stable column's scale is dt kappa_v times its curvature.
 """
 
 from __future__ import annotations
@@ -122,6 +127,62 @@ def integral_correction(dX, target, p):
     err = column_integral(dX, p) - target
     shift = np.where(h > 0, err / np.where(h > 0, h, 1.0), 0.0)
     return (dX - shift[..., None]) * wm
+
+
+# --------------------------------------------------------------------------
+# the surrogate's increment formulation
+# --------------------------------------------------------------------------
+def explicit_increment(X, kappa, p, dt):
+    """dt D_kappa X on wet levels (0 on dry levels), written through the
+    tridiagonal matrix of the implicit step: e = X - (I - dt D) X, with
+    (I - dt D) from ``tridiagonal``.  The flux form of ``mix_residual`` is an
+    independent restatement of the same operator (e = -mix_residual(X, X))."""
+    a, b, c = tridiagonal(kappa, p, dt)
+    X = np.asarray(X, dtype=np.float64)
+    AX = b * X
+    AX[..., 1:] += a[..., 1:] * X[..., :-1]
+    AX[..., :-1] += c[..., :-1] * X[..., 1:]
+    return np.where(p.wet_mask_z > 0.5, X - AX, 0.0)
+
+
+def mixing_scale(e):
+    """Per-column max |e| over the levels (dry levels carry e = 0)."""
+    return np.max(np.abs(e), axis=-1)
+
+
+def surrogate_targets(T, S, Tn, Sn, p, dt, kappa_v, kappa_conv):
+    """(nx, ny, 2 nz) targets: per tracer (X_new - X - e) / s, 0 where s = 0
+    and on dry levels (X_new: the implicit solution of the same state)."""
+    kap = mixing_kappa(T, S, p, kappa_v, kappa_conv)
+    out = []
+    for X, Xn in ((T, Tn), (S, Sn)):
+        e = explicit_increment(X, kap, p, dt)
+        s = mixing_scale(e)
+        safe = np.where(s > 0, s, 1.0)[..., None]
+        t = np.where((s > 0)[..., None] & (p.wet_mask_z > 0.5), (Xn - X - e) / safe, 0.0)
+        out.append(t)
+    return np.concatenate(out, axis=-1)
+
+
+def learned_apply(T, S, z, p, dt, kappa_v, kappa_conv, tol_T, tol_S):
+    """The surrogate's apply step from de-normalised outputs z (nx, ny, 2 nz):
+    dX = e + s z per tracer, shifted so its column integral is zero, X_new =
+    X + dX on wet levels; flags = columns whose implicit-system residual
+    exceeds tol_T or tol_S.  Returns (T_new, S_new, flags, r_T, r_S)."""
+    nz = T.shape[-1]
+    kap = mixing_kappa(T, S, p, kappa_v, kappa_conv)
+    wm = p.wet_mask_z
+    new, res = [], []
+    for X, zx in ((T, z[..., :nz]), (S, z[..., nz:])):
+        e = explicit_increment(X, kap, p, dt)
+        s = mixing_scale(e)
+        dX = (e + s[..., None] * zx) * wm
+        dX = integral_correction(dX, 0.0, p)
+        Xn = np.where(wm > 0.5, X + dX, X)
+        new.append(Xn)
+        res.append(np.max(np.abs(mix_residual(Xn, X, kap, p, dt) * wm), axis=-1))
+    flags = (res[0] > tol_T) | (res[1] > tol_S)
+    return new[0], new[1], flags, res[0], res[1]
 
 
 # --------------------------------------------------------------------------

@@ -173,3 +173,95 @@ def test_column_features_layout(land):
     np.testing.assert_array_equal(row[nz:2 * nz], s.S[i, j] * wm)
     assert row[5 * nz] == q[i, j] and row[5 * nz + 1] == 2 * q[i, j] and row[5 * nz + 2] == 3 * q[i, j]
     assert np.all(row[5 * nz + 3:] == 0.0)
+
+
+# --------------------------------------------------------------------------
+# the surrogate's increment formulation
+# --------------------------------------------------------------------------
+def _unstable_land(land):
+    p, s = land
+    rng = np.random.default_rng(11)
+    Tu = s.T.copy()
+    Tu[:, :, 1] += 3.0 * (rng.uniform(size=Tu.shape[:2]) > 0.6)
+    return p, np.where(p.wet_mask_z > 0.5, Tu, s.T), s.S
+
+
+def _flux_form_increment(X, kap, p, dt):
+    """dt D_kappa X through zm_oracle's interface-flux operator."""
+    flux = O.diffusive_interface_flux(X, 1.0, p) * kap
+    return dt * O.iface_flux_divergence(flux, p) * p.wet_mask_z
+
+
+def test_explicit_increment_matches_the_flux_form_and_telescopes(land):
+    p, T, S = _unstable_land(land)
+    dt = 3600.0
+    kap = C.mixing_kappa(T, S, p, 1e-3, 0.5)
+    for X in (T, S):
+        e = C.explicit_increment(X, kap, p, dt)
+        ref = _flux_form_increment(X, kap, p, dt)
+        assert np.max(np.abs(e - ref)) <= 1e-12 * np.max(np.abs(ref))
+        # the same operator as the implicit step's residual at X_new = X
+        np.testing.assert_allclose(e, -C.mix_residual(X, X, kap, p, dt) * p.wet_mask_z, rtol=0,
+                                   atol=1e-12 * np.max(np.abs(ref)))
+        assert np.max(np.abs(C.column_integral(e, p))) <= 1e-12 * np.max(np.abs(C.column_integral(np.abs(e), p)))
+    # planted mistake: the sub- and super-diagonals exchanged
+    def swapped(X, kappa, p, dt):
+        a, b, c = C.tridiagonal(kappa, p, dt)
+        AX = b * X
+        AX[..., 1:] += c[..., :-1] * X[..., :-1]
+        AX[..., :-1] += a[..., 1:] * X[..., 1:]
+        return np.where(p.wet_mask_z > 0.5, X - AX, 0.0)
+    bad = swapped(T, kap, p, dt)
+    assert np.max(np.abs(bad - _flux_form_increment(T, kap, p, dt))) > 1e-6 * np.max(np.abs(bad))
+
+
+def test_explicit_increment_two_level_closed_form(land):
+    """Two wet levels: e_0 = dt k (X1 - X0) / (dzi h0), e_1 = -dt k (X1 - X0) / (dzi h1)."""
+    p, s = land
+    nw = (p.wet_mask_z > 0.5).sum(axis=-1)
+    i, j = np.argwhere(nw == 2)[0]
+    dt, kv = 5000.0, 2e-3
+    T = s.T.copy()
+    T[i, j, :2] = (20.0, 10.0)
+    S = s.S.copy()
+    S[i, j, :2] = (35.0, 35.0)
+    kap = C.mixing_kappa(T, S, p, kv, 0.0)
+    e = C.explicit_increment(T, kap, p, dt)
+    h0, h1, dzi = p.dz_node[0], p.dz_node[1], p.dz_iface[0]
+    np.testing.assert_allclose(e[i, j, :2], (-10.0 * dt * kv / (dzi * h0), 10.0 * dt * kv / (dzi * h1)), rtol=1e-13)
+    assert C.mixing_scale(e)[i, j] == pytest.approx(10.0 * dt * kv / (dzi * min(h0, h1)), rel=1e-13)
+
+
+def test_exact_targets_fed_back_reproduce_the_thomas_solution(land):
+    p, T, S = _unstable_land(land)
+    dt, kv, kc = 3600.0, 1e-3, 0.5
+    Tn, Sn = C.implicit_mix(T, S, p, dt, kv, kc)
+    z = C.surrogate_targets(T, S, Tn, Sn, p, dt, kv, kc)
+    assert np.all(np.isfinite(z))
+    Ta, Sa, flags, rT, rS = C.learned_apply(T, S, z, p, dt, kv, kc, 1e-9, 1e-9)
+    np.testing.assert_allclose(Ta, Tn, rtol=0, atol=1e-11 * np.max(np.abs(T)))
+    np.testing.assert_allclose(Sa, Sn, rtol=0, atol=1e-11 * np.max(np.abs(S)))
+    assert not flags.any() and max(rT.max(), rS.max()) <= 1e-10
+    # zero outputs give the explicit step, whose residual is large in convecting columns
+    _, _, flags0, rT0, _ = C.learned_apply(T, S, np.zeros_like(z), p, dt, kv, kc, 1e-3, 1e-3)
+    assert flags0.any() and rT0.max() > 1e-2
+    # planted mistake: the explicit increment left out of the apply step
+    def without_e(T, S, z):
+        nz = T.shape[-1]
+        kap = C.mixing_kappa(T, S, p, kv, kc)
+        e = C.explicit_increment(T, kap, p, dt)
+        dX = C.integral_correction(C.mixing_scale(e)[..., None] * z[..., :nz] * p.wet_mask_z, 0.0, p)
+        return np.where(p.wet_mask_z > 0.5, T + dX, T)
+    assert np.max(np.abs(without_e(T, S, z) - Tn)) > 1e-3
+
+
+def test_learned_apply_conserves_column_integrals(land):
+    p, T, S = _unstable_land(land)
+    rng = np.random.default_rng(4)
+    z = rng.standard_normal(T.shape[:2] + (2 * p.nz,))
+    Ta, Sa, _, _, _ = C.learned_apply(T, S, z, p, 3600.0, 1e-3, 0.5, 1e-3, 1e-3)
+    for Xn, X in ((Ta, T), (Sa, S)):
+        i0, i1 = C.column_integral(X, p), C.column_integral(Xn, p)
+        assert np.max(np.abs(i1 - i0)) <= 1e-12 * np.max(np.abs(i0))
+        dry = p.wet_mask_z < 0.5
+        np.testing.assert_array_equal(Xn[dry], X[dry])

@@ -70,7 +70,7 @@ EXPORTS = ("zm_create", "zm_destroy", "zm_last_error", "zm_version", "zm_step", 
            "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget",
            "zm_project_velocity", "zm_projection_iterations", "zm_projection_operator", "zm_budget_step",
            "zm_vertical_mix_noalloc", "zm_set_barotropic_persistent", "zm_barotropic_timeouts",
-           "zm_tracer_terms")
+           "zm_tracer_terms", "zm_mix_targets", "zm_mlp_mix")
 
 # epilogues of zm_gemm_bf16_nt
 EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU, EPI_GELU_GRAD = range(4)
@@ -119,6 +119,9 @@ def load():
     lib.zm_column_features.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp, vp, vp, vp, vp, i64, vp]
     lib.zm_mlp_apply.argtypes = [vp, ctypes.POINTER(ZmState), vp, i32, vp, vp, f64, f64, f64, f64, f64, vp, vp,
                                  vp, vp, vp]
+    lib.zm_mix_targets.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp, f64, f64, f64, vp, i32, vp]
+    lib.zm_mlp_mix.argtypes = [vp, ctypes.POINTER(ZmState), vp, vp, vp, vp, vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                               vp, vp, f64, f64, f64, f64, f64, vp, vp, vp, vp, vp]
     lib.zm_gemm_bf16_nt.argtypes = [i64, i64, i64, vp, vp, i32, vp, vp, vp, vp, vp, vp]
     lib.zm_mse_grad.argtypes = [i64, i32, i32, vp, vp, vp, vp, vp, vp]
     lib.zm_rowsum_bf16.argtypes = [i64, i64, vp, vp, vp]
@@ -136,7 +139,8 @@ def load():
                  "zm_projection_iterations", "zm_projection_operator", "zm_budget_step"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("zm_vertical_mix", "zm_vertical_mix_noalloc", "zm_set_barotropic_persistent",
-                 "zm_barotropic_timeouts", "zm_column_features", "zm_mlp_apply", "zm_gemm_bf16_nt", "zm_mse_grad",
+                 "zm_barotropic_timeouts", "zm_column_features", "zm_mlp_apply", "zm_mix_targets", "zm_mlp_mix",
+                 "zm_gemm_bf16_nt", "zm_mse_grad",
                  "zm_rowsum_bf16", "zm_transpose_bf16", "zm_adam", "zm_set_tiles", "zm_get_tiles", "zm_autotune",
            "zm_ensemble_create", "zm_ensemble_step", "zm_ensemble_set_graph", "zm_ensemble_size",
            "zm_ensemble_destroy", "zm_seasonal_forcing", "zm_monitor_reset", "zm_monitor", "zm_budget"):

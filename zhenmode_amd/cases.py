@@ -11,6 +11,12 @@ g05deg   720 x 320 x 50, H = 5500 m   +-80      900     45  seasonal wind + rest
 g045deg  800 x 356 x 50, H = 5500 m   +-80      900     45  seasonal wind + restore
 =======  ===========================  =====  ======  =====  ======================
 
+g045deg runs the reference's ``monotone_adv`` tracer advection: with the
+centred scheme its T grows without bound at one 72 m deep coastal corner cell
+from about day 7 (the plain and the Thomas-mixing steps alike); with
+``monotone_adv``, ``fct_adv`` or ``project_adv_vel`` it stays bounded over 30
+days.
+
 The 24-member uneven ensemble (2, 1 and 0.75 degrees, dt = 1200 s, 40
 substeps, per-member kappa_v draw) comes from ``ensemble_members``.
 
@@ -122,7 +128,8 @@ _SPECS = {
     "g4deg": dict(seed_offset=147, nx=90, ny=40, nz=12, lat=78.0, H=4000.0, dt=600.0, nsub=30, steps=8, restore=False, seasonal=False),
     "g1deg": dict(seed_offset=148, nx=360, ny=160, nz=50, lat=80.0, H=5500.0, dt=1200.0, nsub=40, steps=72, restore=True, seasonal=False),
     "g05deg": dict(seed_offset=149, nx=720, ny=320, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True),
-    "g045deg": dict(seed_offset=151, nx=800, ny=356, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True),
+    "g045deg": dict(seed_offset=151, nx=800, ny=356, nz=50, lat=80.0, H=5500.0, dt=900.0, nsub=45, steps=2880, restore=True, seasonal=True,
+                    monotone=True),
 }
 
 
@@ -152,6 +159,11 @@ def make_case(name, *, nx=None, ny=None, nz=None, seed=FIELD_SEED):
     kwargs = dict(forcing=case.wind(0.0), polar_cap_rows=ncap, polar_cap_taper=ntaper,
                   mode_split=True, dt_bt=dt / spec["nsub"], use_scan=True,
                   T_init=T, S_init=S)
+    if spec.get("monotone"):
+        # the reference's monotone (limited) tracer advection option: with the
+        # centred scheme a 72 m deep coastal corner cell of this bathymetry
+        # (i = 441, j = 187) gains heat without bound after about a week
+        kwargs["monotone_adv"] = True
     if spec["restore"]:
         lat = np.radians(grid.lat)[None, :]
         sst = -1.0 + 28.0 * np.cos(lat) ** 2 + rng.normal(0.0, 0.2, (nx, ny))

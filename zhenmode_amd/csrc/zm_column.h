@@ -15,8 +15,10 @@ namespace zm {
 struct MixParams {
   double dt, kappa_v, kappa_conv;
   double T_ref, S_ref;
-  const double* dzn;  // dz_node [nz], float64
-  const double* dzi;  // dz_iface [nz-1], float64
+  const double* dzn;   // dz_node [nz], float64
+  const double* dzi;   // dz_iface [nz-1], float64
+  const double* idzn;  // 1 / dz_node [nz]
+  const double* idzi;  // 1 / dz_iface [nz-1]
 };
 
 // work: 3 nz n2 float64 values
@@ -43,12 +45,32 @@ void launch_col_features(const Geo<real>& g, const MixParams& m, const real* T, 
                          const real* tau_x, const real* tau_y, const float* mean, const float* inv_std,
                          void* X, long long rows, hipStream_t s);
 
-// T_new = T + dt * (y * out_std + out_mean) with the column-integral
+// X_new = X + e + s (y * out_std + out_mean) per tracer (e: the explicit
+// mixing increment over dt, s = max_k |e_k|) with the column-integral
 // correction; columns whose implicit-system residual exceeds tol get flag = 1.
 template <typename real>
 void launch_mlp_apply(const Geo<real>& g, const MixParams& m, const real* T, const real* S, const float* Y,
                       int ldy, const float* out_mean, const float* out_std, double tol_T, double tol_S, real* To,
                       real* So, int* flag, int* count, hipStream_t s);
+
+// Training targets of the surrogate: per column and tracer, the Thomas
+// solution (Tt, St) minus the state and minus the explicit mixing increment,
+// in units of the column's mixing scale s = max_k |e_k| (0 where s = 0 and on
+// dry levels); float32 rows of ldy (nz T values, nz S values, zero padding).
+template <typename real>
+void launch_mix_targets(const Geo<real>& g, const MixParams& m, const real* T, const real* S, const real* Tt,
+                        const real* St, float* Y, int ldy, hipStream_t s);
+
+// The surrogate step on a float32 state in one launch: features, the four
+// layers (weights[l]: bf16 [N][256], N = 256, 256, 256, 128; biases[l] fp32
+// [N]) and the apply step of launch_mlp_apply; results equal the unfused
+// sequence bit for bit.  nz <= 50.
+void launch_mlp_fused(const Geo<float>& g, const MixParams& m, const float* T, const float* S, const float* u,
+                      const float* v, const float* q_heat, const float* s_ref2, float restore_coef_S,
+                      const float* tau_x, const float* tau_y, const float* in_mean, const float* in_inv_std,
+                      const void* const* weights, const float* const* biases, const float* out_mean,
+                      const float* out_std, double tol_T, double tol_S, float* To, float* So, int* flag, int* count,
+                      hipStream_t s);
 
 // C[m][n] = sum_k A[m][k] B[n][k], A (M x K) and B (N x K) bf16 row-major,
 // fp32 accumulation on MFMA; M, N multiples of 128, K a multiple of 32.

@@ -23,6 +23,7 @@ namespace zm {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double rho_anom(double T, double S, const MixParams& m) {
   return kRho0 * (-kAlpha * (T - m.T_ref) + kBeta * (S - m.S_ref));
@@ -100,12 +101,75 @@ __global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const re
 }
 
 // ---------------------------------------------------------------------------
-// Feature rows of the surrogate: a workgroup builds the rows of 64 columns in
-// LDS (4 waves: T; S; u, v; N^2, surface fluxes and padding) and writes them
-// as one contiguous block of 64 x 512 bytes.
+// Feature rows of the surrogate.  A column's 256 features are built by four
+// threads (parts): T; S; u, v; N^2, surface fluxes and padding, each value
+// normalised as (x - mean[f]) * inv_std[f] and rounded to bfloat16.  The same
+// routine feeds the standalone feature kernel and the fused surrogate kernel,
+// so both produce the same rows bit for bit.  Part 0 and 1 can also stage the
+// raw T and S of every level (stride `sld`) for a later apply phase.
 // ---------------------------------------------------------------------------
 constexpr int kFeat = 256;
-constexpr int kFeatRow = kFeat + 2;  // bf16 per LDS row: 129 words, conflict-free row writes
+constexpr int kFeatRow = kFeat + 2;  // bf16 per LDS row of the feature kernel: 129 words, conflict-free row writes
+
+// One normalised feature value, rounded to bfloat16 (raw bits).
+__device__ __forceinline__ unsigned short feat_bits(float x, int f, const float* __restrict__ mean,
+                                                    const float* __restrict__ inv_std) {
+  const __bf16 b = (__bf16)((x - mean[f]) * inv_std[f]);
+  return __builtin_bit_cast(unsigned short, b);
+}
+
+// N^2 at the lower interface of level k from the (T, S) of levels k and k + 1.
+__device__ __forceinline__ float n2_value(double Tk, double Sk, double Tk1, double Sk1, int k, const MixParams& m) {
+  return (float)(kG / kRho0 * (rho_anom(Tk1, Sk1, m) - rho_anom(Tk, Sk, m)) / m.dzi[k]);
+}
+
+template <typename real>
+__device__ __forceinline__ void feature_part(const Geo<real>& g, const MixParams& m, const real* __restrict__ T,
+                                             const real* __restrict__ S, const real* __restrict__ u,
+                                             const real* __restrict__ v, const real* __restrict__ q_heat,
+                                             const real* __restrict__ s_ref2, real restore_coef_S,
+                                             const real* __restrict__ tau_x, const real* __restrict__ tau_y,
+                                             const float* __restrict__ mean, const float* __restrict__ inv_std,
+                                             long long c2, int part, unsigned short* row, real* stT, real* stS,
+                                             int sld) {
+  const int nz = g.nz;
+  const long long n2 = g.n2;
+  auto put = [&](int f, float x) { row[f] = feat_bits(x, f, mean, inv_std); };
+  const int nw = g.nwet[c2];
+  if (part == 0) {
+    for (int k = 0; k < nz; ++k) {
+      const real t = T[k * n2 + c2];
+      if (stT) stT[k * sld] = t;
+      put(k, k < nw ? (float)t : 0.f);
+    }
+  } else if (part == 1) {
+    for (int k = 0; k < nz; ++k) {
+      const real s = S[k * n2 + c2];
+      if (stS) stS[k * sld] = s;
+      put(nz + k, k < nw ? (float)s : 0.f);
+    }
+  } else if (part == 2) {
+    for (int k = 0; k < nz; ++k) put(2 * nz + k, k < nw ? (float)u[k * n2 + c2] : 0.f);
+    for (int k = 0; k < nz; ++k) put(3 * nz + k, k < nw ? (float)v[k * n2 + c2] : 0.f);
+  } else {
+    double Tk = nw > 0 ? (double)T[c2] : 0.0, Sk = nw > 0 ? (double)S[c2] : 0.0;
+    for (int k = 0; k < nz; ++k) {
+      float n2k = 0.f;
+      if (k + 1 < nw) {
+        const double Tk1 = (double)T[(k + 1) * n2 + c2], Sk1 = (double)S[(k + 1) * n2 + c2];
+        n2k = n2_value(Tk, Sk, Tk1, Sk1, k, m);
+        Tk = Tk1;
+        Sk = Sk1;
+      }
+      put(4 * nz + k, n2k);
+    }
+    const bool wet = nw > 0;
+    put(5 * nz, wet ? (float)q_heat[c2] : 0.f);
+    put(5 * nz + 1, wet ? (float)(restore_coef_S * (s_ref2[c2] - S[c2])) : 0.f);
+    put(5 * nz + 2, wet ? sqrtf((float)tau_x[c2] * (float)tau_x[c2] + (float)tau_y[c2] * (float)tau_y[c2]) : 0.f);
+    for (int f = 5 * nz + 3; f < kFeat; ++f) put(f, 0.f);
+  }
+}
 
 template <typename real>
 __global__ __launch_bounds__(256) void k_col_features(Geo<real> g, MixParams m, const real* __restrict__ T,
@@ -120,39 +184,10 @@ __global__ __launch_bounds__(256) void k_col_features(Geo<real> g, MixParams m, 
   const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
   const long long r0 = (long long)blockIdx.x * 64;
   const long long c2 = r0 + lane;
-  const int nz = g.nz;
-  const long long n2 = g.n2;
   unsigned short* row = rowbuf + lane * kFeatRow;
-  auto put = [&](int f, float x) {
-    const __bf16 b = (__bf16)((x - mean[f]) * inv_std[f]);
-    row[f] = __builtin_bit_cast(unsigned short, b);
-  };
   if (c2 < g.n2) {
-    const int nw = g.nwet[c2];
-    if (part == 0) {
-      for (int k = 0; k < nz; ++k) put(k, k < nw ? (float)T[k * n2 + c2] : 0.f);
-    } else if (part == 1) {
-      for (int k = 0; k < nz; ++k) put(nz + k, k < nw ? (float)S[k * n2 + c2] : 0.f);
-    } else if (part == 2) {
-      for (int k = 0; k < nz; ++k) put(2 * nz + k, k < nw ? (float)u[k * n2 + c2] : 0.f);
-      for (int k = 0; k < nz; ++k) put(3 * nz + k, k < nw ? (float)v[k * n2 + c2] : 0.f);
-    } else {
-      double rc = nw > 0 ? rho_anom((double)T[c2], (double)S[c2], m) : 0.0;
-      for (int k = 0; k < nz; ++k) {
-        float n2k = 0.f;
-        if (k + 1 < nw) {
-          const double rn = rho_anom((double)T[(k + 1) * n2 + c2], (double)S[(k + 1) * n2 + c2], m);
-          n2k = (float)(kG / kRho0 * (rn - rc) / m.dzi[k]);
-          rc = rn;
-        }
-        put(4 * nz + k, n2k);
-      }
-      const bool wet = nw > 0;
-      put(5 * nz, wet ? (float)q_heat[c2] : 0.f);
-      put(5 * nz + 1, wet ? (float)(restore_coef_S * (s_ref2[c2] - S[c2])) : 0.f);
-      put(5 * nz + 2, wet ? sqrtf((float)tau_x[c2] * (float)tau_x[c2] + (float)tau_y[c2] * (float)tau_y[c2]) : 0.f);
-      for (int f = 5 * nz + 3; f < kFeat; ++f) put(f, 0.f);
-    }
+    feature_part(g, m, T, S, u, v, q_heat, s_ref2, restore_coef_S, tau_x, tau_y, mean, inv_std, c2, part, row,
+                 (real*)nullptr, (real*)nullptr, 0);
   } else if (part == 0) {
     for (int f = 0; f < kFeat; ++f) row[f] = 0;
   }
@@ -166,18 +201,227 @@ __global__ __launch_bounds__(256) void k_col_features(Geo<real> g, MixParams m, 
 }
 
 // ---------------------------------------------------------------------------
-// Apply the surrogate's increments: dX_k = dt (y_k out_std_k + out_mean_k) on
-// wet levels, shifted uniformly so that sum_k dz_node dX_k = 0 in float64 (the
-// column's heat and salt content is conserved exactly before the final
-// rounding to the state dtype), X_new = X + dX.  The residual of the implicit
-// system, r = X_new - dt D(X_new) - X, bounds |X_new - X_thomas| (the matrix is
-// an M-matrix with unit row sums, so ||A^-1||_inf <= 1): a column with
-// max|r_T| > tol_T or max|r_S| > tol_S is flagged for the Thomas fallback.
-// A workgroup stages the Y rows and T, S columns of 64 columns in LDS with
-// coalesced loads (4 waves); one wave then walks the columns.
+// The surrogate's increments and the residual check.
+//
+// For tracer X of a column with nw wet levels, the explicit increment of the
+// mixing operator over dt is
+//   e_k = dt (-f_m (X_k - X_{k-1}) + f_p (X_{k+1} - X_k)) / dz_node[k],
+//   f_m = kappa_{k-1/2} / dz_iface[k-1], f_p = kappa_{k+1/2} / dz_iface[k]
+// (zero-flux ends), and s = max_k |e_k| is the column's mixing scale.  The
+// network predicts the implicit solution's departure from the explicit step in
+// units of s:  dX_k = e_k + s z_k,  z_k = y_k out_std_k + out_mean_k.
+// Stable columns (kappa = kappa_v everywhere, dt kappa / dz^2 ~ 1e-4) have a
+// scale s orders of magnitude below a convectively unstable column's, so the
+// network's error is scaled down with the increment it corrects.  dX is
+// shifted uniformly over the wet levels so that sum_k dz_node dX_k = 0 in
+// float64 (the explicit part telescopes; the shift removes the network's
+// column integral): heat and salt are conserved before the final rounding.
+// The residual of the implicit system, r = X_new - dt D(X_new) - X (on the
+// rounded X_new), bounds |X_new - X_thomas| (the matrix is an M-matrix with
+// unit row sums, so ||A^-1||_inf <= 1): a column with max|r_T| > tol_T or
+// max|r_S| > tol_S is flagged for the Thomas fallback.
+//
+// Work split: four threads per column, each owning a contiguous quarter of the
+// wet levels (kApplyChunks); the column sums and maxima are combined from the
+// four partials in chunk order, so every kernel that uses this routine gets
+// the same bits.  Called by every thread of the block (it has barriers);
+// blockDim.x = kApplyChunks x (columns per block).
 // ---------------------------------------------------------------------------
 constexpr int kApplyMaxNz = 50;
+constexpr int kApplyChunks = 4;
 
+__device__ __forceinline__ void apply_chunk(int nw, int c, int& lo, int& hi) {
+  const int q = (nw + kApplyChunks - 1) / kApplyChunks;
+  lo = min(nw, c * q);
+  hi = min(nw, lo + q);
+}
+
+// Walks the staged column from level k0 down, yielding the explicit increments.
+template <typename real>
+struct MixWalk {
+  const MixParams& m;
+  const real* tc;
+  const real* sc;
+  int ld, nw, k;
+  double Tm, Sm, T0, S0, km;
+  __device__ MixWalk(const MixParams& m_, const real* t, const real* s, int ld_, int nw_, int k0)
+      : m(m_), tc(t), sc(s), ld(ld_), nw(nw_), k(k0) {
+    T0 = (double)tc[k0 * ld];
+    S0 = (double)sc[k0 * ld];
+    Tm = T0;
+    Sm = S0;
+    km = 0.0;
+    if (k0 > 0) {
+      Tm = (double)tc[(k0 - 1) * ld];
+      Sm = (double)sc[(k0 - 1) * ld];
+      km = mix_kappa(Tm, Sm, T0, S0, m);
+    }
+  }
+  // increments of level k (then k advances); Tk, Sk = the level's values, kp = kappa_{k+1/2}
+  __device__ void next(double& eT, double& eS, double& Tk, double& Sk, double& kmo, double& kpo) {
+    double T1 = T0, S1 = S0, kp = 0.0;
+    if (k + 1 < nw) {
+      T1 = (double)tc[(k + 1) * ld];
+      S1 = (double)sc[(k + 1) * ld];
+      kp = mix_kappa(T0, S0, T1, S1, m);
+    }
+    const double fm = k > 0 ? km * m.idzi[k - 1] : 0.0;
+    const double fp = k + 1 < nw ? kp * m.idzi[k] : 0.0;
+    const double w = m.dt * m.idzn[k];
+    eT = w * (-fm * (T0 - Tm) + fp * (T1 - T0));
+    eS = w * (-fm * (S0 - Sm) + fp * (S1 - S0));
+    Tk = T0;
+    Sk = S0;
+    kmo = km;
+    kpo = kp;
+    Tm = T0;
+    Sm = S0;
+    T0 = T1;
+    S0 = S1;
+    km = kp;
+    ++k;
+  }
+};
+
+// Copies the per-level tables the apply step reads at every level into LDS:
+// tab = dz_node, 1/dz_node, 1/dz_iface (nz each, float64), otab = out_mean,
+// out_std (2 nz each).  Followed by a barrier in the caller.
+__device__ __forceinline__ void stage_apply_tables(const MixParams& m, int nz, const float* __restrict__ omean,
+                                                   const float* __restrict__ ostd, double* tab, float* otab) {
+  for (int q = threadIdx.x; q < nz; q += blockDim.x) {
+    tab[q] = m.dzn[q];
+    tab[nz + q] = m.idzn[q];
+    tab[2 * nz + q] = q + 1 < nz ? m.idzi[q] : 0.0;
+  }
+  for (int q = threadIdx.x; q < 2 * nz; q += blockDim.x) {
+    otab[q] = omean[q];
+    otab[2 * nz + q] = ostd[q];
+  }
+}
+
+// red: LDS scratch of 8 x blockDim.x doubles.  ts, ss: staged T, S of the
+// block's columns, level stride ld (column l at ts[k * ld + l]); y: the
+// columns' network outputs (row stride yld, nz T values then nz S values),
+// overwritten with the new wet-level values; tab, otab: the staged tables of
+// stage_apply_tables.
+template <typename real>
+__device__ void learned_apply_block(const Geo<real>& g, const MixParams& mg, long long c0, int ncol, const real* ts,
+                                    const real* ss, int ld, real* y, int yld, const double* tab, const float* otab,
+                                    double tol_T, double tol_S, real* __restrict__ To, real* __restrict__ So,
+                                    int* __restrict__ flag, int* __restrict__ count, double* red) {
+  const int nz = g.nz;
+  MixParams m = mg;
+  m.dzn = tab;
+  m.idzn = tab + nz;
+  m.idzi = tab + 2 * nz;
+  const float* omean = otab;
+  const float* ostd = otab + 2 * nz;
+  const long long n2 = g.n2;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int col = t / kApplyChunks, c = t - col * kApplyChunks;
+  const bool live = col < ncol;
+  const long long c2 = c0 + col;
+  const int nw = live ? g.nwet[c2] : 0;
+  int lo, hi;
+  apply_chunk(nw, c, lo, hi);
+  const real* tc = ts + col;
+  const real* sc = ss + col;
+  real* yr = y + (long long)col * yld;
+  // pass 1: partial sums and maxima of this chunk
+  double sT = 0.0, sS = 0.0, eTs = 0.0, eSs = 0.0, zTs = 0.0, zSs = 0.0, H = 0.0;
+  if (lo < hi) {
+    MixWalk<real> w(m, tc, sc, ld, nw, lo);
+    for (int k = lo; k < hi; ++k) {
+      double eT, eS, Tk, Sk, kmo, kpo;
+      w.next(eT, eS, Tk, Sk, kmo, kpo);
+      const double zT = (double)yr[k] * (double)ostd[k] + (double)omean[k];
+      const double zS = (double)yr[nz + k] * (double)ostd[nz + k] + (double)omean[nz + k];
+      sT = fmax(sT, fabs(eT));
+      sS = fmax(sS, fabs(eS));
+      eTs += m.dzn[k] * eT;
+      eSs += m.dzn[k] * eS;
+      zTs += m.dzn[k] * zT;
+      zSs += m.dzn[k] * zS;
+      H += m.dzn[k];
+    }
+  }
+  red[0 * nt + t] = sT;
+  red[1 * nt + t] = sS;
+  red[2 * nt + t] = eTs;
+  red[3 * nt + t] = eSs;
+  red[4 * nt + t] = zTs;
+  red[5 * nt + t] = zSs;
+  red[6 * nt + t] = H;
+  __syncthreads();
+  const int b = col * kApplyChunks;
+  double tot[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    double a = red[q * nt + b];
+#pragma unroll
+    for (int r = 1; r < kApplyChunks; ++r) a = q < 2 ? fmax(a, red[q * nt + b + r]) : a + red[q * nt + b + r];
+    tot[q] = a;
+  }
+  const double STc = tot[0], SSc = tot[1];
+  const double shT = nw > 0 ? (tot[2] + STc * tot[4]) / tot[6] : 0.0;
+  const double shS = nw > 0 ? (tot[3] + SSc * tot[5]) / tot[6] : 0.0;
+  // pass 2: new values of this chunk's levels into y
+  if (lo < hi) {
+    MixWalk<real> w(m, tc, sc, ld, nw, lo);
+    for (int k = lo; k < hi; ++k) {
+      double eT, eS, Tk, Sk, kmo, kpo;
+      w.next(eT, eS, Tk, Sk, kmo, kpo);
+      const double zT = (double)yr[k] * (double)ostd[k] + (double)omean[k];
+      const double zS = (double)yr[nz + k] * (double)ostd[nz + k] + (double)omean[nz + k];
+      yr[k] = (real)(Tk + ((eT + STc * zT) - shT));
+      yr[nz + k] = (real)(Sk + ((eS + SSc * zS) - shS));
+    }
+  }
+  __syncthreads();
+  // pass 3: residual of the implicit system on the rounded new values, outputs
+  double rT = 0.0, rS = 0.0;
+  if (lo < hi) {
+    MixWalk<real> w(m, tc, sc, ld, nw, lo);
+    double xTm = lo > 0 ? (double)yr[lo - 1] : 0.0, xSm = lo > 0 ? (double)yr[nz + lo - 1] : 0.0;
+    double xT = (double)yr[lo], xS = (double)yr[nz + lo];
+    for (int k = lo; k < hi; ++k) {
+      double eT, eS, Tk, Sk, kmo, kpo;
+      w.next(eT, eS, Tk, Sk, kmo, kpo);
+      const double xTp = k + 1 < nw ? (double)yr[k + 1] : 0.0, xSp = k + 1 < nw ? (double)yr[nz + k + 1] : 0.0;
+      const double fm = k > 0 ? kmo * m.idzi[k - 1] : 0.0;
+      const double fp = k + 1 < nw ? kpo * m.idzi[k] : 0.0;
+      const double w = m.dt * m.idzn[k];
+      const double qT = -fm * (xT - xTm) + fp * (xTp - xT);
+      const double qS = -fm * (xS - xSm) + fp * (xSp - xS);
+      rT = fmax(rT, fabs(xT - w * qT - Tk));
+      rS = fmax(rS, fabs(xS - w * qS - Sk));
+      To[k * n2 + c2] = (real)xT;
+      So[k * n2 + c2] = (real)xS;
+      xTm = xT;
+      xSm = xS;
+      xT = xTp;
+      xS = xSp;
+    }
+  }
+  if (live) {
+    for (int k = nw + c; k < nz; k += kApplyChunks) {
+      To[k * n2 + c2] = tc[k * ld];
+      So[k * n2 + c2] = sc[k * ld];
+    }
+  }
+  __syncthreads();
+  red[t] = (rT > tol_T || rS > tol_S) ? 1.0 : 0.0;
+  __syncthreads();
+  if (live && c == 0) {
+    int bad = 0;
+    for (int r = 0; r < kApplyChunks; ++r) bad |= red[b + r] != 0.0;
+    flag[c2] = bad;
+    if (bad) atomicAdd(count, 1);
+  }
+}
+
+// Standalone apply: a workgroup of 256 threads stages the Y rows and T, S
+// columns of 64 columns in LDS with coalesced loads, then applies them.
 template <typename real>
 __global__ __launch_bounds__(256) void k_mlp_apply(Geo<real> g, MixParams m, const real* __restrict__ T,
                                                    const real* __restrict__ S, const float* __restrict__ Y, int ldy,
@@ -186,15 +430,19 @@ __global__ __launch_bounds__(256) void k_mlp_apply(Geo<real> g, MixParams m, con
                                                    real* __restrict__ So, int* __restrict__ flag,
                                                    int* __restrict__ count) {
   constexpr int YW = 2 * kApplyMaxNz + 1;
-  __shared__ float ys[64 * YW];
+  __shared__ real ys[64 * YW];
   __shared__ real ts[kApplyMaxNz * 64], ss[kApplyMaxNz * 64];
+  __shared__ double red[8 * 256];
+  __shared__ double tab[3 * kApplyMaxNz];
+  __shared__ float otab[4 * kApplyMaxNz];
   const int nz = g.nz;
   const long long n2 = g.n2;
   const long long r0 = (long long)blockIdx.x * 64;
   const int ncol = (int)(n2 - r0 < 64 ? n2 - r0 : 64);
+  stage_apply_tables(m, nz, omean, ostd, tab, otab);
   for (int q = threadIdx.x; q < ncol * 2 * nz; q += 256) {
     const int rr = q / (2 * nz), k = q - rr * 2 * nz;
-    ys[rr * YW + k] = Y[(r0 + rr) * ldy + k];
+    ys[rr * YW + k] = (real)Y[(r0 + rr) * ldy + k];
   }
   for (int q = threadIdx.x; q < nz * 64; q += 256) {
     const int k = q >> 6, l = q & 63;
@@ -204,59 +452,43 @@ __global__ __launch_bounds__(256) void k_mlp_apply(Geo<real> g, MixParams m, con
     }
   }
   __syncthreads();
-  if (threadIdx.x >= 64 || (int)threadIdx.x >= ncol) return;
-  const int l = threadIdx.x;
-  const long long c2 = r0 + l;
-  const int nw = g.nwet[c2];
-  for (int k = nw; k < nz; ++k) {
-    To[k * n2 + c2] = ts[k * 64 + l];
-    So[k * n2 + c2] = ss[k * 64 + l];
-  }
-  if (nw == 0) {
-    flag[c2] = 0;
-    return;
-  }
-  const float* y = ys + l * YW;
-  auto incT = [&](int k) { return m.dt * ((double)y[k] * (double)ostd[k] + (double)omean[k]); };
-  auto incS = [&](int k) { return m.dt * ((double)y[nz + k] * (double)ostd[nz + k] + (double)omean[nz + k]); };
-  double sT = 0.0, sS = 0.0, H = 0.0;
-  for (int k = 0; k < nw; ++k) {
-    sT += m.dzn[k] * incT(k);
-    sS += m.dzn[k] * incS(k);
-    H += m.dzn[k];
-  }
-  const double shT = sT / H, shS = sS / H;
-  double T0 = ts[l], S0 = ss[l];
-  double xT = T0 + (incT(0) - shT), xS = S0 + (incS(0) - shS);
-  double xTm = 0.0, xSm = 0.0, km = 0.0, rT = 0.0, rS = 0.0;
-  for (int k = 0; k < nw; ++k) {
-    double T1 = 0.0, S1 = 0.0, xTp = 0.0, xSp = 0.0, kp = 0.0;
-    if (k + 1 < nw) {
-      T1 = ts[(k + 1) * 64 + l];
-      S1 = ss[(k + 1) * 64 + l];
-      xTp = T1 + (incT(k + 1) - shT);
-      xSp = S1 + (incS(k + 1) - shS);
-      kp = mix_kappa(T0, S0, T1, S1, m);
+  learned_apply_block(g, m, r0, ncol, ts, ss, 64, ys, YW, tab, otab, tol_T, tol_S, To, So, flag, count, red);
+}
+
+// ---------------------------------------------------------------------------
+// Training targets of the surrogate: for every column, the Thomas solution's
+// departure from the explicit step in units of the column's mixing scale,
+// (X_thomas_k - X_k - e_k) / s (0 where s = 0 and on dry levels), as float32
+// rows Y[c][0..ldy) (nz T values, nz S values, zero padding).
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ __launch_bounds__(256) void k_mix_targets(Geo<real> g, MixParams m, const real* __restrict__ T,
+                                                     const real* __restrict__ S, const real* __restrict__ Tt,
+                                                     const real* __restrict__ St, float* __restrict__ Y, int ldy) {
+  const long long c2 = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c2 >= g.n2) return;
+  const int nz = g.nz, nw = g.nwet[c2];
+  const long long n2 = g.n2;
+  float* yr = Y + c2 * ldy;
+  for (int n = 0; n < ldy; ++n) yr[n] = 0.f;
+  if (nw == 0) return;
+  double sT = 0.0, sS = 0.0;
+  {
+    MixWalk<real> w(m, T + c2, S + c2, (int)n2, nw, 0);
+    for (int k = 0; k < nw; ++k) {
+      double eT, eS, Tk, Sk, kmo, kpo;
+      w.next(eT, eS, Tk, Sk, kmo, kpo);
+      sT = fmax(sT, fabs(eT));
+      sS = fmax(sS, fabs(eS));
     }
-    const double fm = k > 0 ? km / m.dzi[k - 1] : 0.0;
-    const double fp = k + 1 < nw ? kp / m.dzi[k] : 0.0;
-    const double qT = -fm * (xT - xTm) + fp * (xTp - xT);
-    const double qS = -fm * (xS - xSm) + fp * (xSp - xS);
-    rT = fmax(rT, fabs(xT - m.dt * qT / m.dzn[k] - T0));
-    rS = fmax(rS, fabs(xS - m.dt * qS / m.dzn[k] - S0));
-    To[k * n2 + c2] = (real)xT;
-    So[k * n2 + c2] = (real)xS;
-    xTm = xT;
-    xSm = xS;
-    xT = xTp;
-    xS = xSp;
-    T0 = T1;
-    S0 = S1;
-    km = kp;
   }
-  const int bad = (rT > tol_T || rS > tol_S) ? 1 : 0;
-  flag[c2] = bad;
-  if (bad) atomicAdd(count, 1);
+  MixWalk<real> w(m, T + c2, S + c2, (int)n2, nw, 0);
+  for (int k = 0; k < nw; ++k) {
+    double eT, eS, Tk, Sk, kmo, kpo;
+    w.next(eT, eS, Tk, Sk, kmo, kpo);
+    yr[k] = sT > 0.0 ? (float)(((double)Tt[k * n2 + c2] - Tk - eT) / sT) : 0.f;
+    yr[nz + k] = sS > 0.0 ? (float)(((double)St[k * n2 + c2] - Sk - eS) / sS) : 0.f;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -268,10 +500,24 @@ __global__ __launch_bounds__(256) void k_mlp_apply(Geo<real> g, MixParams m, con
 // ---------------------------------------------------------------------------
 constexpr int kGB = 128, kGK = 32, kGPad = 40;
 
-__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.7071067811865476f)); }
+// erf by Abramowitz & Stegun 7.1.26 (Handbook of Mathematical Functions,
+// 1964): erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2),
+// t = 1 / (1 + p x) for x >= 0, odd symmetry for x < 0; absolute error below
+// 1.5e-7 (float32 rounding adds a few ulp), far below the bfloat16 rounding of
+// the activations it feeds.  Branch-free: one reciprocal, one exponential.
+__device__ __forceinline__ float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = __frcp_rn(1.0f + 0.3275911f * ax);
+  const float poly =
+      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float y = 1.0f - poly * __expf(-ax * ax);
+  return copysignf(y, x);
+}
+
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.7071067811865476f)); }
 
 __device__ __forceinline__ float gelu_grad_f(float z) {
-  return 0.5f * (1.0f + erff(z * 0.7071067811865476f)) + z * 0.3989422804014327f * expf(-0.5f * z * z);
+  return 0.5f * (1.0f + erf_as(z * 0.7071067811865476f)) + z * 0.3989422804014327f * __expf(-0.5f * z * z);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_bf16_nt(long long M, long long N, long long K,
@@ -351,6 +597,222 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_nt(long long M, long long N, 
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Fused surrogate step on a float32 state: one workgroup of 256 threads (four
+// waves) takes 64 columns through features -> 4 MFMA layers -> apply in one
+// launch, with the activations in LDS and no HBM round trip between stages.
+//
+// Weights stay in registers: wave w owns output columns [64 w, 64 w + 64) of
+// a 256-wide layer (32 of the last, 128-wide layer) and holds their bf16
+// B fragments for all eight 32-k slabs (32 x 16-byte fragments per lane).
+// As soon as a slab's MFMAs are issued, its registers are reloaded with the
+// same slab of the next layer, so each layer's weights stream in from L2
+// while the current layer computes (the first layer's while the features are
+// built); no LDS staging of weights and no barrier per K step.
+//
+// LDS (73,728 bytes: two workgroups per CU, so one workgroup's memory-bound
+// phases overlap the other's MFMA phase):
+//   A    64 x 264 bf16: the activation tile (rows padded by 8: 16-byte A
+//        fragment loads hit all 64 banks), overwritten in place by each
+//        layer's output, then by the float32 outputs Y (64 x 101) and the
+//        apply step's level tables;
+//   TS   the raw T and S of the 64 columns, staged by the feature phase and
+//        kept for the apply phase;
+//   R    the feature normalisation (256 means, 256 inverse stds) during the
+//        feature phase, then the apply step's reduction scratch.
+// Every output element accumulates its K slabs in the same order as
+// k_gemm_bf16_nt and takes the same epilogue, and the features and apply
+// compute the same values as the standalone kernels, so the result equals
+// features -> 4 GEMMs -> apply bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kFusedRows = 64;
+constexpr int kFusedThreads = 4 * kFusedRows;
+constexpr int kFusedArow = 264;  // bf16 per activation row
+constexpr int kFusedYld = 2 * kApplyMaxNz + 1;
+constexpr int kFusedA = kFusedRows * kFusedArow * 2;          // 33,792 bytes
+constexpr int kFusedTS = 2 * kApplyMaxNz * kFusedRows * 4;    // 25,600 bytes
+constexpr int kFusedR = 7 * kFusedThreads * 8;                // 14,336 bytes
+constexpr int kFusedLds = kFusedA + kFusedTS + kFusedR;
+constexpr int kFusedYbytes = (kFusedRows * kFusedYld * 4 + 255) / 256 * 256;  // outputs, then the level tables
+constexpr int kFusedLv = (kApplyMaxNz + 3) / 4;                // levels per thread in the feature phase
+static_assert(kFusedYbytes + 3 * kApplyMaxNz * 8 + 4 * kApplyMaxNz * 4 <= kFusedA,
+              "outputs and level tables must fit the activation tile");
+static_assert(2 * kFeat * 4 <= kFusedR, "feature normalisation must fit the reduction scratch");
+static_assert(kFusedLds <= 160 * 1024, "LDS of one workgroup");
+
+struct MlpWeights {
+  const __bf16* w[4];  // [N][K] row-major, K = 256; N = 256, 256, 256, 128
+  const float* b[4];
+};
+
+// B fragments of wave wc's output columns of a layer with N outputs, slab kk.
+template <int N>
+__device__ __forceinline__ void load_slab(const __bf16* __restrict__ W, int kk, int wc, int lane, u32x4 (&dst)[4]) {
+  constexpr int NT = N / 64;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+#pragma unroll
+  for (int b = 0; b < NT; ++b)
+    dst[b] = *reinterpret_cast<const u32x4*>(W + (long long)(wc * NT * 16 + b * 16 + fr) * 256 + kk * kGK + fk);
+}
+
+// One layer: acc = A W^T over the eight slabs held in wv; after each slab's
+// MFMAs its registers receive the next layer's slab (NN outputs) when Wn is
+// set.  Then bias + GELU (bf16, in place over A) or bias (float32 into Y).
+template <int N, int NN>
+__device__ __forceinline__ void mlp_layer(u32x4 (&wv)[8][4], const __bf16* __restrict__ Wn,
+                                          const float* __restrict__ bias, __bf16* A, float* Y, int nout) {
+  constexpr int NT = N / 64;  // 16-column tiles per wave
+  const int t = threadIdx.x, lane = t & 63, wc = t >> 6, r0w = 0;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  f32x4 acc[4][NT];
+  float bn[NT];
+#pragma unroll
+  for (int b = 0; b < NT; ++b) bn[b] = bias[wc * NT * 16 + b * 16 + fr];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < NT; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 256 / kGK; ++kk) {
+    bf16x8 af[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      af[a] = *reinterpret_cast<const bf16x8*>(A + (r0w + a * 16 + fr) * kFusedArow + kk * kGK + fk);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < NT; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], __builtin_bit_cast(bf16x8, wv[kk][b]), acc[a][b],
+                                                            0, 0, 0);
+    if constexpr (NN > 0) load_slab<NN>(Wn, kk, wc, lane, wv[kk]);
+    __builtin_amdgcn_sched_barrier(0);  // keep each slab's reload next to its MFMAs
+  }
+  __syncthreads();  // every wave has read the whole activation tile
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      const int n = wc * NT * 16 + b * 16 + fr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0w + a * 16 + 4 * (lane >> 4) + r;
+        if (Y) {
+          if (n < nout) Y[row * kFusedYld + n] = acc[a][b][r] + bn[b];
+        } else {
+          A[row * kFusedArow + n] = (__bf16)gelu_f(acc[a][b][r] + bn[b]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256, 2) void k_mlp_fused(Geo<float> g, MixParams m, const float* __restrict__ T,
+                                                      const float* __restrict__ S, const float* __restrict__ u,
+                                                      const float* __restrict__ v, const float* __restrict__ q_heat,
+                                                      const float* __restrict__ s_ref2, float restore_coef_S,
+                                                      const float* __restrict__ tau_x,
+                                                      const float* __restrict__ tau_y,
+                                                      const float* __restrict__ in_mean,
+                                                      const float* __restrict__ in_inv_std, MlpWeights wts,
+                                                      const float* __restrict__ omean,
+                                                      const float* __restrict__ ostd, double tol_T, double tol_S,
+                                                      float* __restrict__ To, float* __restrict__ So,
+                                                      int* __restrict__ flag, int* __restrict__ count) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[kFusedLds];
+  __bf16* A = reinterpret_cast<__bf16*>(lds);
+  unsigned short* Au = reinterpret_cast<unsigned short*>(lds);
+  float* tsT = reinterpret_cast<float*>(lds + kFusedA);
+  float* tsS = tsT + kApplyMaxNz * kFusedRows;
+  double* red = reinterpret_cast<double*>(lds + kFusedA + kFusedTS);
+  float* nmean = reinterpret_cast<float*>(red);
+  float* ninv = nmean + kFeat;
+  const int t = threadIdx.x, lane = t & 63, wc = (t >> 6) & 3, nz = g.nz;
+  const long long n2 = g.n2;
+  const long long r0 = (long long)blockIdx.x * kFusedRows;
+  const int ncol = (int)(n2 - r0 < kFusedRows ? n2 - r0 : kFusedRows);
+  // the first layer's weights stream in while the features are built
+  u32x4 wv[8][4];
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk) load_slab<256>(wts.w[0], kk, wc, lane, wv[kk]);
+  // features: thread t takes column l = t % 64 at levels k = t / 64 + 4 i,
+  // with every level's loads issued before any is used
+  const int l = t & (kFusedRows - 1), k0 = t / kFusedRows;
+  const bool live = l < ncol;
+  const long long c2 = r0 + l;
+  float tv[kFusedLv], sv[kFusedLv], uv[kFusedLv], vv[kFusedLv];
+#pragma unroll
+  for (int i = 0; i < kFusedLv; ++i) {
+    const int k = k0 + 4 * i;
+    if (live && k < nz) {
+      const long long o = k * n2 + c2;
+      tv[i] = T[o];
+      sv[i] = S[o];
+      uv[i] = u[o];
+      vv[i] = v[o];
+    }
+  }
+  if (t < kFeat) {
+    nmean[t] = in_mean[t];
+    ninv[t] = in_inv_std[t];
+  }
+  const int nw = live ? g.nwet[c2] : 0;
+  __syncthreads();
+  auto bits = [&](float x, int f) {
+    const __bf16 b = (__bf16)((x - nmean[f]) * ninv[f]);
+    return __builtin_bit_cast(unsigned short, b);
+  };
+  unsigned short* row = Au + l * kFusedArow;
+  if (live) {
+#pragma unroll
+    for (int i = 0; i < kFusedLv; ++i) {
+      const int k = k0 + 4 * i;
+      if (k < nz) {
+        const bool wet = k < nw;
+        tsT[k * kFusedRows + l] = tv[i];
+        tsS[k * kFusedRows + l] = sv[i];
+        row[k] = bits(wet ? tv[i] : 0.f, k);
+        row[nz + k] = bits(wet ? sv[i] : 0.f, nz + k);
+        row[2 * nz + k] = bits(wet ? uv[i] : 0.f, 2 * nz + k);
+        row[3 * nz + k] = bits(wet ? vv[i] : 0.f, 3 * nz + k);
+      }
+    }
+  } else {
+    for (int f = k0; f < kFeat; f += 4) row[f] = 0;
+  }
+  __syncthreads();
+  if (live) {
+    for (int k = k0; k < nz; k += 4) {
+      float n2k = 0.f;
+      if (k + 1 < nw)
+        n2k = n2_value((double)tsT[k * kFusedRows + l], (double)tsS[k * kFusedRows + l],
+                       (double)tsT[(k + 1) * kFusedRows + l], (double)tsS[(k + 1) * kFusedRows + l], k, m);
+      row[4 * nz + k] = bits(n2k, 4 * nz + k);
+    }
+    // surface fluxes and padding (features 5 nz .. 255)
+    for (int f = 5 * nz + k0; f < kFeat; f += 4) {
+      const bool wet = nw > 0;
+      float x = 0.f;
+      if (wet && f == 5 * nz) x = q_heat[c2];
+      if (wet && f == 5 * nz + 1) x = restore_coef_S * (s_ref2[c2] - tsS[l]);
+      if (wet && f == 5 * nz + 2) x = sqrtf(tau_x[c2] * tau_x[c2] + tau_y[c2] * tau_y[c2]);
+      row[f] = bits(x, f);
+    }
+  }
+  __syncthreads();
+  mlp_layer<256, 256>(wv, wts.w[1], wts.b[0], A, nullptr, 0);
+  mlp_layer<256, 256>(wv, wts.w[2], wts.b[1], A, nullptr, 0);
+  mlp_layer<256, 128>(wv, wts.w[3], wts.b[2], A, nullptr, 0);
+  float* Y = reinterpret_cast<float*>(lds);
+  mlp_layer<128, 0>(wv, nullptr, wts.b[3], A, Y, 2 * nz);
+  double* tab = reinterpret_cast<double*>(lds + kFusedYbytes);
+  float* otab = reinterpret_cast<float*>(tab + 3 * kApplyMaxNz);
+  stage_apply_tables(m, nz, omean, ostd, tab, otab);
+  __syncthreads();
+  learned_apply_block(g, m, r0, ncol, tsT, tsS, kFusedRows, Y, kFusedYld, tab, otab, tol_T, tol_S, To, So, flag,
+                      count, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,6 +985,29 @@ void launch_mlp_apply(const Geo<real>& g, const MixParams& m, const real* T, con
                      tol_S, To, So, flag, count);
 }
 
+template <typename real>
+void launch_mix_targets(const Geo<real>& g, const MixParams& m, const real* T, const real* S, const real* Tt,
+                        const real* St, float* Y, int ldy, hipStream_t s) {
+  const unsigned blocks = (unsigned)((g.n2 + 255) / 256);
+  hipLaunchKernelGGL(k_mix_targets<real>, dim3(blocks), dim3(256), 0, s, g, m, T, S, Tt, St, Y, ldy);
+}
+
+void launch_mlp_fused(const Geo<float>& g, const MixParams& m, const float* T, const float* S, const float* u,
+                      const float* v, const float* q_heat, const float* s_ref2, float restore_coef_S,
+                      const float* tau_x, const float* tau_y, const float* in_mean, const float* in_inv_std,
+                      const void* const* weights, const float* const* biases, const float* out_mean,
+                      const float* out_std, double tol_T, double tol_S, float* To, float* So, int* flag, int* count,
+                      hipStream_t s) {
+  MlpWeights w{};
+  for (int l = 0; l < 4; ++l) {
+    w.w[l] = static_cast<const __bf16*>(weights[l]);
+    w.b[l] = biases[l];
+  }
+  const unsigned blocks = (unsigned)((g.n2 + kFusedRows - 1) / kFusedRows);
+  hipLaunchKernelGGL(k_mlp_fused, dim3(blocks), dim3(kFusedThreads), 0, s, g, m, T, S, u, v, q_heat, s_ref2, restore_coef_S,
+                     tau_x, tau_y, in_mean, in_inv_std, w, out_mean, out_std, tol_T, tol_S, To, So, flag, count);
+}
+
 void launch_gemm_bf16_nt(long long M, long long N, long long K, const void* A, const void* B, int epi,
                          const float* bias, const float* aux, float* C_f32, void* C_bf16, void* CT_bf16,
                          hipStream_t s) {
@@ -566,7 +1051,9 @@ void launch_adam(int n_out, int n_in, float* w, const float* gw, float* m, float
                                           long long, hipStream_t);                                            \
   template void launch_mlp_apply<real>(const Geo<real>&, const MixParams&, const real*, const real*,          \
                                        const float*, int, const float*, const float*, double, double, real*,  \
-                                       real*, int*, int*, hipStream_t);
+                                       real*, int*, int*, hipStream_t);                                       \
+  template void launch_mix_targets<real>(const Geo<real>&, const MixParams&, const real*, const real*,        \
+                                         const real*, const real*, float*, int, hipStream_t);
 
 ZM_COL_INSTANTIATE(float)
 ZM_COL_INSTANTIATE(double)

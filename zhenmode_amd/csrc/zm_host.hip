@@ -14,6 +14,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/zhenmode.h"
@@ -64,6 +65,13 @@ struct SolverBase {
   virtual void mlp_apply(const zm_state& in, const float* Y, int ldy, const float* omean, const float* ostd,
                          double dt, double kv, double kc, double tol_T, double tol_S, void* To, void* So, int* flags,
                          int* count, hipStream_t s) = 0;
+  virtual void mix_targets(const zm_state& in, const void* Tt, const void* St, double dt, double kv, double kc,
+                           float* Y, int ldy, hipStream_t s) = 0;
+  virtual void mlp_fused(const zm_state& in, const void* tau_x, const void* tau_y, const void* q_heat,
+                         const float* in_mean, const float* in_inv_std, const void* const* weights,
+                         const float* const* biases, const float* omean, const float* ostd, double dt, double kv,
+                         double kc, double tol_T, double tol_S, void* To, void* So, int* flags, int* count,
+                         hipStream_t s) = 0;
   virtual void seasonal(const void* stack, int nfields, double day, double blend_days, void* out,
                         hipStream_t s) = 0;
   virtual void monitor(const zm_state& st, long long step, double vlim, double elim, int inclusive, void* status,
@@ -98,7 +106,7 @@ struct Solver : SolverBase {
   int nu_block = 1;
   std::vector<void*> allocs;
   real *twr = nullptr, *twi = nullptr, *fdt = nullptr, *fiden = nullptr;
-  double *dzn64 = nullptr, *dzi64 = nullptr;
+  double *dzn64 = nullptr, *dzi64 = nullptr, *idzn64 = nullptr, *idzi64 = nullptr;
   double* vmix_scratch = nullptr;  // Thomas c', d' (3 n3), allocated on first use
   double* area64 = nullptr;        // cell area per latitude row (dx dy), float64
   double* red_partials = nullptr;  // monitor / budget block partials
@@ -314,6 +322,15 @@ struct Solver : SolverBase {
     ZM_HIP(hipMemcpy(dzn64, c.dz_node, nz * sizeof(double), hipMemcpyHostToDevice));
     dzi64 = dev_alloc<double>(nz > 1 ? nz - 1 : 1);
     if (nz > 1) ZM_HIP(hipMemcpy(dzi64, c.dz_iface, (nz - 1) * sizeof(double), hipMemcpyHostToDevice));
+    {
+      std::vector<double> rn(nz), ri(nz > 1 ? nz - 1 : 1, 0.0);
+      for (int k = 0; k < nz; ++k) rn[k] = 1.0 / c.dz_node[k];
+      for (int k = 0; k + 1 < nz; ++k) ri[k] = 1.0 / c.dz_iface[k];
+      idzn64 = dev_alloc<double>(nz);
+      idzi64 = dev_alloc<double>(ri.size());
+      ZM_HIP(hipMemcpy(idzn64, rn.data(), rn.size() * sizeof(double), hipMemcpyHostToDevice));
+      ZM_HIP(hipMemcpy(idzi64, ri.data(), ri.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     {
       std::vector<double> ar(ny);
       for (int j = 0; j < ny; ++j) ar[j] = c.dx_row[j] * c.dy;
@@ -966,6 +983,8 @@ struct Solver : SolverBase {
     m.S_ref = S_ref64;
     m.dzn = dzn64;
     m.dzi = dzi64;
+    m.idzn = idzn64;
+    m.idzi = idzi64;
     return m;
   }
 
@@ -997,6 +1016,29 @@ struct Solver : SolverBase {
     zm::launch_mlp_apply(g, mix(dt, kv, kc), (const real*)in.T, (const real*)in.S, Y, ldy, omean, ostd, tol_T, tol_S,
                          (real*)To, (real*)So, flags, count, s);
     ZM_HIP(hipGetLastError());
+  }
+
+  void mix_targets(const zm_state& in, const void* Tt, const void* St, double dt, double kv, double kc, float* Y,
+                   int ldy, hipStream_t s) override {
+    zm::launch_mix_targets(g, mix(dt, kv, kc), (const real*)in.T, (const real*)in.S, (const real*)Tt,
+                           (const real*)St, Y, ldy, s);
+    ZM_HIP(hipGetLastError());
+  }
+
+  void mlp_fused(const zm_state& in, const void* tau_x, const void* tau_y, const void* q_heat, const float* in_mean,
+                 const float* in_inv_std, const void* const* weights, const float* const* biases, const float* omean,
+                 const float* ostd, double dt, double kv, double kc, double tol_T, double tol_S, void* To, void* So,
+                 int* flags, int* count, hipStream_t s) override {
+    if constexpr (std::is_same_v<real, float>) {
+      zm::launch_mlp_fused(g, mix(dt, kv, kc), (const float*)in.T, (const float*)in.S, (const float*)in.u,
+                           (const float*)in.v, q_heat ? (const float*)q_heat : P.q_heat, P.s_ref2, P.restore_coef_S,
+                           tau_x ? (const float*)tau_x : P.tau_x, tau_y ? (const float*)tau_y : P.tau_y, in_mean,
+                           in_inv_std, weights, biases, omean, ostd, tol_T, tol_S, (float*)To, (float*)So, flags,
+                           count, s);
+      ZM_HIP(hipGetLastError());
+    } else {
+      throw ConfigError{"the fused surrogate step runs on float32 handles"};
+    }
   }
 
   double time_kernel(int which, int reps) override {
@@ -1388,6 +1430,35 @@ int zm_mlp_apply(zm_handle* h, const zm_state* in, const float* Y, int32_t ldy, 
   return guarded(h, [&] {
     h->solver->mlp_apply(*in, Y, ldy, out_mean, out_std, dt, kappa_v, kappa_conv, tol_T, tol_S, T_out, S_out,
                          flags, n_flagged, (hipStream_t)stream);
+  });
+}
+
+int zm_mix_targets(zm_handle* h, const zm_state* in, const void* T_thomas, const void* S_thomas, double dt,
+                   double kappa_v, double kappa_conv, float* Y, int32_t ldy, void* stream) {
+  if (!h || !in || !T_thomas || !S_thomas || !Y || !(dt >= 0.0) || !(kappa_v >= 0.0) || !(kappa_conv >= 0.0))
+    return fail(h, "invalid zm_mix_targets arguments", ZM_EINVAL);
+  if (ldy < 2 * h->solver->nz_) return fail(h, "ldy must hold 2 nz targets per column", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->mix_targets(*in, T_thomas, S_thomas, dt, kappa_v, kappa_conv, Y, ldy, (hipStream_t)stream);
+  });
+}
+
+int zm_mlp_mix(zm_handle* h, const zm_state* in, const void* tau_x, const void* tau_y, const void* q_heat,
+               const float* in_mean, const float* in_inv_std, const void* const* weights, const float* const* biases,
+               const float* out_mean, const float* out_std, double dt, double kappa_v, double kappa_conv,
+               double tol_T, double tol_S, void* T_out, void* S_out, int32_t* flags, int32_t* n_flagged,
+               void* stream) {
+  if (!h || !in || !in_mean || !in_inv_std || !weights || !biases || !out_mean || !out_std || !T_out || !S_out ||
+      !flags || !n_flagged || !(dt >= 0.0) || !(kappa_v >= 0.0) || !(kappa_conv >= 0.0))
+    return fail(h, "invalid zm_mlp_mix arguments", ZM_EINVAL);
+  for (int l = 0; l < 4; ++l)
+    if (!weights[l] || !biases[l]) return fail(h, "zm_mlp_mix needs the four layers' weights and biases", ZM_EINVAL);
+  if (h->dtype_bytes != 4) return fail(h, "zm_mlp_mix runs on float32 handles", ZM_EINVAL);
+  if (5 * h->solver->nz_ + 3 > 256) return fail(h, "the surrogate takes at most 50 levels", ZM_EINVAL);
+  if (T_out == in->T || S_out == in->S) return fail(h, "zm_mlp_mix output must not alias its input", ZM_EINVAL);
+  return guarded(h, [&] {
+    h->solver->mlp_fused(*in, tau_x, tau_y, q_heat, in_mean, in_inv_std, weights, biases, out_mean, out_std, dt,
+                         kappa_v, kappa_conv, tol_T, tol_S, T_out, S_out, flags, n_flagged, (hipStream_t)stream);
   });
 }
 

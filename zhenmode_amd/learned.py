@@ -3,14 +3,26 @@ mixing step, with its training loop, both on the project's own HIP kernels.
 
 The network maps a column's features (T, S, u, v and N^2 on 50 levels plus
 q_heat, the SSS-restoring salt flux and |tau|: 253 inputs, padded to 256) to
-its 50-level T and S mixing tendencies (100 outputs, padded to 128):
-253 -> 256 -> 256 -> 256 -> 100 with GELU, bfloat16 weights and activations,
-float32 accumulation on MFMA (``zm_gemm_bf16_nt``), float32 master weights
-updated by Adam (``zm_adam``).
+50-level T and S outputs (100 outputs, padded to 128): 253 -> 256 -> 256 ->
+256 -> 100 with GELU, bfloat16 weights and activations, float32 accumulation
+on MFMA, float32 master weights updated by Adam (``zm_adam``).
 
-``LearnedMixer.apply`` runs features -> 4 GEMMs -> ``zm_mlp_apply`` (exact
-column-integral correction, implicit-residual check) -> ``zm_vertical_mix`` on
-the flagged columns only (the Thomas fallback).  The reference has no learned
+What the network predicts: per column and tracer X, with e the explicit
+increment of the mixing operator over dt and s = max_k |e_k| the column's
+mixing scale, the increment is dX = e + s z, where z (the network's output,
+de-normalised per level) is the implicit solution's departure from the
+explicit step in units of s.  Stable columns have a mixing scale orders of
+magnitude below convectively unstable ones, so the network's error shrinks
+with the increment it corrects.  The column integral of dX is shifted to zero
+in float64 (heat and salt conserved exactly) and the implicit-system residual
+of every column is checked; columns above tolerance fall back to the Thomas
+solve.
+
+``LearnedMixer.apply`` runs, on float32 handles, one fused launch
+(``zm_mlp_mix``: features -> 4 MFMA layers -> apply, activations kept on
+chip), then ``zm_vertical_mix`` on the flagged columns only (the Thomas
+fallback).  ``apply_unfused`` (and float64 handles) run the same arithmetic as
+features -> 4 GEMMs -> ``zm_mlp_apply``.  The reference has no learned
 component; the mixing operator it replaces is the backward-Euler counterpart
 of the reference's flux-form vertical operator
 (``src/zhenmode/model/solver/numerics/vertical.py:296-332``).
@@ -170,25 +182,20 @@ class ColumnMLP:
         return loss
 
     # ---------------------------------------------------------- normalisation
-    def fit_normalisation(self, X_raw, dX):
-        """Per-feature mean / inverse std of raw features (float32 rows);
-        per-output mean and per-tracer std of the raw tendencies dX (rows x 2 nz)."""
+    def fit_normalisation(self, X_raw, targets):
+        """Per-feature mean / inverse std of raw features (float32 rows); per
+        output mean and std of the scaled targets (rows x 2 nz), with a floor
+        of 1e-6 so that levels whose target is always zero keep a negligible
+        output scale."""
         mu = X_raw.mean(dim=0)
         sd = X_raw.std(dim=0)
         inv = torch.where(sd > 1e-12, 1.0 / sd.clamp_min(1e-30), torch.zeros_like(sd))
         self.in_mean.copy_(torch.where(sd > 1e-12, mu, torch.zeros_like(mu)))
         self.in_inv_std.copy_(inv)
-        # one scale per tracer (not per level): the loss then weighs absolute
-        # errors of each tracer equally at every level, where they matter
-        nz = self.nz
-        om = dX.mean(dim=0)
-        osd = torch.empty_like(om)
-        osd[:nz] = dX[:, :nz].std().clamp_min(1e-12)
-        osd[nz:] = dX[:, nz:].std().clamp_min(1e-12)
         self.out_mean.zero_()
         self.out_std.fill_(1.0)
-        self.out_mean[:self.n_out] = om
-        self.out_std[:self.n_out] = osd
+        self.out_mean[:self.n_out] = targets.mean(dim=0)
+        self.out_std[:self.n_out] = targets.std(dim=0).clamp_min(1e-6)
 
     def state_dict(self):
         return {"w": [w.cpu() for w in self.w], "b": [b.cpu() for b in self.b],
@@ -232,36 +239,73 @@ class LearnedMixer:
         _lib.check(rc, s.handle)
         return X
 
-    def apply(self, state, dt, tau_x=None, tau_y=None, q_heat=None, out=None):
-        """Mixed state (T, S into ``out`` when given); self.count accumulates
-        the columns sent to the fallback."""
+    def _outputs(self, s, out):
+        from .solver import empty3
+        To = empty3(*s.shape3, s.dtype, s.device) if out is None else out[0]
+        So = empty3(*s.shape3, s.dtype, s.device) if out is None else out[1]
+        return To, So
+
+    def apply_unfused(self, state, dt, tau_x=None, tau_y=None, q_heat=None, out=None):
+        """Surrogate increments through features -> 4 GEMMs -> zm_mlp_apply
+        (the same arithmetic as the fused launch), then the Thomas fallback."""
         s = self.solver
         src = s.coerce(state)
         X = self.features(src, tau_x, tau_y, q_heat)
         Y, _ = self.mlp.forward(X, bufs=self.bufs)
-        from .solver import empty3
-        To = empty3(*s.shape3, s.dtype, s.device) if out is None else out[0]
-        So = empty3(*s.shape3, s.dtype, s.device) if out is None else out[1]
+        To, So = self._outputs(s, out)
         rc = s.lib.zm_mlp_apply(s.handle, ctypes.byref(s.zm_state(src)), _p(Y), N_OUT_PAD, _p(self.mlp.out_mean),
                                 _p(self.mlp.out_std), float(dt), self.kappa_v, self.kappa_conv, self.tol_T,
                                 self.tol_S, _p(To), _p(So), _p(self.flags), _p(self.count), s._stream())
         _lib.check(rc, s.handle)
-        # Thomas fallback on the flagged columns only
+        return self._fallback(src, To, So, dt)
+
+    def apply(self, state, dt, tau_x=None, tau_y=None, q_heat=None, out=None):
+        """Mixed state (T, S into ``out`` when given); self.count accumulates
+        the columns sent to the fallback.  Float32 handles take the fused
+        launch, float64 handles the unfused sequence."""
+        s = self.solver
+        if s.dtype != torch.float32:
+            return self.apply_unfused(state, dt, tau_x, tau_y, q_heat, out)
+        src = s.coerce(state)
+        To, So = self._outputs(s, out)
+        m = self.mlp
+        rc = s.lib.zm_mlp_mix(s.handle, ctypes.byref(s.zm_state(src)), _p(tau_x), _p(tau_y), _p(q_heat),
+                              _p(m.in_mean), _p(m.in_inv_std), self._wptrs, self._bptrs, _p(m.out_mean),
+                              _p(m.out_std), float(dt), self.kappa_v, self.kappa_conv, self.tol_T, self.tol_S,
+                              _p(To), _p(So), _p(self.flags), _p(self.count), s._stream())
+        _lib.check(rc, s.handle)
+        return self._fallback(src, To, So, dt)
+
+    def _fallback(self, src, To, So, dt):
+        """Thomas solve on the flagged columns only."""
+        s = self.solver
         rc = s.lib.zm_vertical_mix(s.handle, _p(src.T), _p(src.S), _p(To), _p(So), float(dt), self.kappa_v,
                                    self.kappa_conv, _p(self.flags), s._stream())
         _lib.check(rc, s.handle)
         return src._replace(T=To, S=So)
 
+    @property
+    def _wptrs(self):
+        return (ctypes.c_void_p * 4)(*(w.data_ptr() for w in self.mlp.w_bf))
+
+    @property
+    def _bptrs(self):
+        return (ctypes.c_void_p * 4)(*(b.data_ptr() for b in self.mlp.b))
+
 
 def mixing_targets(solver, state, dt, kappa_v, kappa_conv):
-    """Raw T/S mixing tendencies (rows x 2 nz, float32, row = j*nx + i) of the
-    Thomas step, for training."""
-    mixed = solver.vertical_mix(state, dt, kappa_v, kappa_conv)
+    """Training targets (rows x 2 nz, float32, row = j*nx + i) of the Thomas
+    step on ``state``: the Thomas solution's departure from the explicit
+    mixing step in units of the column's mixing scale (``zm_mix_targets``)."""
     src = solver.coerce(state)
+    mixed = solver.vertical_mix(src, dt, kappa_v, kappa_conv)
     nz = solver.shape3[2]
-    dT = ((mixed.T - src.T) / dt).permute(1, 0, 2).reshape(-1, nz)
-    dS = ((mixed.S - src.S) / dt).permute(1, 0, 2).reshape(-1, nz)
-    return torch.cat([dT, dS], dim=1).to(torch.float32)
+    n2 = solver.shape2[0] * solver.shape2[1]
+    Y = torch.empty((n2, 2 * nz), dtype=torch.float32, device=solver.device)
+    rc = solver.lib.zm_mix_targets(solver.handle, ctypes.byref(solver.zm_state(src)), _p(mixed.T), _p(mixed.S),
+                                   float(dt), float(kappa_v), float(kappa_conv), _p(Y), 2 * nz, solver._stream())
+    _lib.check(rc, solver.handle)
+    return Y
 
 
 class MixingStep:

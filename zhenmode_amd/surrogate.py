@@ -1,13 +1,15 @@
 """Training and evaluation of the learned vertical-mixing surrogate.
 
-    python -m zhenmode_amd.surrogate [--case g05deg] [--columns 2000000] [--epochs 8]
-                                     [--days 30] [--out surrogate.pt] [--report report.json]
+    python -m zhenmode_amd.surrogate [--train-case g05deg] [--case g045deg] [--columns 2000000]
+                                     [--epochs 8] [--days 30] [--out surrogate.pt] [--report report.json]
 
 1. Columns: the plain model with the implicit Thomas mixing (``MixingStep``,
    float32) is spun up from the case's initial state; every ``every`` steps
    the state that enters the mixing step is sampled: its wet columns' features
-   (``zm_column_features``) and the Thomas mixing tendencies over dt are the
-   training pairs.
+   (``zm_column_features``) and the Thomas solution's departure from the
+   explicit mixing step in units of the column's mixing scale
+   (``zm_mix_targets``) are the training pairs.  ``--train-case`` picks the
+   generator the columns come from (same 50 levels as the evaluation case).
 2. Training: minibatch Adam on the project's own BF16 MFMA forward/backward
    kernels (``ColumnMLP.train_step``).
 3. Evaluation (``--days``): the learned step in float32 and the Thomas step in
@@ -155,7 +157,9 @@ def evaluate(case, mlp, days, tols=((1e-4, 2e-5), (1e-3, 2e-4)), untrained=None,
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", default="g05deg")
+    ap.add_argument("--case", default="g045deg", help="evaluation case")
+    ap.add_argument("--train-case", default="g05deg", help="case whose plain runs generate the training columns")
+    ap.add_argument("--tols", default="1e-4,2e-5;1e-3,2e-4", help="(tol_T,tol_S) pairs of the learned variants")
     ap.add_argument("--columns", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4096)
@@ -168,8 +172,12 @@ def main():
     args = ap.parse_args()
     log = lambda m: print(m, flush=True)  # noqa: E731
     case = make_case(args.case)
+    tcase = make_case(args.train_case)
+    if tcase.grid.nz != case.grid.nz or not np.allclose(tcase.grid.z, case.grid.z):
+        raise SystemExit("the training and evaluation cases must share their levels")
+    tols = tuple(tuple(float(x) for x in pair.split(",")) for pair in args.tols.split(";"))
     t0 = time.perf_counter()
-    X, Y = generate_columns(case, args.columns, spinup=args.spinup, every=args.every, log=log)
+    X, Y = generate_columns(tcase, args.columns, spinup=args.spinup, every=args.every, log=log)
     t1 = time.perf_counter()
     mlp = ColumnMLP(case.grid.nz, device=X.device)
     Xn, Yn = normalise(mlp, X, Y)
@@ -177,7 +185,8 @@ def main():
     losses = train(mlp, Xn, Yn, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    report = {"case": args.case, "columns": int(Xn.shape[0]), "generate_s": t1 - t0, "train_s": t2 - t1,
+    report = {"case": args.case, "train_case": args.train_case, "columns": int(Xn.shape[0]),
+              "generate_s": t1 - t0, "train_s": t2 - t1,
               "epochs": args.epochs, "losses": losses}
     if args.out:
         torch.save(mlp.state_dict(), args.out)
@@ -185,7 +194,7 @@ def main():
         untrained = ColumnMLP(case.grid.nz, device=Xn.device, seed=1)
         for name in ("in_mean", "in_inv_std", "out_mean", "out_std"):
             getattr(untrained, name).copy_(getattr(mlp, name))
-        report["evaluation"] = evaluate(case, mlp, args.days, untrained=untrained, log=log)
+        report["evaluation"] = evaluate(case, mlp, args.days, tols=tols, untrained=untrained, log=log)
     log(json.dumps(report))
     if args.report:
         with open(args.report, "w") as f:
