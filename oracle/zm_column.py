@@ -249,10 +249,12 @@ def bf16_round(x):
 def column_features(T, S, u, v, p, q_heat, fw_flux, tau_mag):
     """(nx*ny, 256) raw features, columns in (j, i) order (i fastest).
 
-    Per level: T, S, u, v and N^2 at the level's lower interface
-    (g/rho0 * (rho'_{k+1} - rho'_k) / dz_iface[k], 0 at the last level and
-    below the seafloor); then q_heat, the freshwater flux and |tau|; then three
-    zero pads.  Dry levels contribute zeros.
+    Per level: the T and S profiles as differences across each level's lower
+    interface (X_k - X_{k+1}, 0 at and below the last wet level) with the
+    surface value X_0 in the last slot; u, v; N^2 at the level's lower
+    interface (g/rho0 * (rho'_{k+1} - rho'_k) / dz_iface[k], 0 at the last
+    level and below the seafloor); then q_heat, the freshwater flux and |tau|;
+    then three zero pads.  Dry levels contribute zeros.
     """
     nx, ny, nz = T.shape
     wm = p.wet_mask_z
@@ -260,7 +262,14 @@ def column_features(T, S, u, v, p, q_heat, fw_flux, tau_mag):
     n2 = np.zeros_like(T)
     wet_iface = (wm[..., :-1] > 0.5) & (wm[..., 1:] > 0.5)
     n2[..., :-1] = np.where(wet_iface, 9.81 / 1025.0 * (rho[..., 1:] - rho[..., :-1]) / p.dz_iface, 0.0)
-    blocks = [T * wm, S * wm, u * wm, v * wm, n2]
+
+    def profile(X):
+        out = np.zeros_like(X)
+        out[..., :-1] = np.where(wet_iface, X[..., :-1] - X[..., 1:], 0.0)
+        out[..., -1] = X[..., 0] * (wm[..., 0] > 0.5)
+        return out
+
+    blocks = [profile(T), profile(S), u * wm, v * wm, n2]
     X = np.zeros((ny, nx, N_FEATURES_PAD))
     for f, blk in enumerate(blocks):
         X[:, :, f * nz:(f + 1) * nz] = np.transpose(blk, (1, 0, 2))

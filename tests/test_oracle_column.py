@@ -169,8 +169,13 @@ def test_column_features_layout(land):
     i, j = 5, 7
     row = X[j * nx + i]
     wm = p.wet_mask_z[i, j]
-    np.testing.assert_array_equal(row[:nz], s.T[i, j] * wm)
-    np.testing.assert_array_equal(row[nz:2 * nz], s.S[i, j] * wm)
+    nw = int(wm.sum())
+    for q0, X in ((0, s.T[i, j]), (nz, s.S[i, j])):
+        # interface differences down to the last wet interface, then zeros; surface value last
+        for k in range(nz - 1):
+            assert row[q0 + k] == (X[k] - X[k + 1] if k + 1 < nw else 0.0)
+        assert row[q0 + nz - 1] == (X[0] if nw > 0 else 0.0)
+    np.testing.assert_array_equal(row[2 * nz:3 * nz], s.u[i, j] * wm)
     assert row[5 * nz] == q[i, j] and row[5 * nz + 1] == 2 * q[i, j] and row[5 * nz + 2] == 3 * q[i, j]
     assert np.all(row[5 * nz + 3:] == 0.0)
 

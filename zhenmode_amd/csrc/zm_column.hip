@@ -102,8 +102,10 @@ __global__ __launch_bounds__(256) void k_vmix(Geo<real> g, MixParams m, const re
 
 // ---------------------------------------------------------------------------
 // Feature rows of the surrogate.  A column's 256 features are built by four
-// threads (parts): T; S; u, v; N^2, surface fluxes and padding, each value
-// normalised as (x - mean[f]) * inv_std[f] and rounded to bfloat16.  The same
+// threads (parts): the T profile; the S profile (profile_feature: differences
+// across the interfaces and the surface value); u, v; N^2, surface fluxes and
+// padding; each value normalised as (x - mean[f]) * inv_std[f] and rounded to
+// bfloat16.  The same
 // routine feeds the standalone feature kernel and the fused surrogate kernel,
 // so both produce the same rows bit for bit.  Part 0 and 1 can also stage the
 // raw T and S of every level (stride `sld`) for a later apply phase.
@@ -116,6 +118,16 @@ __device__ __forceinline__ unsigned short feat_bits(float x, int f, const float*
                                                     const float* __restrict__ inv_std) {
   const __bf16 b = (__bf16)((x - mean[f]) * inv_std[f]);
   return __builtin_bit_cast(unsigned short, b);
+}
+
+// Tracer profile feature of slot k: the difference X_k - X_{k+1} across the
+// lower interface of level k (0 below the last wet interface), and in the last
+// slot the surface value X_0 (0 on land).  Differences keep the vertical
+// structure that mixing acts on at bfloat16 resolution (the absolute values,
+// tens of degrees, would leave 0.1 degC steps).
+__device__ __forceinline__ float profile_feature(float xk, float xk1, float x0, int k, int nz, int nw) {
+  if (k == nz - 1) return nw > 0 ? x0 : 0.f;
+  return k + 1 < nw ? xk - xk1 : 0.f;
 }
 
 // N^2 at the lower interface of level k from the (T, S) of levels k and k + 1.
@@ -136,17 +148,15 @@ __device__ __forceinline__ void feature_part(const Geo<real>& g, const MixParams
   const long long n2 = g.n2;
   auto put = [&](int f, float x) { row[f] = feat_bits(x, f, mean, inv_std); };
   const int nw = g.nwet[c2];
-  if (part == 0) {
+  if (part == 0 || part == 1) {
+    const real* X = part == 0 ? T : S;
+    real* st = part == 0 ? stT : stS;
+    const float x0 = (float)X[c2];
     for (int k = 0; k < nz; ++k) {
-      const real t = T[k * n2 + c2];
-      if (stT) stT[k * sld] = t;
-      put(k, k < nw ? (float)t : 0.f);
-    }
-  } else if (part == 1) {
-    for (int k = 0; k < nz; ++k) {
-      const real s = S[k * n2 + c2];
-      if (stS) stS[k * sld] = s;
-      put(nz + k, k < nw ? (float)s : 0.f);
+      const real x = X[k * n2 + c2];
+      const float x1 = k + 1 < nz ? (float)X[(k + 1) * n2 + c2] : 0.f;
+      if (st) st[k * sld] = x;
+      put(part * nz + k, profile_feature((float)x, x1, x0, k, nz, nw));
     }
   } else if (part == 2) {
     for (int k = 0; k < nz; ++k) put(2 * nz + k, k < nw ? (float)u[k * n2 + c2] : 0.f);
@@ -773,8 +783,6 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(Geo<float> g, MixParams m,
         const bool wet = k < nw;
         tsT[k * kFusedRows + l] = tv[i];
         tsS[k * kFusedRows + l] = sv[i];
-        row[k] = bits(wet ? tv[i] : 0.f, k);
-        row[nz + k] = bits(wet ? sv[i] : 0.f, nz + k);
         row[2 * nz + k] = bits(wet ? uv[i] : 0.f, 2 * nz + k);
         row[3 * nz + k] = bits(wet ? vv[i] : 0.f, 3 * nz + k);
       }
@@ -785,10 +793,13 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(Geo<float> g, MixParams m,
   __syncthreads();
   if (live) {
     for (int k = k0; k < nz; k += 4) {
+      const float Tk = tsT[k * kFusedRows + l], Sk = tsS[k * kFusedRows + l];
+      const float Tk1 = k + 1 < nz ? tsT[(k + 1) * kFusedRows + l] : 0.f;
+      const float Sk1 = k + 1 < nz ? tsS[(k + 1) * kFusedRows + l] : 0.f;
+      row[k] = bits(profile_feature(Tk, Tk1, tsT[l], k, nz, nw), k);
+      row[nz + k] = bits(profile_feature(Sk, Sk1, tsS[l], k, nz, nw), nz + k);
       float n2k = 0.f;
-      if (k + 1 < nw)
-        n2k = n2_value((double)tsT[k * kFusedRows + l], (double)tsS[k * kFusedRows + l],
-                       (double)tsT[(k + 1) * kFusedRows + l], (double)tsS[(k + 1) * kFusedRows + l], k, m);
+      if (k + 1 < nw) n2k = n2_value((double)Tk, (double)Sk, (double)Tk1, (double)Sk1, k, m);
       row[4 * nz + k] = bits(n2k, 4 * nz + k);
     }
     // surface fluxes and padding (features 5 nz .. 255)
