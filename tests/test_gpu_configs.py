@@ -12,8 +12,8 @@
   8 x 480x214x40): one ensemble step equals every member's own step bit for
   bit, and the 2-degree members' float32 steps match the oracle.
 * 0.45 degree, 800x356x50: the learned vertical-mixing step with a surrogate
-  trained here on the case's own columns, one simulated day against the
-  float64 Thomas-mixing step.
+  trained here on columns of the 0.5-degree generator, 30 simulated days
+  against the float64 Thomas-mixing step.
 
 Per-field errors are max |a - b| / max |b| over the whole field; the SST and
 SSS errors are RMSE over wet surface columns.  The float32-vs-float64
@@ -191,29 +191,48 @@ def test_ensemble_at_full_member_sizes():
             assert e <= tol[f], (i, f, e)
 
 
-def test_learned_mixing_at_0_45_degrees_over_one_day():
+def test_learned_mixing_at_0_45_degrees_over_30_days():
+    """The learned vertical-mixing step at 0.45 degrees (800 x 356 x 50) over 30
+    simulated days against the float64 Thomas-mixing step: the surrogate is
+    trained here on 6M columns of plain (Thomas-mixing) runs of the
+    0.5-degree generator; with a residual tolerance of 2e-3 degC / 4e-4 psu
+    at least 80 % of the columns must be accepted, the SST RMSE must stay
+    below 0.05 degC, and heat and salt must drift no more than in the float32
+    Thomas run (the column-integral correction is exact).  An untrained
+    network with the same normalisation is run beside it (what training
+    gains: the explicit increment and the per-level output normalisation do
+    much of the work at this tolerance), and the time per step of each
+    variant is reported beside the Thomas step's."""
     from zhenmode_amd.learned import ColumnMLP
     from zhenmode_amd.surrogate import evaluate, generate_columns, normalise, train
 
     case = make_case("g045deg")
     g = case.grid
-    print(f"case g045deg {g.nx}x{g.ny}x{g.nz}: surrogate trained on 400k columns, one day against the "
+    print(f"case g045deg {g.nx}x{g.ny}x{g.nz}: surrogate trained on 6M columns of g05deg, 30 days against the "
           "float64 Thomas-mixing step", flush=True)
-    X, Y = generate_columns(case, 400_000, spinup=8, every=8, log=lambda m: print("  " + m, flush=True))
+    X, Y = generate_columns(make_case("g05deg"), 6_000_000, spinup=48, every=8, log=lambda m: print("  " + m, flush=True))
     mlp = ColumnMLP(g.nz, device=X.device)
     Xn, Yn = normalise(mlp, X, Y)
     del X, Y
-    losses = train(mlp, Xn, Yn, epochs=3, log=lambda m: print("  " + m, flush=True))
+    losses = train(mlp, Xn, Yn, epochs=30, lr=2e-3, decay_every=8, log=lambda m: None)
+    print(f"  training loss {losses[0]:.3f} -> {losses[-1]:.3f}", flush=True)
     del Xn, Yn
-    rep = evaluate(case, mlp, 1.0, tols=((1e-3, 2e-4),), log=lambda m: print("  " + m, flush=True))
-    f32, learned = rep["thomas_f32"], rep["learned_tolT0.001"]
-    print(f"  float32 Thomas: SST rmse {f32['sst_rmse_degC']:.2e}; learned: SST rmse "
-          f"{learned['sst_rmse_degC']:.2e}, fallback {learned['fallback_fraction']:.3f}, heat diff "
-          f"{learned['heat_rel_diff']:.1e}, salt diff {learned['salt_rel_diff']:.1e}; ms/step "
+    untrained = ColumnMLP(g.nz, device=mlp.device, seed=1)
+    for name in ("in_mean", "in_inv_std", "out_mean", "out_std"):
+        getattr(untrained, name).copy_(getattr(mlp, name))
+    rep = evaluate(case, mlp, 30.0, tols=((2e-3, 4e-4),), untrained=untrained,
+                   log=lambda m: print("  " + m, flush=True))
+    f32, learned, raw = rep["thomas_f32"], rep["learned_tolT0.002"], rep["untrained_tolT0.002"]
+    print(f"  float32 Thomas: SST rmse {f32['sst_rmse_degC']:.4f}; learned: SST rmse {learned['sst_rmse_degC']:.4f}, "
+          f"accepted {1 - learned['fallback_fraction']:.3f}, heat diff {learned['heat_rel_diff']:.1e}, salt diff "
+          f"{learned['salt_rel_diff']:.1e}; untrained: SST rmse {raw['sst_rmse_degC']:.4f}, accepted "
+          f"{1 - raw['fallback_fraction']:.3f}; ms/step "
           + ", ".join(f"{k} {v:.2f}" for k, v in rep["ms_per_step"].items()), flush=True)
-    assert losses[-1] < losses[0]
+    assert losses[-1] < 0.7 * losses[0]
     assert learned["finite"] and f32["finite"]
-    # the column-integral correction keeps heat and salt to float32 rounding of the state
-    assert abs(learned["salt_rel_diff"]) <= 1e-6 and abs(learned["heat_rel_diff"]) <= 1e-5
-    # accuracy at the float32 floor: within 1.5x of the float32 Thomas step's SST RMSE against float64
-    assert learned["sst_rmse_degC"] <= 1.5 * f32["sst_rmse_degC"]
+    assert learned["sst_rmse_degC"] <= 0.05
+    assert learned["fallback_fraction"] <= 0.2
+    assert raw["fallback_fraction"] >= learned["fallback_fraction"]
+    # exact column-integral correction: no more drift than the float32 Thomas run itself
+    assert abs(learned["heat_rel_diff"]) <= 2 * abs(f32["heat_rel_diff"]) + 1e-6
+    assert abs(learned["salt_rel_diff"]) <= 2 * abs(f32["salt_rel_diff"]) + 1e-6

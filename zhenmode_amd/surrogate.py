@@ -66,15 +66,16 @@ def normalise(mlp, X_raw, dX):
     return Xn, Yn
 
 
-def train(mlp, Xn, Yn, epochs=8, batch=4096, lr=1e-3, seed=0, log=print):
-    """Minibatch Adam; returns the per-epoch mean losses."""
+def train(mlp, Xn, Yn, epochs=8, batch=4096, lr=1e-3, seed=0, decay_every=3, log=print):
+    """Minibatch Adam, learning rate halved every ``decay_every`` epochs;
+    returns the per-epoch mean losses."""
     n = (Xn.shape[0] // batch) * batch
     g = torch.Generator(device="cpu").manual_seed(seed)
     losses = []
     for ep in range(epochs):
         perm = torch.randperm(Xn.shape[0], generator=g)[:n].to(Xn.device)
         total = torch.zeros((), dtype=torch.float32, device=Xn.device)
-        step_lr = lr * (0.5 ** (ep // 3))
+        step_lr = lr * (0.5 ** (ep // decay_every))
         for b in range(0, n, batch):
             idx = perm[b:b + batch]
             total += mlp.train_step(Xn[idx].contiguous(), Yn[idx].contiguous(), lr=step_lr)[0]
@@ -164,10 +165,13 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--decay-every", type=int, default=3, help="epochs per halving of the learning rate")
     ap.add_argument("--days", type=float, default=0.0)
     ap.add_argument("--spinup", type=int, default=48)
     ap.add_argument("--every", type=int, default=4)
     ap.add_argument("--out", default="")
+    ap.add_argument("--load", default="", help="evaluate a network saved with --out instead of training one")
+    ap.add_argument("--no-untrained", action="store_true", help="leave out the untrained-network variant")
     ap.add_argument("--report", default="")
     args = ap.parse_args()
     log = lambda m: print(m, flush=True)  # noqa: E731
@@ -177,23 +181,32 @@ def main():
         raise SystemExit("the training and evaluation cases must share their levels")
     tols = tuple(tuple(float(x) for x in pair.split(",")) for pair in args.tols.split(";"))
     t0 = time.perf_counter()
-    X, Y = generate_columns(tcase, args.columns, spinup=args.spinup, every=args.every, log=log)
-    t1 = time.perf_counter()
-    mlp = ColumnMLP(case.grid.nz, device=X.device)
-    Xn, Yn = normalise(mlp, X, Y)
-    del X
-    losses = train(mlp, Xn, Yn, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    report = {"case": args.case, "train_case": args.train_case, "columns": int(Xn.shape[0]),
-              "generate_s": t1 - t0, "train_s": t2 - t1,
-              "epochs": args.epochs, "losses": losses}
+    if args.load:
+        mlp = ColumnMLP(case.grid.nz, device=torch.device("cuda", torch.cuda.current_device()))
+        mlp.load_state_dict(torch.load(args.load, weights_only=True))
+        report = {"case": args.case, "loaded": args.load}
+    else:
+        X, Y = generate_columns(tcase, args.columns, spinup=args.spinup, every=args.every, log=log)
+        t1 = time.perf_counter()
+        mlp = ColumnMLP(case.grid.nz, device=X.device)
+        Xn, Yn = normalise(mlp, X, Y)
+        del X
+        losses = train(mlp, Xn, Yn, epochs=args.epochs, batch=args.batch, lr=args.lr, decay_every=args.decay_every,
+                       log=log)
+        del Xn, Yn
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        report = {"case": args.case, "train_case": args.train_case, "columns": args.columns,
+                  "generate_s": t1 - t0, "train_s": t2 - t1,
+                  "epochs": args.epochs, "lr": args.lr, "decay_every": args.decay_every, "losses": losses}
     if args.out:
         torch.save(mlp.state_dict(), args.out)
     if args.days > 0:
-        untrained = ColumnMLP(case.grid.nz, device=Xn.device, seed=1)
-        for name in ("in_mean", "in_inv_std", "out_mean", "out_std"):
-            getattr(untrained, name).copy_(getattr(mlp, name))
+        untrained = None
+        if not args.no_untrained:
+            untrained = ColumnMLP(case.grid.nz, device=mlp.device, seed=1)
+            for name in ("in_mean", "in_inv_std", "out_mean", "out_std"):
+                getattr(untrained, name).copy_(getattr(mlp, name))
         report["evaluation"] = evaluate(case, mlp, args.days, tols=tols, untrained=untrained, log=log)
     log(json.dumps(report))
     if args.report:
