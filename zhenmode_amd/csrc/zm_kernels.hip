@@ -838,8 +838,63 @@ static inline unsigned segs3(const Geo<real>& g) {
                     ((g.nz + g.seg - 1) / g.seg));
 }
 
+// LDS halo tiles of the column kernels.  A workgroup of the 64 x 4 column
+// tile marches down the levels in step; per level it stages each stencil
+// field's tile with a one-cell halo ((64 + 2) x (4 + 2) cells) in LDS with
+// coalesced loads, periodic in i and edge-clamped in j (the cells make_nb's
+// neighbour indices name), so that each thread reads its four horizontal
+// neighbours from LDS instead of issuing four more loads.  The next level's
+// cells are loaded into registers while the current level computes and are
+// stored into the second of two LDS buffers (one barrier per level).
+constexpr int kHaloW = kTileI + 2, kHaloH = kTileJ + 2, kHaloN = kHaloW * kHaloH;
+constexpr int kHaloSlots = (kHaloN + 255) / 256;
+
+template <typename real, int NF>
+struct Halo {
+  int off[kHaloSlots];  // 2-D offsets of this thread's fill cells (-1: none)
+  real pre[NF][kHaloSlots];
+  __device__ Halo(const Geo<real>& g, int i0, int j0) {
+#pragma unroll
+    for (int s = 0; s < kHaloSlots; ++s) {
+      const int q = (int)threadIdx.x + 256 * s;
+      if (q < kHaloN) {
+        const int r = q / kHaloW, c = q - r * kHaloW;
+        int gj = j0 - 1 + r;
+        gj = gj < 0 ? 0 : (gj >= g.ny ? g.ny - 1 : gj);
+        int gi = i0 - 1 + c;
+        gi = gi < 0 ? gi + g.nx : (gi >= g.nx ? gi - g.nx : gi);
+        gi = gi >= g.nx ? gi % g.nx : gi;
+        off[s] = gj * g.nx + gi;
+      } else {
+        off[s] = -1;
+      }
+    }
+  }
+  // registers <- level plane `pl` of the NF fields (null fields are skipped)
+  __device__ __forceinline__ void load(const real* const (&f)[NF], long long pl) {
+#pragma unroll
+    for (int s = 0; s < kHaloSlots; ++s)
+#pragma unroll
+      for (int q = 0; q < NF; ++q) pre[q][s] = (off[s] >= 0 && f[q]) ? f[q][pl + off[s]] : real(0);
+  }
+  // LDS buffer <- registers; buffer layout [NF][kHaloN]
+  __device__ __forceinline__ void store(real* buf) const {
+#pragma unroll
+    for (int s = 0; s < kHaloSlots; ++s) {
+      const int q = (int)threadIdx.x + 256 * s;
+      if (q < kHaloN) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) buf[f * kHaloN + q] = pre[f][s];
+      }
+    }
+  }
+};
+
 // Remainder of _linear_half_step after the nu_h sub-cycle (step.py:111-173):
 // tracer D1, biharmonics, vertical diffusion, land/ghost hold, Coriolis rotation.
+// Lu0, Lv0, T0, S0, DT0 and DS0 are read at the four horizontal neighbours
+// through the LDS halo tiles; every value equals the one a global load would
+// give, so the arithmetic is unchanged.
 template <typename real>
 __global__ __launch_bounds__(256) void k_linear_finish(
     Geo<real> g, Phys<real> P, const real* __restrict__ u0, const real* __restrict__ v0,
@@ -847,51 +902,72 @@ __global__ __launch_bounds__(256) void k_linear_finish(
     const real* __restrict__ vnu, const real* __restrict__ Lu0, const real* __restrict__ Lv0,
     const real* __restrict__ DT0, const real* __restrict__ DS0, real* __restrict__ uo,
     real* __restrict__ vo, real* __restrict__ To, real* __restrict__ So) {
-  ZM_COLUMN_PROLOGUE
+  constexpr int NF = 6;
+  __shared__ real hl[2][NF * kHaloN];
+  ZM_COLUMN_TILE
+  // every thread of the tile takes part in the halo loads and barriers; the
+  // ones outside the grid work on a clamped column and store nothing
+  const bool active = i < g.nx && j < g.ny;
+  const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1;
+  const int k0 = seg_ * g.seg;
+  const int k1 = k0 + g.seg < g.nz ? k0 + g.seg : g.nz;
+  const Nb n = make_nb(g, ic, jc);
+  const int c2 = n.c2;
+  const long long n2 = g.n2;
+  const int kb = n.nwC > 0 ? n.nwC - 1 : 0;
   const real dth = P.dth;
-  const real idx = g.idx[j], idx2 = g.idx2[j], tr = g.tanR[j], idyc = g.idyc[j];
-  const real cfN = g.cfn[j], cfS = n.hasS ? g.cfn[j - 1] : real(0);
+  const real idx = g.idx[jc], idx2 = g.idx2[jc], tr = g.tanR[jc], idyc = g.idyc[jc];
+  const real cfN = g.cfn[jc], cfS = n.hasS ? g.cfn[jc - 1] : real(0);
   const real kc = P.kh2[c2];
   const real kfe = real(0.5) * (kc + P.kh2[n.e2]), kfw = real(0.5) * (P.kh2[n.w2] + kc);
   const real kfn = real(0.5) * (kc + P.kh2[n.n2e]), kfs = real(0.5) * (P.kh2[n.s2e] + kc);
   const real kv = P.kv2[c2];
-  const real cr = P.rot_c[j], sr = P.rot_s[j];
+  const real cr = P.rot_c[jc], sr = P.rot_s[jc];
   const bool bu = P.nu_bi > real(0), bt = P.kappa_bi > real(0);
+  const real* const fields[NF] = {bu ? Lu0 : nullptr, bu ? Lv0 : nullptr, T0, S0, bt ? DT0 : nullptr,
+                                  bt ? DS0 : nullptr};
+  Halo<real, NF> H(g, tx_ * kTileI, ty_ * kTileJ);
+  const int me = ((int)(threadIdx.x / kTileI) + 1) * kHaloW + (int)(threadIdx.x % kTileI) + 1;
+  H.load(fields, (long long)k0 * n2);
+  H.store(hl[0]);
+  __syncthreads();
   long long o = (long long)k0 * n2 + c2;
   real um = k0 > 0 ? u0[o - n2] : real(0), uc = u0[o];
   real vm = k0 > 0 ? v0[o - n2] : real(0), vc = v0[o];
   real Tm = k0 > 0 ? T0[o - n2] : real(0), Tc = T0[o];
   real Sm = k0 > 0 ? S0[o - n2] : real(0), Sc = S0[o];
   for (int k = k0; k < k1; ++k, o += n2) {
+    const real* L = hl[(k - k0) & 1];
+    if (k + 1 < k1) H.load(fields, (long long)(k + 1) * n2);
     const bool more = k + 1 < g.nz;
     const long long onext = more ? o + n2 : o;  // in bounds at the last level
     const real un_ = u0[onext], vn_ = v0[onext], Tn_ = T0[onext], Sn_ = S0[onext];
     const real up = more ? un_ : uc, vp = more ? vn_ : vc;
     const real Tp = more ? Tn_ : Tc, Sp = more ? Sn_ : Sc;
-    const long long pl = o - c2;
     const bool wc = k < n.nwC;
     const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
     const bool gnE = wc && k < n.nwN, gsE = wc && k < n.nwS;  // mirror ghost rows
     const bool gn = n.hasN && gnE, gs = n.hasS && gsE;        // closed walls
+    auto at = [&](int f, int d) { return L[f * kHaloN + me + d]; };
     real u = unu[o], v = vnu[o];
     if (bu) {
-      u = u - P.nu_bi * lap_vals(Lu0[o], Lu0[pl + n.e2], Lu0[pl + n.w2], Lu0[pl + n.n2e], Lu0[pl + n.s2e], ge, gw, gnE,
-                                 gsE, idx2, g.idy2, g.hidy, tr) * dth;
-      v = v - P.nu_bi * lap_vals(Lv0[o], Lv0[pl + n.e2], Lv0[pl + n.w2], Lv0[pl + n.n2e], Lv0[pl + n.s2e], ge, gw, gnE,
-                                 gsE, idx2, g.idy2, g.hidy, tr) * dth;
+      u = u - P.nu_bi * lap_vals(at(0, 0), at(0, 1), at(0, -1), at(0, kHaloW), at(0, -kHaloW), ge, gw, gnE, gsE, idx2,
+                                 g.idy2, g.hidy, tr) * dth;
+      v = v - P.nu_bi * lap_vals(at(1, 0), at(1, 1), at(1, -1), at(1, kHaloW), at(1, -kHaloW), ge, gw, gnE, gsE, idx2,
+                                 g.idy2, g.hidy, tr) * dth;
     }
     u = u + P.nu_v * d2z_win(um, uc, up, k, kb, g) * dth;
     v = v + P.nu_v * d2z_win(vm, vc, vp, k, kb, g) * dth;
-    real T = Tc + d1_vals(Tc, T0[pl + n.e2], T0[pl + n.w2], T0[pl + n.n2e], T0[pl + n.s2e], ge, gw, gn, gs, kfe, kfw,
-                          kfn, kfs, idx, g.idy, cfN, cfS, idyc) * dth;
-    real S = Sc + d1_vals(Sc, S0[pl + n.e2], S0[pl + n.w2], S0[pl + n.n2e], S0[pl + n.s2e], ge, gw, gn, gs, kfe, kfw,
-                          kfn, kfs, idx, g.idy, cfN, cfS, idyc) * dth;
+    real T = Tc + d1_vals(Tc, at(2, 1), at(2, -1), at(2, kHaloW), at(2, -kHaloW), ge, gw, gn, gs, kfe, kfw, kfn, kfs,
+                          idx, g.idy, cfN, cfS, idyc) * dth;
+    real S = Sc + d1_vals(Sc, at(3, 1), at(3, -1), at(3, kHaloW), at(3, -kHaloW), ge, gw, gn, gs, kfe, kfw, kfn, kfs,
+                          idx, g.idy, cfN, cfS, idyc) * dth;
     if (bt) {
       const real one = real(1);
-      T = T - P.kappa_bi * d1_vals(DT0[o], DT0[pl + n.e2], DT0[pl + n.w2], DT0[pl + n.n2e], DT0[pl + n.s2e], ge, gw,
-                                   gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
-      S = S - P.kappa_bi * d1_vals(DS0[o], DS0[pl + n.e2], DS0[pl + n.w2], DS0[pl + n.n2e], DS0[pl + n.s2e], ge, gw,
-                                   gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
+      T = T - P.kappa_bi * d1_vals(at(4, 0), at(4, 1), at(4, -1), at(4, kHaloW), at(4, -kHaloW), ge, gw, gn, gs, one,
+                                   one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
+      S = S - P.kappa_bi * d1_vals(at(5, 0), at(5, 1), at(5, -1), at(5, kHaloW), at(5, -kHaloW), ge, gw, gn, gs, one,
+                                   one, one, one, idx, g.idy, cfN, cfS, idyc) * dth;
     }
     T = T + vdiff_win(Tm, Tc, Tp, k, n.nwC, kb, kv, P.cons_kv != 0, g) * dth;
     S = S + vdiff_win(Sm, Sc, Sp, k, n.nwC, kb, kv, P.cons_kv != 0, g) * dth;
@@ -901,12 +977,16 @@ __global__ __launch_bounds__(256) void k_linear_finish(
       T = Tc;
       S = Sc;
     }
-    uo[o] = cr * u + sr * v;
-    vo[o] = -sr * u + cr * v;
-    To[o] = T;
-    So[o] = S;
+    if (active) {
+      uo[o] = cr * u + sr * v;
+      vo[o] = -sr * u + cr * v;
+      To[o] = T;
+      So[o] = S;
+    }
     um = uc; uc = up; vm = vc; vc = vp;
     Tm = Tc; Tc = Tp; Sm = Sc; Sc = Sp;
+    if (k + 1 < k1) H.store(hl[(k + 1 - k0) & 1]);
+    __syncthreads();
   }
 }
 
