@@ -764,9 +764,9 @@ struct Solver : SolverBase {
   void post_barotropic(const zm_state& in, const zm_state& out, const zm::Phys<real>& Pf, hipStream_t s) {
     zm::launch_cap_means(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, s);
     zm::launch_final(g, Pf, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, (const real*)in.u, (const real*)in.v,
-                     (const real*)in.T, (const real*)in.S, (real*)out.u, (real*)out.v, (real*)out.T, (real*)out.S, s);
-    zm::launch_copy((const real*)etaw, (real*)out.eta, n2, s);
-    if (out.ice != in.ice) zm::launch_copy((const real*)in.ice, (real*)out.ice, n2, s);
+                     (const real*)in.T, (const real*)in.S, (real*)out.u, (real*)out.v, (real*)out.T, (real*)out.S,
+                     (const real*)etaw, (real*)out.eta, (const real*)in.ice,
+                     out.ice != in.ice ? (real*)out.ice : (real*)nullptr, s);
   }
 
   void one_step(const zm_state& in, const zm_state& out, const zm::Phys<real>& Pf, hipStream_t s) {
@@ -946,8 +946,8 @@ struct Solver : SolverBase {
         barotropic(P, ei, s);
         zm::launch_cap_means(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, s);
         zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, ui, vi, Ti, Si, (real*)out.u,
-                         (real*)out.v, (real*)out.T, (real*)out.S, s);
-        zm::launch_copy((const real*)etaw, (real*)out.eta, n2, s);
+                         (real*)out.v, (real*)out.T, (real*)out.S, (const real*)etaw, (real*)out.eta,
+                         (const real*)nullptr, (real*)nullptr, s);
         break;
       case ZM_STAGE_TRACER_RES:
         zm::launch_fz(g, ui, vi, Fz1, s);
@@ -1085,7 +1085,8 @@ struct Solver : SolverBase {
           zm::launch_mom_adv(g, u1, v1, Fz1, au, av, s);
           break;
         case 12:
-          zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, u1, v1, T1, S1, tmpA, tmpB, u2, v2, s);
+          zm::launch_final(g, P, u3, v3, T3, S3, ubt, ubt0, vbt, vbt0, means, u1, v1, T1, S1, tmpA, tmpB, u2, v2,
+                           (const real*)etaw, etaw2, (const real*)nullptr, (real*)nullptr, s);
           break;
         case 4:
           zm::launch_bt_sub(g, P, etaw, ubt, vbt, etaw2, ubt2, vbt2, Frx, Fry, fdt, fiden, dtH, dt_bt, drag, s);

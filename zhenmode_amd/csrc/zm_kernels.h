@@ -151,7 +151,8 @@ template <typename real>
 void launch_final(const Geo<real>& g, const Phys<real>& P, const real* u3, const real* v3, const real* T3,
                   const real* S3, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
                   const real* means, const real* uin, const real* vin, const real* Tin, const real* Sin,
-                  real* uo, real* vo, real* To, real* So, hipStream_t s);
+                  real* uo, real* vo, real* To, real* So, const real* eta_src, real* eta_dst,
+                  const real* ice_src, real* ice_dst, hipStream_t s);
 template <typename real>
 void launch_diag(const Geo<real>& g, const Phys<real>& P, const real* T, const real* S, const real* eta,
                  const real* pbc, const real* Fz, real* rho, real* pres, real* w, hipStream_t s);

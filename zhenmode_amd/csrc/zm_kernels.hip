@@ -3035,9 +3035,15 @@ __global__ __launch_bounds__(256) void k_final(Geo<real> g, Phys<real> P, const 
                                                const real* __restrict__ uin, const real* __restrict__ vin,
                                                const real* __restrict__ Tin, const real* __restrict__ Sin,
                                                real* __restrict__ uo, real* __restrict__ vo,
-                                               real* __restrict__ To, real* __restrict__ So) {
+                                               real* __restrict__ To, real* __restrict__ So,
+                                               const real* __restrict__ eta_src, real* __restrict__ eta_dst,
+                                               const real* __restrict__ ice_src, real* __restrict__ ice_dst) {
   ZM_TILE_COORDS
   const int c2 = j * g.nx + i;
+  if (k == 0) {  // the step's 2-D outputs: eta of the last substep, ice unchanged
+    eta_dst[c2] = eta_src[c2];
+    if (ice_dst) ice_dst[c2] = ice_src[c2];
+  }
   const bool wc = k < g.nwet[c2];
   if (!wc) {  // land and ghost nodes hold their pre-step values
     uo[c] = uin[c];
@@ -3399,9 +3405,10 @@ template <typename real>
 void launch_final(const Geo<real>& g, const Phys<real>& P, const real* u3, const real* v3, const real* T3,
                   const real* S3, const real* ubt, const real* ubt0, const real* vbt, const real* vbt0,
                   const real* means, const real* uin, const real* vin, const real* Tin, const real* Sin,
-                  real* uo, real* vo, real* To, real* So, hipStream_t s) {
+                  real* uo, real* vo, real* To, real* So, const real* eta_src, real* eta_dst,
+                  const real* ice_src, real* ice_dst, hipStream_t s) {
   hipLaunchKernelGGL(k_final<real>, dim3(tiles3(g)), dim3(256), 0, s, g, P, u3, v3, T3, S3, ubt,
-                     ubt0, vbt, vbt0, means, uin, vin, Tin, Sin, uo, vo, To, So);
+                     ubt0, vbt, vbt0, means, uin, vin, Tin, Sin, uo, vo, To, So, eta_src, eta_dst, ice_src, ice_dst);
 }
 
 template <typename real>
@@ -3475,7 +3482,8 @@ void launch_copy(const real* a, real* b, long long n, hipStream_t s) {
   template void launch_final<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,           \
                                    const real*, const real*, const real*, const real*, const real*,         \
                                    const real*, const real*, const real*, const real*, const real*,         \
-                                   const real*, real*, real*, real*, real*, hipStream_t);                    \
+                                   const real*, real*, real*, real*, real*, const real*, real*, const real*, \
+                                   real*, hipStream_t);                                                      \
   template void launch_diag<real>(const Geo<real>&, const Phys<real>&, const real*, const real*,            \
                                   const real*, const real*, const real*, real*, real*, real*, hipStream_t); \
   template void launch_copy<real>(const real*, real*, long long, hipStream_t);
