@@ -282,6 +282,8 @@ struct Solver : SolverBase {
     g.seg = n2 >= 65536 ? nz : std::min(10, nz);
     g.nu_chunk = 28;  // rows per wave chunk of the streaming nu_h kernel (measured best at 0.5 degrees)
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
+    // independent stages as parallel branches of the step graph (1, default) or one chain (0)
+    if (const char* env = std::getenv("ZHENMODE_FORK_STAGES")) fork_stages = std::atoi(env) != 0;
     g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
     {
       // nu_h sub-cycle: padded row coefficients of the substep x + nu_h dts
