@@ -306,15 +306,12 @@ def projection_leg(case, plain_ms, tiles, steps, warmup, dtype):
             "finite": bool(torch.isfinite(a.T).all())}
 
 
-# Launch shapes (nu_h substeps per launch, column-kernel levels per thread)
-# per member size, as the on-device autotuner chose them for these sizes;
-# results do not depend on them.
-ENSEMBLE_TILES = {(180, 80, 30): (1, 4), (360, 160, 50): (4, 13), (480, 214, 40): (4, 10)}
-
-
 def ensemble_leg(steps, warmup):
     """24 uneven members (8 x 180x80x30, 8 x 360x160x50, 8 x 480x214x40) in one
-    hipGraph launch per ensemble step, fixed launch shapes per member size."""
+    hipGraph launch per ensemble step; the launch shapes of each member size
+    (nu_h substeps per launch, column-kernel levels per thread) are picked by
+    the on-device autotuner (zm_autotune, eagerly issued steps) on the first
+    member of that size and set on the others; results do not depend on them."""
     import torch
 
     from zhenmode_amd.cases import ensemble_members
@@ -326,8 +323,12 @@ def ensemble_leg(steps, warmup):
         step, init_state, _ = make_solver_global(m.grid, m.physics, m.dt, dtype="float32", **m.solver_kwargs)
         sols.append(step.solver)
         states.append(init_state(m.T_init, m.S_init))
-    for sol in sols:
-        sol.set_tiles(*ENSEMBLE_TILES[sol.shape3])
+    tiles = {}
+    for sol, st in zip(sols, states):
+        if sol.shape3 not in tiles:
+            nu, seg, ms = sol.autotune(st, reps=3)
+            tiles[sol.shape3] = (nu, seg, ms)
+        sol.set_tiles(*tiles[sol.shape3][:2])
     ens = Ensemble(sols)
     # two preallocated state sets stepped into each other, so that the timed
     # steps replay the two captured graphs (one per pointer set)
@@ -357,7 +358,8 @@ def ensemble_leg(steps, warmup):
             "member_sim_days_per_s": len(sols) * dt / 86400.0 / (ms * 1e-3),
             "aggregate_sypd": len(sols) * dt / (365.0 * ms * 1e-3),
             "ms_per_step_members_separately": ms_sep, "batching_speedup": ms_sep / ms,
-            "tiles": {f"{k[0]}x{k[1]}x{k[2]}": {"nu_block": v[0], "seg": v[1]} for k, v in ENSEMBLE_TILES.items()},
+            "tiles_autotuned": {f"{k[0]}x{k[1]}x{k[2]}": {"nu_block": v[0], "seg": v[1], "eager_ms": v[2]}
+                                for k, v in tiles.items()},
             "finite": finite}
 
 

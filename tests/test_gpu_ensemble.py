@@ -85,3 +85,19 @@ def test_autotuner_candidates_give_identical_steps():
         sol.set_tiles(0, 5)
     with pytest.raises(ValueError):
         sol.set_tiles(2, 13)
+
+
+def test_eager_autotuner_picks_a_shape_that_gives_identical_steps():
+    """zm_autotune times eagerly issued steps for every (nu_block, seg)
+    candidate (no graph capture) and leaves the fastest set; the step it
+    then takes equals the default shapes' step bit for bit."""
+    print("case eager autotune 90x40x12 float32", flush=True)
+    sol, s0 = member(90, 40, 12)
+    ref = sol.step(s0)
+    nu, seg, ms = sol.autotune(s0, reps=2)
+    print(f"  chose nu_block {nu}, seg {seg}: {ms:.3f} ms per eager step", flush=True)
+    assert 1 <= nu <= 4 and 1 <= seg <= 12 and ms > 0
+    assert sol.get_tiles() == (nu, seg)
+    out = sol.step(s0)
+    for f in FIELDS:
+        assert torch.equal(getattr(out, f), getattr(ref, f)), f

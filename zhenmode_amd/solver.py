@@ -244,8 +244,9 @@ class Solver:
         return a.value, b.value
 
     def autotune(self, sample, reps=5):
-        """Pick (nu_block, seg) by timing steps from ``sample``; returns (nu_block, seg, ms).
-        The native autotuner is switched off and raises ValueError; use ``set_tiles``."""
+        """Pick (nu_block, seg) by timing eagerly issued steps from ``sample``
+        for every candidate (no graph capture); leaves the fastest set and
+        returns (nu_block, seg, ms per eager step)."""
         src = self.coerce(sample)
         a, b, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
         torch.cuda.synchronize(self.device)
