@@ -147,6 +147,8 @@ struct Solver : SolverBase {
   // side by side on the ensemble's own streams and each member's stages stay
   // on the stream it is given
   bool fork_stages = true;
+  bool fork_linear = true;     // the tracer biharmonic's D1 beside the nu_h chain
+  bool fork_nonlinear = true;  // convection and momentum advection beside each other
   hipStream_t lane_a(hipStream_t s) const { return fork_stages ? sideA : s; }
   hipStream_t lane_b(hipStream_t s) const { return fork_stages ? sideB : s; }
   hipEvent_t ev_pool[kEvents] = {};
@@ -283,7 +285,13 @@ struct Solver : SolverBase {
     g.nu_chunk = 28;  // rows per wave chunk of the streaming nu_h kernel (measured best at 0.5 degrees)
     if (const char* env = std::getenv("ZHENMODE_NU_CHUNK")) g.nu_chunk = std::max(0, std::atoi(env));
     // independent stages as parallel branches of the step graph (1, default) or one chain (0)
-    if (const char* env = std::getenv("ZHENMODE_FORK_STAGES")) fork_stages = std::atoi(env) != 0;
+    if (const char* env = std::getenv("ZHENMODE_FORK_STAGES")) {
+      // 0: one chain; 1: both stages forked; 2: the nonlinear stage only; 3: the linear stages only
+      const int mode = std::atoi(env);
+      fork_stages = mode != 0;
+      fork_linear = mode == 1 || mode == 3;
+      fork_nonlinear = mode == 1 || mode == 2;
+    }
     g.nu_chunk = std::min(g.nu_chunk, zm::kNuChunkMax);
     {
       // nu_h sub-cycle: padded row coefficients of the substep x + nu_h dts
@@ -617,7 +625,7 @@ struct Solver : SolverBase {
               real* So, hipStream_t s) {
     const bool bih_u = P.nu_bi > real(0), bih_t = P.kappa_bi > real(0);
     // the inner D1 of the tracer biharmonic (memory-bound) beside the nu_h chain (VALU-bound)
-    const hipStream_t A = lane_a(s);
+    const hipStream_t A = fork_linear ? lane_a(s) : s;
     if (bih_t) {
       after(A, s);
       zm::launch_d1_unit(g, T0, S0, DT, DS, A);
@@ -655,7 +663,7 @@ struct Solver : SolverBase {
     // tracer residual #1 at the N-step start (u1, v1, T1, S1); the convective
     // gate and sub-cycle (side stream A) need only T1, S1, the momentum
     // advection and its de-aliasing (side stream B) only u1, v1 and Fz1
-    const hipStream_t A = lane_a(s), B = lane_b(s);
+    const hipStream_t A = fork_nonlinear ? lane_a(s) : s, B = fork_nonlinear ? lane_b(s) : s;
     after(A, s);
     zm::launch_conv(g, Pf, T1, S1, cT, cS, gate, ubits, A);
     zm::launch_fz(g, u1, v1, Fz1, s);
