@@ -395,12 +395,11 @@ int zm_barotropic_timeouts(zm_handle* h, int32_t* count);
 int zm_set_tiles(zm_handle* h, int32_t nu_block, int32_t seg);
 int zm_get_tiles(const zm_handle* h, int32_t* nu_block, int32_t* seg);
 
-/* Switched off: returns ZM_EINVAL and times nothing (an autotuning run of
- * about 40 graph captures in a row aborted the process without a message and
- * the cause is not identified); set launch shapes with zm_set_tiles.  When
- * enabled it times `reps` steps from a copy of *sample for every (nu_block,
- * seg) candidate on this device, keeps the fastest and reports it, with two
- * scratch states for the duration of the call; *sample is not modified. */
+/* Times `reps` eagerly issued steps (no graph capture) from a copy of *sample
+ * for every (nu_block, seg) candidate up to the handle's current nu_block on
+ * this device, keeps the fastest on the handle and reports it; two scratch
+ * states are allocated for the duration of the call; *sample is not modified.
+ * Results of later steps do not depend on the choice. */
 int zm_autotune(zm_handle* h, const zm_state* sample, int32_t reps, int32_t* nu_block, int32_t* seg,
                 double* best_ms);
 

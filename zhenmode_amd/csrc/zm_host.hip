@@ -1556,10 +1556,6 @@ int zm_get_tiles(const zm_handle* h, int32_t* nu_block, int32_t* seg) {
   return ZM_OK;
 }
 
-// Switched off: one autotuning run on an MI355X (about 40 graph captures of
-// the stage-forked step in a row on a fresh stream) ended in a process abort
-// that printed nothing, and its cause is not identified.  The entry point
-// reports that and times nothing; launch shapes are set with zm_set_tiles.
 // The autotuner times eagerly issued steps (no graph capture: an earlier
 // version that captured a step graph for every candidate on a fresh stream
 // aborted once without a message, cause not identified).
@@ -1635,6 +1631,7 @@ int zm_autotune(zm_handle* h, const zm_state* sample, int32_t reps, int32_t* nu_
       }
     } catch (...) {
       sv->use_graph = graph;
+      sv->set_tiles(nu_max, cur_seg);
       hipEventDestroy(e0);
       hipEventDestroy(e1);
       hipStreamDestroy(s);
