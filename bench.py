@@ -25,6 +25,8 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "SYPD at 0.5deg x 50L on 1 GPU: learned+mixed vs plain FP32; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS = 1024  # 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9
 
 
 def algorithmic_step_bytes(nx, ny, nz, n_nu, n_sub, word):
@@ -475,6 +477,11 @@ def main():
     step_bytes = algorithmic_step_bytes(g.nx, g.ny, g.nz, params.n_nu, params.n_subcyc, word)
     step_traffic = (read_profile_value(f"step_{args.case}_hbm_traffic.md", "traffic_bytes_per_step")
                     if profiled else None)
+    # vector issue: the launch's VALU wave64 instructions (SQ_INSTS_VALU) at one
+    # instruction per SIMD every 4 cycles, 1024 SIMDs at 2.4 GHz
+    valu = (read_profile_value(f"step_{args.case}_sq_counters.md", "valu_instructions_per_launch")
+            if profiled and nu_h == 4 else None)
+    valu_ms = valu * 4 / (VALU_SIMDS * VALU_CLOCK_HZ) * 1e3 if valu else None
     sypd = case.dt / (365.0 * ms_per_step * 1e-3)
     out = {
         "metric": METRIC, "value": sypd, "unit": "SYPD", "n_gpus": 1, "steps": args.steps,
@@ -498,7 +505,11 @@ def main():
                      "equivalent_unfused_GBps": nu_h * nu_bytes / (nu_ms * 1e-3) / 1e9,
                      "single_substep_launch_ms": nu_single_ms,
                      "single_substep_GBps": nu_bytes / (nu_single_ms * 1e-3) / 1e9,
-                     "copy_1GiB_GBps": copy_gbps},
+                     "copy_1GiB_GBps": copy_gbps,
+                     "valu_issue": ({"instructions_per_launch": valu, "issue_ms": valu_ms,
+                                     "frac": valu_ms / nu_ms,
+                                     "rate": "one wave64 VALU instruction per SIMD every 4 cycles, "
+                                             "1024 SIMDs at 2.4 GHz"} if valu else None)},
         "step_roofline": {"algorithmic_bytes": step_bytes,
                           "algorithmic_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "algorithmic_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
