@@ -730,24 +730,6 @@ __device__ __forceinline__ real d1_vals(real c, real e, real w, real nn, real s,
   return (zE - zW) * idx + (mN - mS) * idyc;
 }
 
-template <typename real>
-__global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __restrict__ T,
-                                                 const real* __restrict__ S, real* __restrict__ dT,
-                                                 real* __restrict__ dS) {
-  ZM_CELL_PROLOGUE
-  // d1_flux with unit coefficients, neighbours loaded unconditionally
-  const bool wc = k < n.nwC;
-  const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
-  const bool gn = n.hasN && wc && k < n.nwN, gs = n.hasS && wc && k < n.nwS;
-  const real one = real(1);
-  const real idx = g.idx[n.j], idyc = g.idyc[n.j], cfN = g.cfn[n.j], cfS = n.hasS ? g.cfn[n.j - 1] : real(0);
-  dT[c] = d1_vals(T[plane + n.c2], T[plane + n.e2], T[plane + n.w2], T[plane + n.n2e], T[plane + n.s2e], ge, gw,
-                  gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc);
-  dS[c] = d1_vals(S[plane + n.c2], S[plane + n.e2], S[plane + n.w2], S[plane + n.n2e], S[plane + n.s2e], ge, gw,
-                  gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc);
-}
-
-
 // ---------------------------------------------------------------------------
 // Column-marching form.  A thread owns one (i, j) column over a segment of
 // g.seg levels; vertical neighbours come from a register window, the column's
@@ -780,6 +762,36 @@ __global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __rest
   ZM_COLUMN_TILE                                                             \
   if (i >= g.nx || j >= g.ny) return;                                        \
   ZM_COLUMN_SEGMENT
+
+// Unit-coefficient D1 of T and S (the inner D1 of _horizontal_biharmonic_tracer,
+// horizontal.py:240-246), one thread per column marching down its segment;
+// the next level's ten neighbour values are loaded before the current level
+// is combined (d1_vals: the arithmetic of d1_flux with unit coefficients).
+template <typename real>
+__global__ __launch_bounds__(256) void k_d1_unit(Geo<real> g, const real* __restrict__ T,
+                                                 const real* __restrict__ S, real* __restrict__ dT,
+                                                 real* __restrict__ dS) {
+  ZM_COLUMN_PROLOGUE
+  const real one = real(1);
+  const real idx = g.idx[j], idyc = g.idyc[j], cfN = g.cfn[j], cfS = n.hasS ? g.cfn[j - 1] : real(0);
+  const long long dE = n.e2 - c2, dW = n.w2 - c2, dN = n.n2e - c2, dS2 = n.s2e - c2;
+  long long o = (long long)k0 * n2 + c2;
+  real tc = T[o], te = T[o + dE], tw = T[o + dW], tn = T[o + dN], ts = T[o + dS2];
+  real sc = S[o], se = S[o + dE], sw = S[o + dW], sn = S[o + dN], ss = S[o + dS2];
+  for (int k = k0; k < k1; ++k, o += n2) {
+    // the next level's values (the last level reloads itself: in bounds)
+    const long long on = k + 1 < k1 ? o + n2 : o;
+    const real tc1 = T[on], te1 = T[on + dE], tw1 = T[on + dW], tn1 = T[on + dN], ts1 = T[on + dS2];
+    const real sc1 = S[on], se1 = S[on + dE], sw1 = S[on + dW], sn1 = S[on + dN], ss1 = S[on + dS2];
+    const bool wc = k < n.nwC;
+    const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
+    const bool gn = n.hasN && wc && k < n.nwN, gs = n.hasS && wc && k < n.nwS;
+    dT[o] = d1_vals(tc, te, tw, tn, ts, ge, gw, gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc);
+    dS[o] = d1_vals(sc, se, sw, sn, ss, ge, gw, gn, gs, one, one, one, one, idx, g.idy, cfN, cfS, idyc);
+    tc = tc1; te = te1; tw = tw1; tn = tn1; ts = ts1;
+    sc = sc1; se = se1; sw = sw1; sn = sn1; ss = ss1;
+  }
+}
 
 template <typename real>
 __device__ __forceinline__ real lap_vals(real c, real e, real w, real nn, real s, bool ge, bool gw, bool gn,
@@ -3175,7 +3187,7 @@ void launch_nu_block(const Geo<real>& g, const real* u, const real* v, real* uo,
 
 template <typename real>
 void launch_d1_unit(const Geo<real>& g, const real* T, const real* S, real* dT, real* dS, hipStream_t s) {
-  hipLaunchKernelGGL(k_d1_unit<real>, dim3(tiles3(g)), dim3(256), 0, s, g, T, S, dT, dS);
+  hipLaunchKernelGGL(k_d1_unit<real>, dim3(segs3(g)), dim3(256), 0, s, g, T, S, dT, dS);
 }
 
 template <typename real>
