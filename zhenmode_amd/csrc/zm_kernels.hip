@@ -1432,27 +1432,50 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
   real Tm = k0 > 0 ? T[o - n2] : real(0), Tc = T[o];
   real Sm = k0 > 0 ? S[o - n2] : real(0), Sc = S[o];
   real fz0 = Fz[o];
+  // the horizontal loads of a level (all in bounds) are issued one level
+  // ahead of their use; the last level of a segment reloads itself
+  struct Lv {
+    real TE, TW, TN, TS, SE, SW, SN, SS, uc, vc, uE, uW, vN, vS, cT, cS;
+    real TEE, TWW, TNN, TSS, SEE, SWW, SNN, SSS, Tb, Sb, dT1, dS1;
+  };
+  auto load = [&](long long oo, Lv& L) {
+    const long long pl = oo - c2;
+    L.TE = T[pl + n.e2], L.TW = T[pl + n.w2], L.TN = T[pl + n.n2e], L.TS = T[pl + n.s2e];
+    L.SE = S[pl + n.e2], L.SW = S[pl + n.w2], L.SN = S[pl + n.n2e], L.SS = S[pl + n.s2e];
+    L.uc = u[oo], L.vc = v[oo];
+    L.uE = u[pl + n.e2], L.uW = u[pl + n.w2], L.vN = v[pl + n.n2e], L.vS = v[pl + n.s2e];
+    L.cT = cT[oo], L.cS = cS[oo];
+    if (ADV == 2) {
+      L.TEE = T[pl + ee2], L.TWW = T[pl + ww2], L.TNN = T[pl + nn2], L.TSS = T[pl + ss2];
+      L.SEE = S[pl + ee2], L.SWW = S[pl + ww2], L.SNN = S[pl + nn2], L.SSS = S[pl + ss2];
+    }
+    if (!TERMS && !LED && mode != 0) {  // the second stage's inputs (never written by it)
+      L.Tb = Tb[oo], L.Sb = Sb[oo], L.dT1 = dT1[oo], L.dS1 = dS1[oo];
+    }
+  };
+  Lv cur, nxt;
+  load(o, cur);
   for (int k = k0; k < k1; ++k, o += n2) {
     const bool more = k + 1 < g.nz;
     const long long onext = more ? o + n2 : o;  // in bounds at the last level
     const real Tn_ = T[onext], Sn_ = S[onext];
     const real Tp = more ? Tn_ : Tc, Sp = more ? Sn_ : Sc;
     const real fz1 = Fz[o + n2];
-    const long long pl = o - c2;
+    load(k + 1 < k1 ? o + n2 : o, nxt);
     const bool wc = k < n.nwC;
     const bool ge = wc && k < n.nwE, gw = wc && k < n.nwW;
     const bool gn = n.hasN && wc && k < n.nwN, gs = n.hasS && wc && k < n.nwS;
-    const real TE = T[pl + n.e2], TW = T[pl + n.w2], TN = T[pl + n.n2e], TS = T[pl + n.s2e];
-    const real SE = S[pl + n.e2], SW = S[pl + n.w2], SN = S[pl + n.n2e], SS = S[pl + n.s2e];
+    const real TE = cur.TE, TW = cur.TW, TN = cur.TN, TS = cur.TS;
+    const real SE = cur.SE, SW = cur.SW, SN = cur.SN, SS = cur.SS;
     const real dhT = d1_vals(Tc, TE, TW, TN, TS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
     const real dhS = d1_vals(Sc, SE, SW, SN, SS, ge, gw, gn, gs, kfe, kfw, kfn, kfs, idx, g.idy, cfN, cfS, idyc);
     const real dvT = vdiff_win(Tm, Tc, Tp, k, n.nwC, kb, kv, P.cons_kv != 0, g);
     const real dvS = vdiff_win(Sm, Sc, Sp, k, n.nwC, kb, kv, P.cons_kv != 0, g);
     // the velocity and convection loads of the level are in bounds: issued
     // unconditionally, used on wet nodes
-    const real uc = u[o], vc = v[o];
-    const real uE = u[pl + n.e2], uW = u[pl + n.w2], vN = v[pl + n.n2e], vS = v[pl + n.s2e];
-    const real cTo = cT[o], cSo = cS[o];
+    const real uc = cur.uc, vc = cur.vc;
+    const real uE = cur.uE, uW = cur.uW, vN = cur.vN, vS = cur.vS;
+    const real cTo = cur.cT, cSo = cur.cS;
     real dTdt = 0, dSdt = 0;
     real term_adv = 0, term_conv = 0;
     if (wc) {
@@ -1475,8 +1498,8 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
       } else {  // minmod-limited upwind reconstruction (fct_adv)
         const bool wC = true, wE = k < n.nwE, wW = k < n.nwW, wN = k < n.nwN, wS = k < n.nwS;
         const bool wEE = k < nwEE, wWW = k < nwWW, wNN = k < nwNN, wSS = k < nwSS;
-        const real TEE = T[pl + ee2], TWW = T[pl + ww2], TNN = T[pl + nn2], TSS = T[pl + ss2];
-        const real SEE = S[pl + ee2], SWW = S[pl + ww2], SNN = S[pl + nn2], SSS_ = S[pl + ss2];
+        const real TEE = cur.TEE, TWW = cur.TWW, TNN = cur.TNN, TSS = cur.TSS;
+        const real SEE = cur.SEE, SWW = cur.SWW, SNN = cur.SNN, SSS_ = cur.SSS;
         const real tsx = minmod_slope(TW, Tc, TE, wW, wC, wE), tsE = minmod_slope(Tc, TE, TEE, wC, wE, wEE);
         const real tsW = minmod_slope(TWW, TW, Tc, wWW, wW, wC);
         const real tsy = minmod_slope(TS, Tc, TN, wS, wC, wN), tsN = minmod_slope(Tc, TN, TNN, wC, wN, wNN);
@@ -1568,12 +1591,13 @@ __global__ __launch_bounds__(256) void k_tracer_residual(
         To[o] = Tc + dTdt * P.dt;
         So[o] = Sc + dSdt * P.dt;
       } else {
-        To[o] = Tb[o] + real(0.5) * (dT1[o] + dTdt) * P.dt;
-        So[o] = Sb[o] + real(0.5) * (dS1[o] + dSdt) * P.dt;
+        To[o] = cur.Tb + real(0.5) * (cur.dT1 + dTdt) * P.dt;
+        So[o] = cur.Sb + real(0.5) * (cur.dS1 + dSdt) * P.dt;
       }
     }
     Tm = Tc; Tc = Tp; Sm = Sc; Sc = Sp;
     fz0 = fz1;
+    cur = nxt;
   }
   }
   if constexpr (LED) {
