@@ -465,6 +465,12 @@ __device__ __forceinline__ void buf_st(double x, __amdgpu_buffer_rsrc_t r, unsig
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64, x), r, voff, soff, 0);
 }
 
+// The bits of |x| (zero exactly when x is +0 or -0).
+__device__ __forceinline__ unsigned long long magnitude_bits(float x) { return __float_as_uint(x) & 0x7fffffffu; }
+__device__ __forceinline__ unsigned long long magnitude_bits(double x) {
+  return (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull;
+}
+
 // Buffer descriptors of the streaming kernel's input and output level planes.
 struct NuBufs {
   __amdgpu_buffer_rsrc_t u, v, uo, vo, lu, lv;
@@ -636,6 +642,34 @@ __global__ __launch_bounds__(256) void k_nu_stream(Geo<real> g, const real* __re
   const bool store_lane = lane >= NS && lane < 64 - NS && io < g.nx;
   // 32-bit byte offsets of this lane's column (row bases stay uniform)
   const unsigned ib = (unsigned)i * (unsigned)sizeof(real), iob = store_lane ? (unsigned)io * (unsigned)sizeof(real) : 0u;
+  if constexpr (!LU) {
+    // A wave whose whole input tile (64 columns, rows [qs, qe)) is land
+    // first checks whether every input value is +-0 (land velocities are
+    // held at rest away from the coasts).  Then every substep of every node
+    // gives +0 whatever its face gates: each difference of zeros is +0, so
+    // each product, fused multiply-add and final sum is +0 (a -0 centre
+    // value plus +0 is +0), and the wave stores +0 without the march.
+    const unsigned long long Wt = g.nu_wet[((long long)k * nch + ch) * g.nx + i];
+    const int b0 = qs - wb, nb = qe - qs;  // the input rows' bits of the wet word (b0 + nb <= 64)
+    const unsigned long long rows = (nb >= 64 ? ~0ull : (1ull << nb) - 1ull) << b0;
+    if (__ballot((Wt & rows) != 0ull) == 0ull) {  // uniform
+      unsigned long long acc = 0ull;
+#pragma unroll 8
+      for (int r = qs; r < qe; ++r) {
+        const int ro = r * g.nx * (int)sizeof(real);
+        acc |= magnitude_bits(buf_ld((real*)nullptr, B.u, ib, ro)) | magnitude_bits(buf_ld((real*)nullptr, B.v, ib, ro));
+      }
+      if (__ballot(acc != 0ull) == 0ull) {  // uniform
+        if (store_lane)
+          for (int r = jb; r < je; ++r) {
+            const int ro = r * g.nx * (int)sizeof(real);
+            buf_st(real(0), B.uo, iob, ro);
+            buf_st(real(0), B.vo, iob, ro);
+          }
+        return;
+      }
+    }
+  }
   NuStream<real, NS> st;
 #pragma unroll
   for (int l = 0; l < NS; ++l) st.w[l] = NuWin<real>{real(0), real(0), real(0), real(0), real(0), real(0)};
