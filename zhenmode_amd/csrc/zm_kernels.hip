@@ -181,6 +181,23 @@ constexpr int kColAhead = 6;
   const long long plane = (long long)k * g.n2;                              \
   const long long c = plane + (long long)j * g.nx + i;
 
+// The same cells with the level fastest: each XCD's contiguous tile range is
+// a band of (i, j) tiles over all their levels, so a per-cell kernel that
+// reads 2-D column data keeps that band's share in its L2 instead of
+// streaming every 2-D array once per level.
+#define ZM_TILE_COORDS_LEVEL_FASTEST                                        \
+  const int ntx_ = (g.nx + kTileI - 1) / kTileI;                            \
+  long long t_ = xcd_tile(blockIdx.x, gridDim.x);                           \
+  const int k = (int)(t_ % g.nz);                                           \
+  t_ /= g.nz;                                                               \
+  const int tx_ = (int)(t_ % ntx_);                                         \
+  const int ty_ = (int)(t_ / ntx_);                                         \
+  const int i = tx_ * kTileI + (int)(threadIdx.x % kTileI);                 \
+  const int j = ty_ * kTileJ + (int)(threadIdx.x / kTileI);                 \
+  if (i >= g.nx || j >= g.ny) return;                                       \
+  const long long plane = (long long)k * g.n2;                              \
+  const long long c = plane + (long long)j * g.nx + i;
+
 #define ZM_CELL_PROLOGUE \
   ZM_TILE_COORDS         \
   const Nb n = make_nb(g, i, j);
@@ -3108,7 +3125,7 @@ __global__ __launch_bounds__(256) void k_final(Geo<real> g, Phys<real> P, const 
                                                real* __restrict__ To, real* __restrict__ So,
                                                const real* __restrict__ eta_src, real* __restrict__ eta_dst,
                                                const real* __restrict__ ice_src, real* __restrict__ ice_dst) {
-  ZM_TILE_COORDS
+  ZM_TILE_COORDS_LEVEL_FASTEST
   const int c2 = j * g.nx + i;
   if (k == 0) {  // the step's 2-D outputs: eta of the last substep, ice unchanged
     eta_dst[c2] = eta_src[c2];
