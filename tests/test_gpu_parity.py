@@ -169,6 +169,35 @@ def test_blocked_nu_subcycle_equals_single_substeps(dtype, block, chunk):
         np.testing.assert_array_equal(getattr(outs[1], f), getattr(outs[0], f), err_msg=f)
 
 
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_blocked_nu_subcycle_with_land_velocities_off_rest(dtype):
+    """Velocities on land need not be at rest (a restart may hold any values):
+    with random u, v on every node of the southern half and zero in the
+    northern half, the streaming kernel's all-land tiles take both paths
+    (nonzero inputs: the march; all-zero inputs: the +0 shortcut), and one
+    step still gives the single-substep bits."""
+    print(f"case g4deg nu blocking, land velocities off rest, {dtype}", flush=True)
+    case = make_case("g4deg")
+    outs = []
+    for single in (True, False):
+        if single:
+            os.environ["ZHENMODE_NU_BLOCK"] = "1"
+        try:
+            step, init_state, _ = make_solver_global(case.grid, case.physics, case.dt, dtype=dtype,
+                                                     **case.solver_kwargs)
+        finally:
+            os.environ.pop("ZHENMODE_NU_BLOCK", None)
+        s = init_state(case.T_init, case.S_init)
+        gen = torch.Generator().manual_seed(7)
+        for f in (s.u, s.v):  # (nx, ny, nz) views
+            x = 0.05 * torch.randn(f.shape, generator=gen, dtype=torch.float64)
+            x[:, f.shape[1] // 2:, :] = 0.0
+            f.copy_(x.to(f.dtype))
+        outs.append(to_numpy_state(step(s)))
+    for f in FIELDS:
+        np.testing.assert_array_equal(getattr(outs[1], f), getattr(outs[0], f), err_msg=f)
+
+
 def test_graph_replays_match_eager_steps():
     print("case g4deg graph replay float32", flush=True)
     case, step, init_state, _, _ = g4deg_solver("float32")
